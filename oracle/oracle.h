@@ -1,0 +1,133 @@
+/*
+ * CPU oracle for the pairwise matching path of skylens-stream
+ * (crates/core/src/matching.rs). TEST INFRASTRUCTURE ONLY: this library is
+ * the checker that the HIP path is compared against. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path (skylens-stream-rs_amd/) never links or calls it.
+ *
+ * Every function restates the reference statement by statement and cites
+ * the file:line it follows. Build flags: -O2 -ffp-contract=off, no fast-math
+ * (Rust never contracts a*b+c into an FMA).
+ */
+#ifndef SKYLENS_ORACLE_H
+#define SKYLENS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_DESC_LEN 128
+#define ORC_MATCH_BLOCK 64
+
+/* ---- deterministic test generators (matching.rs:1141-1154) ---- */
+typedef struct { uint64_t s; } orc_lcg;
+double orc_lcg_next(orc_lcg* g);
+double orc_lcg_gauss(orc_lcg* g);
+
+/* feature(): 128 uniforms, L2-normalised in f32 (matching.rs:1580-1595). */
+void orc_feature(orc_lcg* g, float* desc);
+/* match_fixture(seed, na, extra) (matching.rs:1638-1656). b_cap rows of
+ * storage must be provided; returns the row count of b (-1 if b_cap short). */
+long orc_match_fixture(uint64_t seed, size_t na, size_t extra, float* a, float* b, size_t b_cap);
+size_t orc_match_fixture_nb(size_t na, size_t extra);
+
+/* ---- descriptor matching ---- */
+float orc_sq_dist(const float* p, const float* r);            /* matching.rs:144-155 */
+float orc_sq_dist_spec(const float* p, const float* r);       /* matching.rs:2345-2357 */
+float orc_sq_dist_seq(const float* p, const float* r);        /* matching.rs:2372-2376 */
+/* ratio_match (matching.rs:157-215). out_ij holds 2*cap uint64 (i,j) pairs,
+ * returns the match count (always <= na). */
+size_t orc_ratio_match(const float* a, size_t na, const float* b, size_t nb, float ratio,
+                       int mutual, uint64_t* out_ij);
+/* ratio_match_reference: the brute-force form (matching.rs:1598-1635). */
+size_t orc_ratio_match_brute(const float* a, size_t na, const float* b, size_t nb, float ratio,
+                             int mutual, uint64_t* out_ij);
+/* Exact row state and column winner of ratio_match, exposed so tests can
+ * compare every intermediate: row_j1/row_d1/row_d2 (na each), col_d/col_i (nb). */
+void orc_ratio_match_state(const float* a, size_t na, const float* b, size_t nb,
+                           uint64_t* row_j1, float* row_d1, float* row_d2, float* col_d,
+                           uint64_t* col_i);
+/* FNV-1a 64 over (i, j) as u64 LE (matching.rs:1867-1878). */
+uint64_t orc_pairs_hash(const uint64_t* ij, size_t n);
+
+/* candidate_pairs (matching.rs:14-40). views: n (cam, pos) pairs as uint64.
+ * Returns the pair count; writes pairs if out != NULL (cap pairs). */
+size_t orc_candidate_pairs(const uint64_t* views, size_t n, uint64_t temporal, uint64_t cross,
+                           uint64_t pow2_max, uint64_t* out, size_t cap);
+
+/* ---- fundamental-matrix RANSAC (matching.rs:217-1104) ---- */
+typedef struct {
+    double threshold_px;
+    double min_inlier_ratio;
+    uint64_t max_iters;
+    double confidence;
+    uint64_t seed;
+    uint64_t min_inliers;
+    double max_chance_prob;
+} orc_ransac_cfg;
+
+void orc_ransac_cfg_default(orc_ransac_cfg* c);  /* matching.rs:781-793 */
+
+typedef struct {
+    int has_selection;
+    int model;          /* 0 = Fundamental, 1 = Homography */
+    double gric_f, gric_h;
+    uint64_t parallax, parallax_needed;
+    double sigma_px;
+} orc_model_sel;
+
+uint64_t orc_adaptive_iterations(double w, int32_t sample, double confidence, uint64_t max_iters);
+double orc_binomial_tail(uint64_t n, uint64_t k, double p);
+double orc_chance_inlier_probability(uint64_t n, uint64_t cnt, double p);
+double orc_epipolar_band_probability(const double* x2, size_t n, double th_px);
+uint64_t orc_parallax_needed(uint64_t n);
+/* F is written row-major (F[3*r+c]). x1, x2: n x 2 doubles. Return 1 = Some. */
+int orc_fundamental_8pt(const double* x1, const double* x2, size_t n, double* F);
+double orc_sampson_error(const double* F, const double* p, const double* q);
+void orc_sampson_residual_jacobian(const double* g, double s1, double s2, const double* a,
+                                   const double* b, double* r, double* j9);
+void orc_normalizer(const double* p, size_t n, double* T);
+int orc_refine_sampson(const double* F, const double* x1, const double* x2, size_t n, size_t iters,
+                       double* out, uint64_t* taken, int* stop);
+uint64_t orc_homography_support(const double* x1, const double* x2, size_t n, double th_px);
+int orc_select_two_view_model(const double* x1, const double* x2, size_t n, const double* F,
+                              double sigma_px, orc_model_sel* out);
+/* Returns 1 = Some(FundamentalFit). F row-major, inliers n bytes. */
+int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const orc_ransac_cfg* cfg,
+                           double* F, uint8_t* inliers, orc_model_sel* sel);
+/* Diagnostics of the last orc_ransac_fundamental call on this thread:
+ * hypotheses drawn, LO runs, LM calls. */
+void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
+                           uint64_t* lm_converged, uint64_t* lm_stalled_at_start);
+
+/* symmetric 9x9 eigen-decomposition used by the 8-point and DLT solvers
+ * (stands in for nalgebra SymmetricEigen, matching.rs:539, two_view.rs:638). */
+void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor);
+
+/* ---- test-geometry generators (matching.rs:1165-1220, camera.rs:41-52,123-159) ---- */
+/* two_cameras(): K from_hfov(960,540,70deg), poses as in matching.rs:1165-1178.
+ * cam: 9 (R row-major) + 3 (t) + 4 (fx, fy, cx, cy) + 2 (w, h) = 18 doubles. */
+void orc_two_cameras(double* c1, double* c2);
+int orc_project(const double* cam, const double* X, double* px);
+void orc_fundamental_from_cameras(const double* c1, const double* c2, double* F);
+/* correspondences(n, sigma, out, seed) (matching.rs:1182-1220). */
+void orc_correspondences(size_t n, double sigma, double out, uint64_t seed, double* x1, double* x2,
+                         uint8_t* truth);
+/* planar_correspondences (matching.rs:1825-1845). */
+void orc_planar_correspondences(size_t n, double sigma, uint64_t seed, double* x1, double* x2);
+/* ground_with_buildings (matching.rs:1925-1952). */
+void orc_ground_with_buildings(size_t n, double bld, double sigma, uint64_t seed, double* x1,
+                               double* x2);
+/* noisy_lines (matching.rs:1792-1806): 60 points per image. */
+void orc_noisy_lines(double sigma, uint64_t seed, double* l1, double* l2);
+/* line_plus_general (matching.rs:2201-2230): x1/x2 hold 60+k, h1/h2 hold 100. */
+void orc_line_plus_general(size_t k, double sigma, uint64_t seed, double* x1, double* x2,
+                           double* h1, double* h2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,60 @@
+// The context behind skh_ctx: device, stream, grow-only device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <initializer_list>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "skh_internal.hpp"
+
+namespace skh {
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string last_error;
+
+    // single-call inputs/outputs
+    DevBuf in_a, in_b, out_ij, out_len;
+    // exact state
+    DevBuf tasks, blocks, row_j1, row_d1, row_d2, col_key, row_match, out_base;
+    // low-precision state
+    DevBuf q_a, q_b, stat_a, stat_b, row_key, row_m2, col_c, amb, amb_cnt;
+
+    std::vector<RowBlock> h_blocks;
+
+    std::vector<DevBuf*> all_bufs() {
+        return {&in_a,   &in_b,   &out_ij,    &out_len, &tasks,  &blocks, &row_j1,
+                &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a, &q_b,
+                &stat_a, &stat_b, &row_key,   &row_m2,  &col_c,  &amb,    &amb_cnt};
+    }
+
+    ~Ctx();
+
+    int fail(int code, const char* fmt, ...);
+    int hip(hipError_t e, const char* what);
+    int ensure(DevBuf& b, size_t bytes, const char* what);
+
+    int exact_state_single(const float* d_a, size_t na, const float* d_b, size_t nb);
+    int ratio_match_dev(const float* d_a, size_t na, const float* d_b, size_t nb, float ratio,
+                        int mutual, int precision, uint32_t* d_out_ij, uint32_t* d_out_len);
+    int ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t nb, float ratio,
+                         int mutual, int precision, uint32_t* d_out_ij, uint32_t* d_out_len);
+};
+
+}  // namespace skh

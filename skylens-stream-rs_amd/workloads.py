@@ -1,0 +1,246 @@
+"""Seeded synthetic workloads for the matching path (numpy only).
+
+Descriptors are unit 128-d f32 rows; keypoints are f32 pixel positions, one
+per feature, in the bench's raw pixel convention (benches/pipeline.rs:276-283
+gathers `kp.x as f64` without +0.5).
+
+* ``single_pair()`` -- the single CPU pair: A 800 unit Gaussian rows; B 480
+  noisy copies (sigma 0.05, renormalised) of distinct random A rows plus 320
+  fresh rows, permuted; ratio 0.8, mutual. Its RANSAC input is 600 projected
+  points (two pinholes fx=fy=300, 320x180, 5 m baseline, 15 deg yaw, sigma
+  0.5 px) plus 400 uniform outliers (40 % of the 1000 correspondences).
+* ``scene(...)`` -- a multi-view scene: 3 cameras x P positions (view index
+  3p + c, matching synth.rs:284-305), per-view feature counts drawn by the
+  caller's rule, 60 % noisy copies (sigma 0.05) of a shared pool of 20k unit
+  vectors and 40 % fresh rows. Every pool vector owns a 3D point; a view only
+  copies pool vectors whose point it sees, and a copy's keypoint is that
+  point's projection (960x540, fx=fy=750, sigma 0.5 px) unless the view's
+  outlier draw replaces it with a uniform position. Each view has its own
+  outlier rate o_v, so a true match survives with (1 - o_a)(1 - o_b).
+  - ``full_scene()``: 240 views, M ~ U[1500, 4096], pair outlier ratio in
+    [20 %, 60 %] (o_v ~ U[0.106, 0.368]); 3663 candidate pairs.
+  - ``uneven_scene()``: 264 views, M log-uniform in [200, 8000], pair inlier
+    ratio in [10 %, 90 %] (o_v ~ U[0.051, 0.684]); the first 4000 of the 4047
+    candidate pairs (48P - 177 has no P giving exactly 4000).
+  - ``fixed_scene()``: 240 views with 8192 features each.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+DESC = 128
+POOL = 20000
+
+
+def unit_rows(rng: np.random.Generator, n: int) -> np.ndarray:
+    x = rng.standard_normal((n, DESC), dtype=np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    return x.astype(np.float32)
+
+
+def noisy_copies(rng: np.random.Generator, src: np.ndarray, sigma: float) -> np.ndarray:
+    y = src + np.float32(sigma) * rng.standard_normal(src.shape, dtype=np.float32)
+    y /= np.linalg.norm(y, axis=1, keepdims=True)
+    return y.astype(np.float32)
+
+
+def scene_pair_descriptors(seed: int, na: int, nb: int, pool: int = 4000) -> Tuple[np.ndarray, np.ndarray]:
+    """One pair drawn like the full scene (60 % pool copies, 40 % fresh)."""
+    rng = np.random.default_rng(seed)
+    P = unit_rows(rng, pool)
+
+    def view(n):
+        k = int(round(0.6 * n))
+        out = np.empty((n, DESC), np.float32)
+        out[:k] = noisy_copies(rng, P[rng.integers(0, pool, k)], 0.05)
+        out[k:] = unit_rows(rng, n - k)
+        return out[rng.permutation(n)]
+
+    return view(na), view(nb)
+
+
+# --------------------------------------------------------------------------
+# cameras
+# --------------------------------------------------------------------------
+
+def rot_y(deg: float) -> np.ndarray:
+    t = np.deg2rad(deg)
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+@dataclass
+class Camera:
+    R: np.ndarray  # world -> camera rotation
+    C: np.ndarray  # centre
+    f: float
+    cx: float
+    cy: float
+    w: int
+    h: int
+
+    def project(self, X: np.ndarray):
+        """(n, 3) world points -> (n, 2) pixels and a visibility mask."""
+        xc = (X - self.C) @ self.R.T
+        z = xc[:, 2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u = self.f * xc[:, 0] / z + self.cx
+            v = self.f * xc[:, 1] / z + self.cy
+        vis = (z > 0) & (u >= 0) & (v >= 0) & (u < self.w) & (v < self.h)
+        return np.stack([u, v], 1), vis
+
+
+# --------------------------------------------------------------------------
+# single CPU pair
+# --------------------------------------------------------------------------
+
+@dataclass
+class SinglePair:
+    a: np.ndarray
+    b: np.ndarray
+    b_source: np.ndarray  # A row each B row copies, -1 for fresh
+    x1: np.ndarray        # (1000, 2) f64 RANSAC input
+    x2: np.ndarray
+    truth: np.ndarray     # True = projected inlier
+
+
+def single_pair(seed: int = 0) -> SinglePair:
+    rng = np.random.default_rng(seed)
+    a = unit_rows(rng, 800)
+    src = rng.choice(800, 480, replace=False)
+    b = np.concatenate([noisy_copies(rng, a[src], 0.05), unit_rows(rng, 320)])
+    b_source = np.concatenate([src, -np.ones(320, np.int64)])
+    perm = rng.permutation(800)
+    b, b_source = b[perm], b_source[perm]
+
+    w, h, f = 320, 180, 300.0
+    c1 = Camera(np.eye(3), np.zeros(3), f, 160.0, 90.0, w, h)
+    c2 = Camera(rot_y(15.0), np.array([5.0, 0.0, 0.0]), f, 160.0, 90.0, w, h)
+    pts = []
+    while sum(len(p) for p in pts) < 600:
+        X = np.stack([rng.uniform(-50, 50, 4000), rng.uniform(-50, 50, 4000),
+                      rng.uniform(60, 80, 4000)], 1)
+        p1, v1 = c1.project(X)
+        p2, v2 = c2.project(X)
+        ok = v1 & v2
+        pts.append(np.concatenate([p1[ok], p2[ok]], 1))
+    P = np.concatenate(pts)[:600]
+    x1 = P[:, :2] + 0.5 * rng.standard_normal((600, 2))
+    x2 = P[:, 2:] + 0.5 * rng.standard_normal((600, 2))
+    o1 = np.stack([rng.uniform(0, w, 400), rng.uniform(0, h, 400)], 1)
+    o2 = np.stack([rng.uniform(0, w, 400), rng.uniform(0, h, 400)], 1)
+    x1 = np.concatenate([x1, o1])
+    x2 = np.concatenate([x2, o2])
+    truth = np.concatenate([np.ones(600, bool), np.zeros(400, bool)])
+    perm = rng.permutation(1000)
+    return SinglePair(a, b, b_source, x1[perm].copy(), x2[perm].copy(), truth[perm].copy())
+
+
+# --------------------------------------------------------------------------
+# multi-view scenes
+# --------------------------------------------------------------------------
+
+@dataclass
+class Scene:
+    views: List[Tuple[int, int]]      # (camera, position), index 3p + c
+    counts: np.ndarray                # features per view
+    offsets: np.ndarray               # CSR offsets into desc/kp (len V + 1)
+    desc: np.ndarray                  # (total, 128) f32
+    kp: np.ndarray                    # (total, 2) f32 pixel positions
+    pool_id: np.ndarray               # (total,) pool vector copied, -1 fresh
+    outlier: np.ndarray               # (total,) keypoint replaced by a uniform draw
+    pairs: np.ndarray                 # (n_pairs, 2) view indices
+    width: int = 960
+    height: int = 540
+
+    def view_desc(self, v: int) -> np.ndarray:
+        return self.desc[self.offsets[v]:self.offsets[v + 1]]
+
+    def view_kp(self, v: int) -> np.ndarray:
+        return self.kp[self.offsets[v]:self.offsets[v + 1]]
+
+
+def _candidate_pairs(views, temporal=5, cross=4, pow2_max=16) -> np.ndarray:
+    """matching.rs:18-40, vectorised (i ascending, then j)."""
+    v = np.asarray(views, np.int64)
+    n = len(v)
+    out = []
+    for i in range(n):
+        j = np.arange(i + 1, n)
+        ca, pa = v[i]
+        cb, pb = v[j, 0], v[j, 1]
+        d = np.abs(pb - pa)
+        pow2 = (d > 0) & ((d & (d - 1)) == 0)
+        same = (d >= 1) & ((d <= temporal) | (pow2 & (d <= pow2_max)))
+        ok = np.where(cb == ca, same, d <= cross)
+        out.append(np.stack([np.full(ok.sum(), i), j[ok]], 1))
+    return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 2), np.int64)
+
+
+def scene(seed: int, positions: int, counts: np.ndarray, outlier_lo: float, outlier_hi: float,
+          max_pairs: Optional[int] = None) -> Scene:
+    rng = np.random.default_rng(seed)
+    W, H, F = 960, 540, 750.0
+    pool = unit_rows(rng, POOL)
+    X = np.stack([rng.uniform(-40, positions + 40, POOL), rng.uniform(-30, 30, POOL),
+                  rng.uniform(60, 80, POOL)], 1)
+    views = [(c, p) for p in range(positions) for c in range(3)]
+    yaw = {0: 0.0, 1: 15.0, 2: -15.0}
+    V = len(views)
+    assert len(counts) == V
+    offsets = np.zeros(V + 1, np.int64)
+    offsets[1:] = np.cumsum(counts)
+    total = int(offsets[-1])
+    desc = np.empty((total, DESC), np.float32)
+    kp = np.empty((total, 2), np.float32)
+    pool_id = np.full(total, -1, np.int64)
+    outl = np.zeros(total, bool)
+    for v, (c, p) in enumerate(views):
+        m = int(counts[v])
+        cam = Camera(rot_y(yaw[c]), np.array([float(p), 0.0, 0.0]), F, W / 2, H / 2, W, H)
+        px, vis = cam.project(X)
+        visible = np.flatnonzero(vis)
+        o_v = rng.uniform(outlier_lo, outlier_hi)
+        k = int(round(0.6 * m))
+        q = visible[rng.integers(0, len(visible), k)]
+        d = np.empty((m, DESC), np.float32)
+        d[:k] = noisy_copies(rng, pool[q], 0.05)
+        d[k:] = unit_rows(rng, m - k)
+        key = np.empty((m, 2))
+        key[:k] = px[q] + 0.5 * rng.standard_normal((k, 2))
+        key[k:] = np.stack([rng.uniform(0, W, m - k), rng.uniform(0, H, m - k)], 1)
+        rep = np.zeros(m, bool)
+        rep[:k] = rng.random(k) < o_v
+        key[rep] = np.stack([rng.uniform(0, W, rep.sum()), rng.uniform(0, H, rep.sum())], 1)
+        pid = np.full(m, -1, np.int64)
+        pid[:k] = q
+        perm = rng.permutation(m)
+        s = slice(offsets[v], offsets[v + 1])
+        desc[s] = d[perm]
+        kp[s] = np.clip(key[perm], 0.0, [W - 1e-3, H - 1e-3]).astype(np.float32)
+        pool_id[s] = pid[perm]
+        outl[s] = rep[perm]
+    pairs = _candidate_pairs(views)
+    if max_pairs is not None:
+        pairs = pairs[:max_pairs]
+    return Scene(views, np.asarray(counts, np.int64), offsets, desc, kp, pool_id, outl, pairs,
+                 W, H)
+
+
+def full_scene(seed: int = 1, positions: int = 80) -> Scene:
+    rng = np.random.default_rng(seed + 1000)
+    counts = rng.integers(1500, 4097, 3 * positions)
+    return scene(seed, positions, counts, 0.106, 0.368)
+
+
+def uneven_scene(seed: int = 4, positions: int = 88, n_pairs: int = 4000) -> Scene:
+    rng = np.random.default_rng(seed + 1000)
+    counts = np.exp(rng.uniform(np.log(200), np.log(8000), 3 * positions)).astype(np.int64)
+    return scene(seed, positions, counts, 0.051, 0.684, max_pairs=n_pairs)
+
+
+def fixed_scene(seed: int = 3, positions: int = 80, features: int = 8192) -> Scene:
+    return scene(seed, positions, np.full(3 * positions, features), 0.106, 0.368)

@@ -1,0 +1,69 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+headers declare, and the host-only entry points match the oracle."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import skylens_hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(skh_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = skylens_hip.load_library()
+    declared = declared_symbols()
+    assert declared, "no declarations found"
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in skylens_hip.EXPORTS}
+    assert declared == bound, (declared ^ bound)
+
+
+def _grid(npos, ncam=3):
+    return [(c, p) for p in range(npos) for c in range(ncam)]
+
+
+@pytest.mark.parametrize("args", [(10, 2, 1, 0), (80, 5, 4, 16), (80, 5, 4, 2**64 - 1),
+                                  (80, 5, 4, 0), (7, 0, 0, 0), (0, 5, 4, 16), (1, 5, 4, 16)])
+def test_candidate_pairs_matches_oracle(args):
+    npos, t, c, p = args
+    views = _grid(npos)
+    assert skylens_hip.candidate_pairs(views, t, c, p) == ob.candidate_pairs(views, t, c, p)
+
+
+def test_candidate_pairs_unsorted_views():
+    rng = np.random.default_rng(3)
+    views = [(int(rng.integers(0, 3)), int(rng.integers(0, 40))) for _ in range(150)]
+    assert skylens_hip.candidate_pairs(views, 5, 4, 16) == ob.candidate_pairs(views, 5, 4, 16)
+
+
+def test_candidate_pairs_capacity_error():
+    lib = skylens_hip.load_library()
+    v = np.array(_grid(10), np.uint64)
+    out = np.zeros((3, 2), np.uint64)
+    n = ctypes.c_size_t(0)
+    rc = lib.skh_candidate_pairs(v.ctypes.data, v.shape[0], 2, 1, 0, out.ctypes.data, 3,
+                                 ctypes.byref(n))
+    assert rc == skylens_hip.SKH_EINVAL
+    assert n.value == 135
+
+
+def test_null_context_is_rejected():
+    lib = skylens_hip.load_library()
+    n = ctypes.c_size_t(0)
+    assert lib.skh_ratio_match(None, None, 0, None, 0, ctypes.c_float(0.8), 1, 0, None, 0,
+                               ctypes.byref(n)) == skylens_hip.SKH_EINVAL
