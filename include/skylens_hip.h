@@ -52,6 +52,14 @@ typedef struct skh_ctx skh_ctx;
  * use one context per thread for concurrency. */
 int skh_ctx_create(int device, skh_ctx** out);
 int skh_ctx_destroy(skh_ctx* ctx);
+/* MFMA ranking tile (rows per workgroup x columns per chunk). SKH_TILE_AUTO
+ * lets the context pick (the autotuned table when one has been loaded). */
+#define SKH_TILE_AUTO -1
+#define SKH_TILE_128x64 0
+#define SKH_TILE_128x128 1
+#define SKH_TILE_256x64 2
+#define SKH_TILE_256x128 3
+int skh_ctx_set_tile(skh_ctx* ctx, int tile);
 /* The HIP stream (hipStream_t) the context launches on. */
 void* skh_ctx_stream(skh_ctx* ctx);
 /* Last error message of this context (static storage owned by ctx). */

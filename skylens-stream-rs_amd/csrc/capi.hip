@@ -81,6 +81,8 @@ int Ctx::exact_state_single(const float* d_a, size_t na, const float* d_b, size_
                                          hipMemcpyHostToDevice, stream),
                           "copy blocks")))
         return rc;
+    // `t` and h_blocks are host memory that the next call may reuse.
+    if ((rc = hip(hipStreamSynchronize(stream), "sync"))) return rc;
     launch_fill_u64(col_key.as<unsigned long long>(), nb, kColEmpty, stream);
     launch_exact_state(tasks.as<PairTask>(), blocks.as<RowBlock>(), nblk, row_j1.as<uint32_t>(),
                        row_d1.as<float>(), row_d2.as<float>(), col_key.as<unsigned long long>(),
@@ -148,6 +150,13 @@ extern "C" int skh_ctx_destroy(skh_ctx* ctx) {
 }
 
 extern "C" void* skh_ctx_stream(skh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int skh_ctx_set_tile(skh_ctx* ctx, int tile) {
+    if (!ctx || tile < SKH_TILE_AUTO || tile > SKH_TILE_256x128) return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tile_override = tile;
+    return SKH_OK;
+}
 
 extern "C" const char* skh_ctx_last_error(skh_ctx* ctx) {
     return ctx ? ctx->last_error.c_str() : "null context";

@@ -1,10 +1,598 @@
-// Low-precision (BF16 / FP8) ratio_match: placeholder until the MFMA kernel lands.
-#include "skylens_hip.h"
-#include "skh_ctx.hpp"
+// Low-precision ratio_match: BF16 MFMA ranking with a rigorous error bound,
+// then the reference's FP32 formula for every decision the bound leaves open.
+//
+// Reference semantics: crates/core/src/matching.rs:157-215 (ratio_match),
+// reproduced exactly; this file only changes how candidates are found.
+//
+// 1. quantize: per descriptor row, q = bf16(a) (round to nearest even), the
+//    f32 norm n = |a|^2 split into bf16 hi + lo, and per-view maxima of |a|,
+//    |a - q| and |q| (the error bound's inputs). Rows with non-finite or huge
+//    values flag the view "bad": pairs touching it are scored exactly.
+// 2. mfma: per workgroup 4 waves x (32*RB) rows of a against all of b in
+//    chunks of 32*CB columns. v_mfma_f32_32x32x16_bf16 over K = 144 =
+//    128 components + [n_hi, n_lo, -1/2, -1/2] on the a side and
+//    [-1/2, -1/2, n_hi, n_lo] on the b side, so the accumulator is
+//    -(|a|^2 + |b|^2 - 2 a.b)/2 = -D/2 directly. Each element then costs
+//    5 VALU ops: pack the column-tile id t into the low mantissa bits,
+//    row top-2 (v_min, v_med3) and column top-2 (v_min, v_med3). The
+//    distance matrix never reaches HBM: rows keep a running top-2 per lane,
+//    columns leave one (c1, c2) partial per workgroup.
+// 3. colmerge: partials -> per-column approximate top-2 values.
+// 4. decide: with eps bounding |2*key - D_ref| for every element of the pair,
+//    each row is a certain accept, a certain reject, or queued.
+// 5. rescore: queued rows get the exact FP32 row scan (and, for mutual,
+//    the exact column scan of their j1) in the reference's order.
+#pragma clang fp contract(off)
+
+#include "skh_internal.hpp"
+#include "skh_lowp.hpp"
 
 namespace skh {
-int Ctx::ratio_match_lowp(const float*, size_t, const float*, size_t, float, int, int, uint32_t*,
-                          uint32_t*) {
-    return fail(SKH_EINVAL, "low-precision ratio_match not built yet");
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
 }
+__device__ __forceinline__ float bf16_to_f(uint32_t h) { return __uint_as_float(h << 16); }
+
+// min(a, -p) as one v_min_f32 with a source negation. fminf would first
+// canonicalise its inputs (one more v_max per element); keys of pairs that
+// reach a decision are never NaN, so the plain instruction is exact here.
+__device__ __forceinline__ float min_neg(float a, float p) {
+    float r;
+    asm("v_min_f32 %0, %1, -%2" : "=v"(r) : "v"(a), "v"(p));
+    return r;
+}
+
+// Rounds a positive float up by a relative 2^-20 so that bounds computed in
+// f32 stay upper bounds.
+__device__ __forceinline__ float up(float x) { return x * (1.0f + 0x1p-20f) + 0x1p-126f; }
+
+__device__ __forceinline__ void atomic_max_pos(float* p, float v) {
+    atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
+}
+
+// One thread per row: bf16 quantisation, packed norm, bound statistics.
+__global__ void quantize_kernel(const float* __restrict__ x, uint32_t n, uint16_t* __restrict__ q,
+                                uint32_t* __restrict__ nrm, ViewStats* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = x + (size_t)i * kDescLen;
+    float s = 0.f, res = 0.f, qq = 0.f;
+    bool bad = false;
+    uint32_t* qo = reinterpret_cast<uint32_t*>(q + (size_t)i * kDescLen);
+    for (int c = 0; c < kDescLen; c += 4) {
+        float4 v = *reinterpret_cast<const float4*>(r + c);
+        float vv[4] = {v.x, v.y, v.z, v.w};
+        uint32_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            bad |= !(fabsf(vv[e]) <= 1e15f);  // NaN, inf, or large enough to overflow
+            h[e] = bf16_rne(vv[e]);
+            float qf = bf16_to_f(h[e]);
+            float d = vv[e] - qf;  // exact: a rounding residual is representable
+            s = s + vv[e] * vv[e];
+            res = res + d * d;
+            qq = qq + qf * qf;
+        }
+        qo[c / 2] = h[0] | (h[1] << 16);
+        qo[c / 2 + 1] = h[2] | (h[3] << 16);
+    }
+    uint32_t hi = bf16_rne(s);
+    float lo = s - bf16_to_f(hi);
+    uint32_t lo16 = bf16_rne(lo);
+    if (bad) {
+        hi = 0;
+        lo16 = 0;
+    }
+    nrm[i] = hi | (lo16 << 16);
+    if (bad) {
+        atomicOr(&st->bad, 1u);
+        return;
+    }
+    // |a|, |a - q| and |q| from f32 sums of squares: the sums carry a relative
+    // error below 130 * 2^-24, covered by the 1.0001 factor.
+    atomic_max_pos(&st->max_norm, up(sqrtf(s) * 1.0001f));
+    atomic_max_pos(&st->max_res, up(sqrtf(res) * 1.0001f));
+    atomic_max_pos(&st->max_qnorm, up(sqrtf(qq) * 1.0001f));
+    atomic_max_pos(&st->max_sq, up(s * 1.0001f));
+}
+
+__global__ void stats_reset_kernel(ViewStats* st, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st[i] = ViewStats{0.f, 0.f, 0.f, 0.f, 0u, 0u};
+}
+
+// ---------------------------------------------------------------------------
+// MFMA ranking kernel
+// ---------------------------------------------------------------------------
+
+// a-side fragment of the last K step: lanes 0..31 carry [n_hi, n_lo, -1/2, -1/2].
+__device__ __forceinline__ bf16x8 tail_frag(uint32_t nrm_packed, bool a_side, bool lo_half) {
+    union {
+        uint32_t u[4];
+        bf16x8 v;
+    } f;
+    const uint32_t half2 = 0xbf00bf00u;  // two bf16 -0.5
+    f.u[0] = lo_half ? (a_side ? nrm_packed : half2) : 0u;
+    f.u[1] = lo_half ? (a_side ? half2 : nrm_packed) : 0u;
+    f.u[2] = 0u;
+    f.u[3] = 0u;
+    return f.v;
+}
+
+__device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
+    union {
+        uint4 u;
+        bf16x8 v;
+    } f;
+    f.u = *reinterpret_cast<const uint4*>(p);
+    return f.v;
+}
+
+// Huge norm used for padding rows/columns: their keys never win a top-2.
+constexpr uint32_t kPadNorm = 0x00007149u;  // bf16 hi = 0x7149 (~1e30), lo = 0
+
+template <int RB, int CB>
+__global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __restrict__ tasks,
+                                                           const RowBlock* __restrict__ blocks,
+                                                           LowpOut out, uint32_t tbits) {
+    constexpr int WR = 32 * RB;         // rows per wave
+    constexpr int BM = 4 * WR;          // rows per workgroup
+    constexpr int BN = 32 * CB;         // columns per chunk
+    constexpr int LDS_STRIDE = 136;     // bf16 per LDS row: 128 + 8 pad (272 B)
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
+    __shared__ uint32_t Bn[BN];
+    __shared__ float2 Cs[4][BN];
+
+    const RowBlock blk = blocks[blockIdx.x];
+    const LowpTask t = tasks[blk.pair];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int l32 = lane & 31;
+    const bool lo_half = lane < 32;
+    const uint32_t tmask = (1u << tbits) - 1u;
+    uint32_t keep_v;  // ~tmask, pinned to a VGPR
+    asm volatile("v_mov_b32 %0, %1" : "=v"(keep_v) : "s"(~tmask));
+
+    // a fragments held in registers for the whole sweep.
+    bf16x8 afrag[RB][9];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const uint32_t row = blk.row0 + wave * WR + rb * 32 + l32;
+        const bool valid = row < t.na;
+        const uint16_t* src = t.qa + (size_t)(valid ? row : 0) * kDescLen + 8 * (lane >> 5);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            afrag[rb][s] = load_frag(src + 16 * s);
+            if (!valid) afrag[rb][s] = bf16x8{};
+        }
+        afrag[rb][8] = tail_frag(valid ? t.nrm_a[row] : kPadNorm, true, lo_half);
+    }
+
+    // Running row top-2 per accumulator register: m1 = packed key, m2 = value.
+    float m1[RB][16], m2[RB][16];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            m1[rb][r] = INFINITY;
+            m2[rb][r] = INFINITY;
+        }
+
+    const uint32_t nchunks = (t.nb + BN - 1) / BN;
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        const uint32_t col0 = ch * BN;
+        __syncthreads();
+        // Previous chunk's column partials: merge the 4 waves, store.
+        if (ch > 0 && tid < BN) {
+            const uint32_t col = col0 - BN + tid;
+            if (col < t.nb) {
+                float a1 = Cs[0][tid].x, a2 = Cs[0][tid].y;
+#pragma unroll
+                for (int w = 1; w < 4; ++w) {
+                    float b1 = Cs[w][tid].x, b2 = Cs[w][tid].y;
+                    float n2 = fminf(fmaxf(a1, b1), fminf(a2, b2));
+                    a1 = fminf(a1, b1);
+                    a2 = n2;
+                }
+                out.col_part[(size_t)t.part_base + (size_t)(blk.row0 / BM) * t.nb + col] =
+                    make_float2(a1, a2);
+            }
+        }
+        // b chunk -> LDS (bf16 rows, padded stride), packed norms.
+        for (int e = tid; e < BN * 16; e += 256) {
+            const int c = e >> 4, seg = e & 15;
+            const uint32_t col = col0 + c;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (col < t.nb) v = *reinterpret_cast<const uint4*>(t.qb + (size_t)col * kDescLen + 8 * seg);
+            *reinterpret_cast<uint4*>(&Bs[c * LDS_STRIDE + 8 * seg]) = v;
+        }
+        if (tid < BN) Bn[tid] = (col0 + tid < t.nb) ? t.nrm_b[col0 + tid] : kPadNorm;
+        __syncthreads();
+
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            const uint16_t* brow = &Bs[(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
+            bf16x8 bfrag[9];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) bfrag[s] = load_frag(brow + 16 * s);
+            bfrag[8] = tail_frag(Bn[cb * 32 + l32], false, lo_half);
+            const uint32_t tile = (ch * CB + cb) & tmask;
+            float c1 = INFINITY, c2 = INFINITY;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                f32x16 acc = {};
+#pragma unroll
+                for (int s = 0; s < 9; ++s)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[rb][s], bfrag[s], acc, 0,
+                                                                  0, 0);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    // key = -(acc with the tile id in its low bits) ~ D/2: one v_and_or_b32
+                    // (keep_v is a VGPR so the SGPR tile id fits gfx9's constant bus).
+                    const float p = __uint_as_float((__float_as_uint(acc[r]) & keep_v) | tile);
+                    m2[rb][r] = __builtin_amdgcn_fmed3f(m1[rb][r], m2[rb][r], -p);
+                    m1[rb][r] = min_neg(m1[rb][r], p);
+                    c2 = __builtin_amdgcn_fmed3f(c1, c2, -p);
+                    c1 = min_neg(c1, p);
+                }
+            }
+            // Merge the two half-waves (same column, other 16*RB rows).
+            const float o1 = __shfl_xor(c1, 32);
+            const float o2 = __shfl_xor(c2, 32);
+            const float n2 = fminf(fmaxf(c1, o1), fminf(c2, o2));
+            c1 = fminf(c1, o1);
+            c2 = n2;
+            if (lo_half) Cs[wave][cb * 32 + l32] = make_float2(c1, c2);
+        }
+    }
+    __syncthreads();
+    if (nchunks > 0 && tid < BN) {
+        const uint32_t col = (nchunks - 1) * BN + tid;
+        if (col < t.nb) {
+            float a1 = Cs[0][tid].x, a2 = Cs[0][tid].y;
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                float b1 = Cs[w][tid].x, b2 = Cs[w][tid].y;
+                float n2 = fminf(fmaxf(a1, b1), fminf(a2, b2));
+                a1 = fminf(a1, b1);
+                a2 = n2;
+            }
+            out.col_part[(size_t)t.part_base + (size_t)(blk.row0 / BM) * t.nb + col] =
+                make_float2(a1, a2);
+        }
+    }
+
+    // Rows: resolve each lane's winning column, merge the 32 lanes of a half.
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float k1 = m1[rb][r], k2 = m2[rb][r];
+            // column of this lane's winner: tile id from the key, lane from l32
+            uint32_t tl = __float_as_uint(-k1) & tmask;
+            uint32_t j = k1 < INFINITY ? (tl * 32u + l32) : kNone;
+            // the tile id wraps modulo 2^tbits only when tbits is too small; the
+            // host sizes tbits so that it never does.
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const float ok1 = __shfl_xor(k1, off);
+                const float ok2 = __shfl_xor(k2, off);
+                const uint32_t oj = __shfl_xor(j, off);
+                const bool other = (ok1 < k1) || (ok1 == k1 && oj < j);
+                const float n2 = other ? fminf(k1, ok2) : fminf(ok1, k2);
+                if (other) {
+                    k1 = ok1;
+                    j = oj;
+                }
+                k2 = n2;
+            }
+            const uint32_t row =
+                blk.row0 + wave * WR + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (l32 == 0 && row < t.na) {
+                const uint32_t g = t.row_base + row;
+                out.row_k1[g] = k1;
+                out.row_j1[g] = j;
+                out.row_k2[g] = k2;
+            }
+        }
+    }
+}
+
+// Per column of every pair: merge the workgroup partials.
+__global__ void colmerge_kernel(const LowpTask* __restrict__ tasks, LowpOut out, uint32_t bm) {
+    const LowpTask t = tasks[blockIdx.y];
+    const uint32_t nparts = (t.na + bm - 1) / bm;
+    for (uint32_t col = blockIdx.x * blockDim.x + threadIdx.x; col < t.nb;
+         col += gridDim.x * blockDim.x) {
+        float a1 = INFINITY, a2 = INFINITY;
+        for (uint32_t p = 0; p < nparts; ++p) {
+            const float2 v = out.col_part[(size_t)t.part_base + (size_t)p * t.nb + col];
+            const float n2 = fminf(fmaxf(a1, v.x), fminf(a2, v.y));
+            a1 = fminf(a1, v.x);
+            a2 = n2;
+        }
+        out.col_top2[t.col_base + col] = make_float2(a1, a2);
+    }
+}
+
+__device__ __forceinline__ double pair_eps(const LowpTask& t, uint32_t tbits) {
+    const ViewStats& A = *t.sa;
+    const ViewStats& B = *t.sb;
+    const double u = 0x1p-24;
+    const double dot_q = (double)A.max_norm * B.max_res + (double)A.max_res * B.max_qnorm;
+    const double qq = (double)A.max_qnorm * B.max_qnorm;
+    const double nn = (double)A.max_sq + B.max_sq;
+    // |acc| bound: |q_a . q_b| + (n_a + n_b)/2 (the bf16 hi+lo norms are within 2^-15 of n)
+    const double acc = qq + 0.5 * nn * (1.0 + 0x1p-15);
+    double e = 2.0 * dot_q                      // bf16 rounding of the components
+               + 2.0 * 2.0 * 132.0 * u * acc    // f32 accumulation inside the MFMA (2x margin)
+               + 2.0 * ldexp(1.0, (int)tbits - 22) * acc  // tile id in the low bits
+               + nn * 0x1p-15                   // hi + lo split of the norms
+               + 64.0 * u * (nn + 2.0 * A.max_norm * (double)B.max_norm)  // reference f32 rounding
+               + 128.0 * u * nn;                // f32 norms of the quantizer
+    return e * 1.25 + 1e-6;
+}
+
+// Certain decisions; uncertain rows go to the exact queue.
+__global__ void decide_kernel(const LowpTask* __restrict__ tasks, LowpOut out, float ratio,
+                              int mutual, uint32_t tbits) {
+    const LowpTask t = tasks[blockIdx.y];
+    const bool bad = (t.sa->bad | t.sb->bad) != 0;
+    const double eps = pair_eps(t, tbits);
+    const float r2 = ratio * ratio;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.na;
+         i += gridDim.x * blockDim.x) {
+        const uint32_t g = t.row_base + i;
+        uint32_t verdict = kNone;
+        bool queue = false, need_row = true;
+        uint32_t j1 = kNone;
+        if (t.nb < 2) {
+            verdict = kNone;  // no second neighbour (matching.rs:206): never a match
+        } else if (bad) {
+            queue = true;
+        } else {
+            const double e1 = 2.0 * (double)out.row_k1[g];
+            const double e2 = 2.0 * (double)out.row_k2[g];
+            j1 = out.row_j1[g];
+            const bool j1_certain = (e2 - e1) > 2.0 * eps && j1 < t.nb;
+            const float lo1 = __double2float_rd(e1 - eps), hi1 = __double2float_ru(e1 + eps);
+            const float lo2 = __double2float_rd(e2 - eps), hi2 = __double2float_ru(e2 + eps);
+            const bool reject = lo1 >= r2 * hi2;
+            const bool accept = hi1 < r2 * fmaxf(lo2, 0.0f);
+            if (reject) {
+                verdict = kNone;
+            } else if (accept && j1_certain) {
+                if (!mutual) {
+                    verdict = j1;
+                } else {
+                    const float2 c = out.col_top2[t.col_base + j1];
+                    const double c1 = 2.0 * (double)c.x, c2 = 2.0 * (double)c.y;
+                    if (c2 - e1 > 2.0 * eps)
+                        verdict = j1;
+                    else if (e1 - c1 > 2.0 * eps)
+                        verdict = kNone;
+                    else {
+                        queue = true;
+                        need_row = false;
+                    }
+                }
+            } else {
+                queue = true;
+            }
+        }
+        out.row_match[g] = verdict;
+        if (queue) {
+            const uint32_t slot = atomicAdd(out.queue_len, 1u);
+            out.queue[slot] = make_uint4(blockIdx.y, i, need_row ? kNone : j1, 0u);
+        }
+    }
+}
+
+// Exact squared distance, the reference's formula (8 lanes, no FMA).
+__device__ __forceinline__ float exact_dist(const float* __restrict__ p, const float* __restrict__ q) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int c = 0; c < kDescLen; c += 8) {
+        float4 p0 = *reinterpret_cast<const float4*>(p + c);
+        float4 p1 = *reinterpret_cast<const float4*>(p + c + 4);
+        float4 q0 = *reinterpret_cast<const float4*>(q + c);
+        float4 q1 = *reinterpret_cast<const float4*>(q + c + 4);
+        float d;
+        d = p0.x - q0.x; acc[0] = acc[0] + d * d;
+        d = p0.y - q0.y; acc[1] = acc[1] + d * d;
+        d = p0.z - q0.z; acc[2] = acc[2] + d * d;
+        d = p0.w - q0.w; acc[3] = acc[3] + d * d;
+        d = p1.x - q1.x; acc[4] = acc[4] + d * d;
+        d = p1.y - q1.y; acc[5] = acc[5] + d * d;
+        d = p1.z - q1.z; acc[6] = acc[6] + d * d;
+        d = p1.w - q1.w; acc[7] = acc[7] + d * d;
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s = s + acc[k];
+    return s;
+}
+
+struct Top2x {
+    uint32_t j;
+    float d1, d2;
+};
+
+__device__ __forceinline__ Top2x merge2(Top2x a, Top2x b) {
+    Top2x r;
+    const bool bw = (b.d1 < a.d1) || (b.d1 == a.d1 && b.j < a.j);
+    if (bw) {
+        r.j = b.j;
+        r.d1 = b.d1;
+        r.d2 = a.d1 < b.d2 ? a.d1 : b.d2;
+    } else {
+        r.j = a.j;
+        r.d1 = a.d1;
+        r.d2 = b.d1 < a.d2 ? b.d1 : a.d2;
+    }
+    return r;
+}
+
+// Exact argmin of column j over all rows of a (lowest i among equal d;
+// d = inf and NaN never win, matching.rs:188-203). Returns kNone if none.
+__device__ uint32_t exact_col_argmin(const LowpTask& t, uint32_t j, unsigned long long* red) {
+    __shared__ __attribute__((aligned(16))) float qrow[kDescLen];
+    __syncthreads();
+    if (threadIdx.x < kDescLen) qrow[threadIdx.x] = t.b[(size_t)j * kDescLen + threadIdx.x];
+    __syncthreads();
+    unsigned long long best = kColEmpty;
+    for (uint32_t i = threadIdx.x; i < t.na; i += blockDim.x) {
+        const float d = exact_dist(t.a + (size_t)i * kDescLen, qrow);
+        if (d < INFINITY) {
+            const unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | i;
+            best = key < best ? key : best;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(best, off);
+        best = o < best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    unsigned long long b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[w] < b ? red[w] : b;
+    __syncthreads();
+    return b == kColEmpty ? kNone : (uint32_t)(b & 0xffffffffu);
+}
+
+// Exact row top-2 of row i against all of b (matching.rs:182-187).
+__device__ Top2x exact_row_top2(const LowpTask& t, uint32_t i, Top2x* red) {
+    __shared__ __attribute__((aligned(16))) float arow[kDescLen];
+    __syncthreads();
+    if (threadIdx.x < kDescLen) arow[threadIdx.x] = t.a[(size_t)i * kDescLen + threadIdx.x];
+    __syncthreads();
+    Top2x st{kNone, INFINITY, INFINITY};
+    for (uint32_t j = threadIdx.x; j < t.nb; j += blockDim.x) {
+        const float d = exact_dist(arow, t.b + (size_t)j * kDescLen);
+        if (d < st.d1) {
+            st.d2 = st.d1;
+            st.d1 = d;
+            st.j = j;
+        } else if (d < st.d2) {
+            st.d2 = d;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        Top2x o;
+        o.j = __shfl_xor(st.j, off);
+        o.d1 = __shfl_xor(st.d1, off);
+        o.d2 = __shfl_xor(st.d2, off);
+        st = merge2(st, o);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = st;
+    __syncthreads();
+    Top2x r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = merge2(r, red[w]);
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict__ tasks,
+                                                      LowpOut out, float ratio, int mutual,
+                                                      uint32_t tbits) {
+    __shared__ Top2x red_t[4];
+    __shared__ unsigned long long red_c[4];
+    const float r2 = ratio * ratio;
+    const uint32_t n = *out.queue_len;
+    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const uint4 item = out.queue[q];
+        const LowpTask t = tasks[item.x];
+        const uint32_t i = item.y;
+        const bool bad = (t.sa->bad | t.sb->bad) != 0;
+        const double eps = pair_eps(t, tbits);
+        uint32_t j1 = item.z;
+        bool accept;
+        bool col_needed = true;
+        if (j1 == kNone) {
+            const Top2x st = exact_row_top2(t, i, red_t);
+            j1 = st.j;
+            accept = st.j != kNone && isfinite(st.d2) && !(st.d1 >= r2 * st.d2);
+            if (accept && mutual && !bad) {
+                const float2 c = out.col_top2[t.col_base + j1];
+                const double d1 = st.d1;
+                if ((2.0 * (double)c.y) - d1 > eps) {
+                    col_needed = false;  // row i is the unique column winner
+                } else if (d1 - 2.0 * (double)c.x > eps) {
+                    accept = false;  // another row is strictly closer
+                    col_needed = false;
+                }
+            }
+        } else {
+            accept = true;  // ratio accepted with certainty, j1 certain
+        }
+        if (accept && mutual && col_needed) accept = exact_col_argmin(t, j1, red_c) == i;
+        if (threadIdx.x == 0) out.row_match[t.row_base + i] = accept ? j1 : kNone;
+    }
+}
+
+}  // namespace
+
+void launch_stats_reset(ViewStats* st, uint32_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(stats_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, st, n);
+}
+
+void launch_quantize(const float* x, uint32_t n, uint16_t* q, uint32_t* nrm, ViewStats* st,
+                     hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(quantize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, n, q, nrm, st);
+}
+
+uint32_t lowp_block_rows(int tile) { return tile_rb(tile) * 128u; }
+
+void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
+                      const LowpOut& out, uint32_t tbits, hipStream_t s) {
+    if (!n_blocks) return;
+    switch (tile) {
+        case kTile128x64:
+            hipLaunchKernelGGL((mfma_rank_kernel<1, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
+                               blocks, out, tbits);
+            break;
+        case kTile128x128:
+            hipLaunchKernelGGL((mfma_rank_kernel<1, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
+                               blocks, out, tbits);
+            break;
+        case kTile256x64:
+            hipLaunchKernelGGL((mfma_rank_kernel<2, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
+                               blocks, out, tbits);
+            break;
+        default:
+            hipLaunchKernelGGL((mfma_rank_kernel<2, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
+                               blocks, out, tbits);
+            break;
+    }
+}
+
+void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
+                     hipStream_t s) {
+    if (!n_pairs) return;
+    hipLaunchKernelGGL(colmerge_kernel, dim3(8, n_pairs), dim3(256), 0, s, tasks, out, bm);
+}
+
+void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
+                   int mutual, uint32_t tbits, hipStream_t s) {
+    if (!n_pairs) return;
+    hipLaunchKernelGGL(decide_kernel, dim3(8, n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
+                       tbits);
+}
+
+void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
+                    uint32_t tbits, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(rescore_kernel, dim3(grid), dim3(256), 0, s, tasks, out, ratio, mutual,
+                       tbits);
+}
+
 }  // namespace skh

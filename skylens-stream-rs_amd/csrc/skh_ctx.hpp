@@ -25,6 +25,7 @@ struct DevBuf {
 struct Ctx {
     int device = 0;
     int num_cu = 0;
+    int tile_override = -1;  // SKH_TILE_AUTO
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string last_error;
@@ -34,14 +35,16 @@ struct Ctx {
     // exact state
     DevBuf tasks, blocks, row_j1, row_d1, row_d2, col_key, row_match, out_base;
     // low-precision state
-    DevBuf q_a, q_b, stat_a, stat_b, row_key, row_m2, col_c, amb, amb_cnt;
+    DevBuf q_a, q_b, nrm_a, nrm_b, stats, ltasks, row_k1, row_k2, col_part, col_top2, queue,
+        queue_len;
 
     std::vector<RowBlock> h_blocks;
 
     std::vector<DevBuf*> all_bufs() {
-        return {&in_a,   &in_b,   &out_ij,    &out_len, &tasks,  &blocks, &row_j1,
-                &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a, &q_b,
-                &stat_a, &stat_b, &row_key,   &row_m2,  &col_c,  &amb,    &amb_cnt};
+        return {&in_a,   &in_b,   &out_ij,    &out_len,  &tasks,    &blocks, &row_j1,
+                &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a,    &q_b,
+                &nrm_a,  &nrm_b,  &stats,     &ltasks,   &row_k1,   &row_k2, &col_part,
+                &col_top2, &queue, &queue_len};
     }
 
     ~Ctx();

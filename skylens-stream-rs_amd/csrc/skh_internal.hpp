@@ -11,7 +11,7 @@ namespace skh {
 constexpr int kDescLen = 128;
 
 // Which low-precision ranking kernels this build contains.
-constexpr bool kHaveBf16 = false;
+constexpr bool kHaveBf16 = true;
 constexpr bool kHaveFp8 = false;
 
 // One ratio_match problem: a (na rows) against b (nb rows), both row-major

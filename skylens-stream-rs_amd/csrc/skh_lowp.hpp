@@ -1,0 +1,73 @@
+// Shared declarations of the low-precision (MFMA-ranked) ratio_match.
+#pragma once
+
+#include "skh_internal.hpp"
+
+namespace skh {
+
+// Per-view inputs of the error bound (all upper bounds, f32): max |a|,
+// max |a - bf16(a)|, max |bf16(a)|, max |a|^2; bad != 0 if any component is
+// non-finite or too large for the bound.
+struct ViewStats {
+    float max_norm, max_res, max_qnorm, max_sq;
+    uint32_t bad, pad;
+};
+
+struct LowpTask {
+    const float* a;            // f32 rows (exact re-scoring)
+    const float* b;
+    const uint16_t* qa;        // bf16 rows [n][128]
+    const uint16_t* qb;
+    const uint32_t* nrm_a;     // packed bf16 (hi | lo << 16) of |row|^2
+    const uint32_t* nrm_b;
+    const ViewStats* sa;
+    const ViewStats* sb;
+    uint32_t na, nb;
+    uint32_t row_base;         // per-row arrays
+    uint32_t col_base;         // per-column arrays (col_top2)
+    uint64_t part_base;        // column partials: ceil(na / BM) * nb float2
+};
+
+struct LowpOut {
+    float* row_k1;             // packed key of the row minimum (~ D/2)
+    uint32_t* row_j1;          // its column
+    float* row_k2;             // second smallest key
+    float2* col_part;          // per workgroup column (c1, c2)
+    float2* col_top2;          // merged column (c1, c2)
+    uint32_t* row_match;       // final j or kNone
+    uint4* queue;              // (pair, row, j1 | kNone, 0) needing exact work
+    uint32_t* queue_len;
+};
+
+enum TileShape : int {
+    kTile128x64 = 0,   // 4 waves x 32 rows, 64-column chunks
+    kTile128x128 = 1,  // 4 waves x 32 rows, 128-column chunks
+    kTile256x64 = 2,   // 4 waves x 64 rows, 64-column chunks
+    kTile256x128 = 3,  // 4 waves x 64 rows, 128-column chunks
+};
+constexpr int kNumTiles = 4;
+inline uint32_t tile_rb(int tile) { return (tile == kTile256x64 || tile == kTile256x128) ? 2u : 1u; }
+inline uint32_t tile_bn(int tile) { return (tile == kTile128x128 || tile == kTile256x128) ? 128u : 64u; }
+
+uint32_t lowp_block_rows(int tile);
+void launch_stats_reset(ViewStats* st, uint32_t n, hipStream_t s);
+void launch_quantize(const float* x, uint32_t n, uint16_t* q, uint32_t* nrm, ViewStats* st,
+                     hipStream_t s);
+void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
+                      const LowpOut& out, uint32_t tbits, hipStream_t s);
+void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
+                     hipStream_t s);
+void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
+                   int mutual, uint32_t tbits, hipStream_t s);
+void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
+                    uint32_t tbits, uint32_t grid, hipStream_t s);
+
+// Bits needed for a column-tile id of a pair with nb columns (tiles of 32).
+inline uint32_t tile_bits(uint64_t nb_max) {
+    uint64_t tiles = (nb_max + 31) / 32;
+    uint32_t b = 1;
+    while ((1ull << b) < tiles) ++b;
+    return b;
+}
+
+}  // namespace skh

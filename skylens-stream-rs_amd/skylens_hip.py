@@ -34,6 +34,7 @@ SKH_ENOMEM = -3
 SKH_ENODEV = -4
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp8": 2}
+TILES = {"auto": -1, "128x64": 0, "128x128": 1, "256x64": 2, "256x128": 3}
 
 # matching.rs:7-12
 PAIR_TEMPORAL = 5
@@ -48,6 +49,7 @@ EXPORTS = [
     ("skh_precision_supported", ctypes.c_int, [ctypes.c_int]),
     ("skh_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     ("skh_ctx_destroy", ctypes.c_int, [_vp]),
+    ("skh_ctx_set_tile", ctypes.c_int, [_vp, ctypes.c_int]),
     ("skh_ctx_stream", _vp, [_vp]),
     ("skh_ctx_last_error", ctypes.c_char_p, [_vp]),
     ("skh_candidate_pairs", ctypes.c_int,
@@ -125,6 +127,10 @@ class Context:
         if rc == SKH_EINVAL:
             raise ValueError(f"{what}: {msg}")
         raise SkhError(f"{what}: status {rc}: {msg}")
+
+    def set_tile(self, tile: int):
+        """MFMA ranking tile (TILES index; -1 = automatic choice)."""
+        self._check(self.lib.skh_ctx_set_tile(self.handle, int(tile)), "skh_ctx_set_tile")
 
     @property
     def stream(self) -> int:

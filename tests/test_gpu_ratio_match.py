@@ -11,7 +11,10 @@ from test_oracle_matching import _boundary_cases, permutation_fixture
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ["fp32"]
+import skylens_hip
+
+PRECS = skylens_hip.available_precisions()
+LOWP = [p for p in PRECS if p != "fp32"]
 
 
 def _say(msg):
@@ -150,3 +153,22 @@ def test_scene_sized_pair(ctx, prec):
     want = ob.ratio_match(a, b, 0.8, True)
     assert got == want
     assert len(want) > 50
+
+
+@pytest.mark.parametrize("prec", LOWP)
+@pytest.mark.parametrize("tile", ["128x64", "128x128", "256x64", "256x128"])
+def test_every_tile_shape(ctx, prec, tile):
+    """Each MFMA tile gives the exact list (ragged row blocks and column chunks)."""
+    _say(f"tile {tile} {prec}")
+    ctx.set_tile(skylens_hip.TILES[tile])
+    try:
+        for seed, na, extra in [(3, 64, 64), (5, 700, 300), (21, 2000, 700)]:
+            a, b = ob.match_fixture(seed, na, extra)
+            for mutual in (True, False):
+                assert ctx.ratio_match(a, b, 0.8, mutual, prec) == ob.ratio_match(a, b, 0.8, mutual)
+                assert ctx.ratio_match(b, a, 0.8, mutual, prec) == ob.ratio_match(b, a, 0.8, mutual)
+        from workloads import scene_pair_descriptors
+        a, b = scene_pair_descriptors(seed=11, na=1537, nb=4097)
+        assert ctx.ratio_match(a, b, 0.8, True, prec) == ob.ratio_match(a, b, 0.8, True)
+    finally:
+        ctx.set_tile(-1)
