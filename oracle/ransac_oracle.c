@@ -1,0 +1,1054 @@
+/*
+ * CPU oracle, fundamental-matrix RANSAC half. TEST INFRASTRUCTURE ONLY (see
+ * oracle.h): the checker for the HIP path, never the product.
+ *
+ * Restates crates/core/src/matching.rs:217-1104 and
+ * crates/core/src/two_view.rs:605-642 (homography_dlt). The reference's
+ * linear algebra comes from nalgebra 0.33.3 (Cargo.lock:166-168):
+ * SymmetricEigen (matching.rs:539, two_view.rs:638), SVD (matching.rs:550,
+ * 577), Cholesky (:714), try_inverse (:263, :340, :359, :675). nalgebra is not
+ * vendored and cannot be built here, so the oracle uses its own solvers with
+ * the same mathematical contract:
+ *   - symmetric 9x9 eigen: cyclic Jacobi (Golub & Van Loan, Matrix
+ *     Computations, 4th ed., Alg. 8.5.3) with the classic tiny-element rule;
+ *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958);
+ *   - Cholesky: right-looking column factorisation, forward/back solves;
+ *   - 3x3 inverse: cofactor expansion.
+ * Rotations are applied unconditionally with (c, s, t) = (1, 0, 0) where a
+ * rotation is skipped, so a GPU lane executing the same operations produces
+ * the same bits. The F bits therefore pin the HIP path to this oracle, not to
+ * the Rust binary (parity with the reference is at the level of its tests).
+ *
+ * Matrices are row-major double[9] / double[81]. All sums run in the
+ * reference's order (first element first).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+/* ---------------------------------------------------------------- helpers */
+
+static const double NM[6] = {3.0, 2.0, 1.5, 1.0, 1.0, 1.0}; /* matching.rs:998, :1043 */
+
+static inline int all_finite2(const double* p, size_t n) {
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!isfinite(p[i])) return 0;
+    return 1;
+}
+
+/* y = M x (nalgebra gemv order: column 0, 1, 2). */
+static inline void mv3(const double* m, const double* x, double* y) {
+    double a = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+    double b = m[3] * x[0] + m[4] * x[1] + m[5] * x[2];
+    double c = m[6] * x[0] + m[7] * x[1] + m[8] * x[2];
+    y[0] = a; y[1] = b; y[2] = c;
+}
+
+/* y = M^T x. */
+static inline void mtv3(const double* m, const double* x, double* y) {
+    double a = m[0] * x[0] + m[3] * x[1] + m[6] * x[2];
+    double b = m[1] * x[0] + m[4] * x[1] + m[7] * x[2];
+    double c = m[2] * x[0] + m[5] * x[1] + m[8] * x[2];
+    y[0] = a; y[1] = b; y[2] = c;
+}
+
+/* C = A B. */
+static inline void mm3(const double* A, const double* B, double* C) {
+    double t[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            t[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+    memcpy(C, t, sizeof t);
+}
+
+static inline void tr3(const double* A, double* T) {
+    double t[9] = {A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
+    memcpy(T, t, sizeof t);
+}
+
+/* Frobenius norm, summed column by column (nalgebra norm_squared). */
+static inline double fro3(const double* m) {
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        double col = m[c] * m[c] + m[3 + c] * m[3 + c] + m[6 + c] * m[6 + c];
+        s = s + col;
+    }
+    return sqrt(s);
+}
+
+static inline double norm2(double x, double y) { return sqrt(x * x + y * y); }
+
+/* 3x3 inverse by cofactors; returns 0 when the determinant is zero. */
+static int inv3(const double* m, double* out) {
+    double m11 = m[4] * m[8] - m[5] * m[7];
+    double m12 = m[3] * m[8] - m[5] * m[6];
+    double m13 = m[3] * m[7] - m[4] * m[6];
+    double det = m[0] * m11 - m[1] * m12 + m[2] * m13;
+    if (det == 0.0) return 0;
+    double o[9];
+    o[0] = m11 / det;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+    o[3] = -m12 / det;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+    o[6] = m13 / det;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+    memcpy(out, o, sizeof o);
+    return 1;
+}
+
+/* LCG step of the samplers (matching.rs:957-962, :315-320, :884-889). */
+static inline uint64_t rnd(uint64_t* st, uint64_t m) {
+    *st = *st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (*st >> 33) % m;
+}
+
+/* ------------------------------------------------------------ eigen / SVD */
+
+/* One Jacobi rotation of the symmetric 9x9 `a` (and eigenvector matrix `v`)
+ * annihilating a[p][q]. (c, s, t) = (1, 0, 0) leaves everything as is but the
+ * explicit zero of a[p][q]. */
+static inline void jacobi_rotate9(double* a, double* v, int p, int q, double c, double s, double t) {
+    const double apq = a[9 * p + q];
+    for (int k = 0; k < 9; ++k) {
+        if (k == p || k == q) continue;
+        double akp = a[9 * k + p], akq = a[9 * k + q];
+        double np = c * akp - s * akq;
+        double nq = s * akp + c * akq;
+        a[9 * k + p] = np; a[9 * p + k] = np;
+        a[9 * k + q] = nq; a[9 * q + k] = nq;
+    }
+    a[9 * p + p] = a[9 * p + p] - t * apq;
+    a[9 * q + q] = a[9 * q + q] + t * apq;
+    a[9 * p + q] = 0.0;
+    a[9 * q + p] = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        double vkp = v[9 * k + p], vkq = v[9 * k + q];
+        v[9 * k + p] = c * vkp - s * vkq;
+        v[9 * k + q] = s * vkp + c * vkq;
+    }
+}
+
+/* Rotation parameters for a[p][q] in sweep `sweep` (classic rule: after the
+ * fourth sweep an element negligible against both diagonal entries is set to
+ * zero without rotating). Returns 1 if a rotation is due. */
+static inline int jacobi_params(double app, double aqq, double apq, int sweep, double* c,
+                                double* s, double* t) {
+    *c = 1.0; *s = 0.0; *t = 0.0;
+    if (apq == 0.0) return 0;
+    double g = 100.0 * fabs(apq);
+    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return 0;
+    double h = aqq - app;
+    double tt;
+    if (fabs(h) + g == fabs(h)) {
+        tt = apq / h;
+    } else {
+        double theta = 0.5 * h / apq;
+        tt = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+        if (theta < 0.0) tt = -tt;
+    }
+    double cc = 1.0 / sqrt(1.0 + tt * tt);
+    *c = cc; *s = tt * cc; *t = tt;
+    return 1;
+}
+
+#define JACOBI_MAX_SWEEPS 50
+
+void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor) {
+    double a[81], v[81];
+    memcpy(a, A, sizeof a);
+    for (int i = 0; i < 81; ++i) v[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < JACOBI_MAX_SWEEPS; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < 8; ++p)
+            for (int q = p + 1; q < 9; ++q) off = off + fabs(a[9 * p + q]);
+        if (off == 0.0) break;
+        for (int p = 0; p < 8; ++p)
+            for (int q = p + 1; q < 9; ++q) {
+                double c, s, t;
+                jacobi_params(a[9 * p + p], a[9 * q + q], a[9 * p + q], sweep, &c, &s, &t);
+                jacobi_rotate9(a, v, p, q, c, s, t);
+            }
+    }
+    for (int k = 0; k < 9; ++k) {
+        evals[k] = a[9 * k + k];
+        for (int i = 0; i < 9; ++i) evecs_colmajor[9 * k + i] = v[9 * i + k];
+    }
+}
+
+static int imin9(const double* e) {
+    int k = 0;
+    for (int i = 1; i < 9; ++i)
+        if (e[i] < e[k]) k = i;
+    return k;
+}
+
+static int cmp_double(const void* x, const void* y) {
+    double a = *(const double*)x, b = *(const double*)y;
+    return (a > b) - (a < b);
+}
+
+/* One-sided Jacobi SVD of F (3x3): W = F V with orthogonal columns, then the
+ * column of least norm is dropped and F' = W' V^T (matching.rs:550-553,
+ * :577-580: zero the smallest singular value, recompose). */
+#define SVD3_MAX_SWEEPS 30
+static void rank2_recompose(const double* F, double* out) {
+    double w[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(w, F, sizeof w);
+    static const int PP[3] = {0, 0, 1}, QQ[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < SVD3_MAX_SWEEPS; ++sweep) {
+        int rotated = 0;
+        for (int r = 0; r < 3; ++r) {
+            int p = PP[r], q = QQ[r];
+            double alpha = w[p] * w[p] + w[3 + p] * w[3 + p] + w[6 + p] * w[6 + p];
+            double beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
+            double gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
+            double c = 1.0, s = 0.0;
+            if (gamma != 0.0 && fabs(gamma) > 1e-15 * sqrt(alpha * beta)) {
+                double zeta = (beta - alpha) / (2.0 * gamma);
+                double t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                if (zeta < 0.0) t = -t;
+                c = 1.0 / sqrt(1.0 + t * t);
+                s = c * t;
+                rotated = 1;
+            }
+            for (int k = 0; k < 3; ++k) {
+                double wp = w[3 * k + p], wq = w[3 * k + q];
+                w[3 * k + p] = c * wp - s * wq;
+                w[3 * k + q] = s * wp + c * wq;
+                double vp = v[3 * k + p], vq = v[3 * k + q];
+                v[3 * k + p] = c * vp - s * vq;
+                v[3 * k + q] = s * vp + c * vq;
+            }
+        }
+        if (!rotated) break;
+    }
+    double sig[3];
+    for (int c = 0; c < 3; ++c) sig[c] = w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c];
+    int m = 0;
+    for (int c = 1; c < 3; ++c)
+        if (sig[c] < sig[m]) m = c;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                if (i == m) continue;
+                acc = acc + w[3 * r + i] * v[3 * c + i];
+            }
+            out[3 * r + c] = acc;
+        }
+}
+
+/* matching.rs:576-583. */
+static int rank2_unit(const double* f, double* out) {
+    double g[9];
+    rank2_recompose(f, g);
+    double n = fro3(g);
+    if (!(isfinite(n) && n > 0.0)) return 0;
+    for (int i = 0; i < 9; ++i) out[i] = g[i] / n;
+    return 1;
+}
+
+/* --------------------------------------------------------------- geometry */
+
+/* matching.rs:225-241. */
+static void axis_spread(const double* p, size_t len, double* lo_out, double* hi_out) {
+    double n = (double)(len > 0 ? len : 1);
+    double cx = 0.0, cy = 0.0;
+    for (size_t i = 0; i < len; ++i) { cx = cx + p[2 * i]; cy = cy + p[2 * i + 1]; }
+    cx = cx / n; cy = cy / n;
+    double sxx = 0.0, sxy = 0.0, syy = 0.0;
+    for (size_t i = 0; i < len; ++i) {
+        double dx = p[2 * i] - cx, dy = p[2 * i + 1] - cy;
+        sxx += dx * dx;
+        sxy += dx * dy;
+        syy += dy * dy;
+    }
+    sxx = sxx / n; sxy = sxy / n; syy = syy / n;
+    double tr = sxx + syy;
+    double dd = sxx - syy;
+    double disc = sqrt(dd * dd + 4.0 * sxy * sxy);
+    double hi = 0.5 * (tr + disc);
+    double lo = 0.5 * (tr - disc);
+    if (!(lo >= 0.0)) lo = 0.0; /* f64::max(0.0) */
+    *lo_out = sqrt(lo);
+    *hi_out = sqrt(hi);
+}
+
+/* matching.rs:244-247. */
+static int nearly_collinear(const double* p, size_t n) {
+    double lo, hi;
+    axis_spread(p, n, &lo, &hi);
+    return !(hi > 0.0 && lo > 0.02 * hi);
+}
+
+/* matching.rs:499-505. */
+void orc_normalizer(const double* p, size_t len, double* T) {
+    double n = (double)len;
+    double cx = 0.0, cy = 0.0;
+    for (size_t i = 0; i < len; ++i) { cx = cx + p[2 * i]; cy = cy + p[2 * i + 1]; }
+    cx = cx / n; cy = cy / n;
+    double d = 0.0;
+    for (size_t i = 0; i < len; ++i) d = d + norm2(p[2 * i] - cx, p[2 * i + 1] - cy);
+    d = d / n;
+    double s = d > 0.0 ? 1.4142135623730951 / d : 1.0;
+    double t[9] = {s, 0.0, -s * cx, 0.0, s, -s * cy, 0.0, 0.0, 1.0};
+    memcpy(T, t, sizeof t);
+}
+
+/* matching.rs:512-557. */
+int orc_fundamental_8pt(const double* x1, const double* x2, size_t n, double* F) {
+    if (n < 8 || !all_finite2(x1, n) || !all_finite2(x2, n)) return 0;
+    if (nearly_collinear(x1, n) || nearly_collinear(x2, n)) return 0;
+    double t1[9], t2[9];
+    orc_normalizer(x1, n, t1);
+    orc_normalizer(x2, n, t2);
+    double ata[81];
+    memset(ata, 0, sizeof ata);
+    for (size_t i = 0; i < n; ++i) {
+        double pa[3] = {x1[2 * i], x1[2 * i + 1], 1.0}, pb[3] = {x2[2 * i], x2[2 * i + 1], 1.0};
+        double a[3], b[3];
+        mv3(t1, pa, a);
+        mv3(t2, pb, b);
+        double row[9] = {b[0] * a[0], b[0] * a[1], b[0], b[1] * a[0], b[1] * a[1], b[1],
+                         a[0], a[1], 1.0};
+        for (int r = 0; r < 9; ++r)
+            for (int c = 0; c < 9; ++c) ata[9 * r + c] = ata[9 * r + c] + row[r] * row[c];
+    }
+    double ev[9], vec[81];
+    orc_sym_eigen9(ata, ev, vec);
+    int k = imin9(ev);
+    double sorted[9];
+    memcpy(sorted, ev, sizeof sorted);
+    qsort(sorted, 9, sizeof(double), cmp_double);
+    if (sorted[1] <= 1e-10 * sorted[8]) return 0;
+    double fn[9];
+    for (int i = 0; i < 9; ++i) fn[i] = vec[9 * k + i];
+    double r2[9];
+    rank2_recompose(fn, r2);
+    double t2t[9], tmp[9], f[9];
+    tr3(t2, t2t);
+    mm3(t2t, r2, tmp);
+    mm3(tmp, t1, f);
+    double nn = fro3(f);
+    if (!(isfinite(nn) && nn > 0.0)) return 0;
+    for (int i = 0; i < 9; ++i) F[i] = f[i] / nn;
+    return 1;
+}
+
+/* matching.rs:561-573. */
+double orc_sampson_error(const double* F, const double* p, const double* q) {
+    double a[3] = {p[0], p[1], 1.0}, b[3] = {q[0], q[1], 1.0};
+    double fa[3], ftb[3];
+    mv3(F, a, fa);
+    mtv3(F, b, ftb);
+    double e = b[0] * fa[0] + b[1] * fa[1] + b[2] * fa[2];
+    double den = fa[0] * fa[0] + fa[1] * fa[1] + ftb[0] * ftb[0] + ftb[1] * ftb[1];
+    if (den <= 0.0) return INFINITY;
+    return e * e / den;
+}
+
+/* matching.rs:591-618. */
+void orc_sampson_residual_jacobian(const double* g, double s1, double s2, const double* a,
+                                   const double* b, double* r, double* j) {
+    double ga[3], gtb[3];
+    mv3(g, a, ga);
+    mtv3(g, b, gtb);
+    double e = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
+    double q1 = s1 * s1, q2 = s2 * s2;
+    double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+    if (den <= 0.0) {
+        *r = 0.0;
+        for (int k = 0; k < 9; ++k) j[k] = 0.0;
+        return;
+    }
+    double inv = 1.0 / sqrt(den);
+    double c = e * inv * inv * inv;
+    double u[3] = {q2 * ga[0], q2 * ga[1], 0.0};
+    double v[3] = {q1 * gtb[0], q1 * gtb[1], 0.0};
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l)
+            j[3 * k + l] = b[k] * a[l] * inv - c * (u[k] * a[l] + b[k] * v[l]);
+    *r = e * inv;
+}
+
+/* matching.rs:621-641, summed into the cost sum_e e^2 in order. */
+static double sampson_cost(const double* g, double s1, double s2, const double* a,
+                           const double* b, size_t n) {
+    double q1 = s1 * s1, q2 = s2 * s2;
+    double cost = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double ga[3], gtb[3];
+        mv3(g, a + 3 * i, ga);
+        mtv3(g, b + 3 * i, gtb);
+        double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+        double e;
+        if (den > 0.0) {
+            const double* bb = b + 3 * i;
+            e = (bb[0] * ga[0] + bb[1] * ga[1] + bb[2] * ga[2]) / sqrt(den);
+        } else {
+            e = 0.0;
+        }
+        cost = cost + e * e;
+    }
+    return cost;
+}
+
+/* 9x9 Cholesky (nalgebra's column-by-column, right-looking update) and solve
+ * m x = rhs. Returns 0 if a pivot is not positive. */
+static int cholesky_solve9(const double* m_in, const double* rhs, double* x) {
+    double L[81];
+    memcpy(L, m_in, sizeof L);
+    for (int j = 0; j < 9; ++j) {
+        for (int k = 0; k < j; ++k) {
+            double factor = -L[9 * j + k];
+            for (int i = j; i < 9; ++i) L[9 * i + j] = L[9 * i + j] + factor * L[9 * i + k];
+        }
+        double diag = L[9 * j + j];
+        if (!(diag > 0.0)) return 0;
+        double den = sqrt(diag);
+        L[9 * j + j] = den;
+        for (int i = j + 1; i < 9; ++i) L[9 * i + j] = L[9 * i + j] / den;
+    }
+    double y[9];
+    memcpy(y, rhs, sizeof y);
+    for (int i = 0; i < 9; ++i) {
+        y[i] = y[i] / L[9 * i + i];
+        for (int k = i + 1; k < 9; ++k) y[k] = y[k] - L[9 * k + i] * y[i];
+    }
+    for (int i = 8; i >= 0; --i) {
+        double dot = 0.0;
+        for (int k = i + 1; k < 9; ++k) dot = dot + L[9 * k + i] * y[k];
+        y[i] = (y[i] - dot) / L[9 * i + i];
+    }
+    memcpy(x, y, sizeof y);
+    return 1;
+}
+
+static __thread uint64_t g_lm_calls, g_lm_conv, g_lm_stall0, g_iters, g_lo_runs;
+
+/* matching.rs:665-760. stop: 0 = converged, 1 = stalled, 2 = cap. */
+int orc_refine_sampson(const double* F, const double* x1, const double* x2, size_t n,
+                       size_t iters, double* out, uint64_t* taken_out, int* stop_out) {
+    memcpy(out, F, sizeof(double) * 9);
+    if (taken_out) *taken_out = 0;
+    if (stop_out) *stop_out = 2;
+    if (n < 8) return 0;
+    double t1[9], t2[9], t1i[9], t2i[9];
+    orc_normalizer(x1, n, t1);
+    orc_normalizer(x2, n, t2);
+    if (!inv3(t1, t1i) || !inv3(t2, t2i)) return 0;
+    double s1 = t1[0], s2 = t2[0];
+    double* a = (double*)malloc(sizeof(double) * 3 * n);
+    double* b = (double*)malloc(sizeof(double) * 3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        double pa[3] = {x1[2 * i], x1[2 * i + 1], 1.0}, pb[3] = {x2[2 * i], x2[2 * i + 1], 1.0};
+        mv3(t1, pa, a + 3 * i);
+        mv3(t2, pb, b + 3 * i);
+    }
+    double t2it[9], tmp[9], g0[9], g[9];
+    tr3(t2i, t2it);
+    mm3(t2it, F, tmp);
+    mm3(tmp, t1i, g0);
+    if (!rank2_unit(g0, g)) { free(a); free(b); return 0; }
+    double cost = sampson_cost(g, s1, s2, a, b, n);
+    double lambda = 1e-3;
+    uint64_t taken = 0;
+    int stop = 2;
+    for (size_t it = 0; it < iters; ++it) {
+        double jtj[81], jtr[9];
+        memset(jtj, 0, sizeof jtj);
+        memset(jtr, 0, sizeof jtr);
+        for (size_t i = 0; i < n; ++i) {
+            double ri, ji[9];
+            orc_sampson_residual_jacobian(g, s1, s2, a + 3 * i, b + 3 * i, &ri, ji);
+            for (int c = 0; c < 9; ++c)
+                for (int r = c; r < 9; ++r) jtj[9 * r + c] = ji[c] * ji[r] + jtj[9 * r + c];
+            for (int k = 0; k < 9; ++k) jtr[k] = jtr[k] + ji[k] * ri;
+        }
+        for (int r = 0; r < 9; ++r)
+            for (int c = r + 1; c < 9; ++c) jtj[9 * r + c] = jtj[9 * c + r];
+        int improved = 0, accepted = 0;
+        for (int tries = 0; tries < 8; ++tries) {
+            double m[81];
+            memcpy(m, jtj, sizeof m);
+            for (int k = 0; k < 9; ++k) m[9 * k + k] = m[9 * k + k] + lambda * (jtj[9 * k + k] + 1e-12);
+            double nrhs[9], d[9];
+            for (int k = 0; k < 9; ++k) nrhs[k] = -jtr[k];
+            if (!cholesky_solve9(m, nrhs, d)) { lambda *= 10.0; continue; }
+            double gs[9], cand[9];
+            for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
+            if (!rank2_unit(gs, cand)) { lambda *= 10.0; continue; }
+            double c = sampson_cost(cand, s1, s2, a, b, n);
+            if (c < cost) {
+                memcpy(g, cand, sizeof g);
+                double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
+                cost = c;
+                lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;
+                taken += 1;
+                accepted = 1;
+                improved = rel > 1e-10;
+                break;
+            }
+            lambda *= 10.0;
+        }
+        if (!improved) {
+            stop = accepted ? 0 : 1;
+            break;
+        }
+    }
+    free(a);
+    free(b);
+    g_lm_calls++;
+    if (stop == 0) g_lm_conv++;
+    if (stop == 1 && taken == 0) g_lm_stall0++;
+    if (taken_out) *taken_out = taken;
+    if (stop_out) *stop_out = stop;
+    double t2t[9], fp[9];
+    tr3(t2, t2t);
+    mm3(t2t, g, tmp);
+    mm3(tmp, t1, fp);
+    double nn = fro3(fp);
+    if (isfinite(nn) && nn > 0.0)
+        for (int i = 0; i < 9; ++i) out[i] = fp[i] / nn;
+    return 1;
+}
+
+/* Binary exponentiation (Rust powi, compiler-rt __powidf2). */
+static double powi_(double a, int b) {
+    int recip = b < 0;
+    unsigned ub = recip ? (unsigned)(-b) : (unsigned)b;
+    double r = 1.0;
+    for (;;) {
+        if (ub & 1) r *= a;
+        ub /= 2;
+        if (ub == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+/* matching.rs:799-811. */
+uint64_t orc_adaptive_iterations(double w, int32_t sample, double confidence, uint64_t max_iters) {
+    uint64_t floor_ = 50 < max_iters ? 50 : max_iters;
+    double wc = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+    double p_good = powi_(wc, sample);
+    if (p_good >= 1.0) return floor_;
+    double denom = log1p(-p_good);
+    const double cmax = 1.0 - 1e-15;
+    double cc = confidence < 0.0 ? 0.0 : (confidence > cmax ? cmax : confidence);
+    double need = log1p(-cc) / denom;
+    if (!isfinite(need) || need >= (double)max_iters) return max_iters;
+    double ce = ceil(need);
+    uint64_t v = ce <= 0.0 ? 0 : (uint64_t)ce;
+    if (v < floor_) v = floor_;
+    if (v > max_iters) v = max_iters;
+    return v;
+}
+
+/* matching.rs:825-840. */
+double orc_epipolar_band_probability(const double* x2, size_t n, double th_px) {
+    double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY;
+    for (size_t i = 0; i < n; ++i) {
+        lox = fmin(lox, x2[2 * i]); loy = fmin(loy, x2[2 * i + 1]);
+        hix = fmax(hix, x2[2 * i]); hiy = fmax(hiy, x2[2 * i + 1]);
+    }
+    double dx = hix - lox, dy = hiy - loy;
+    double area = dx * dy;
+    if (!(isfinite(area) && area > 0.0)) return 1.0;
+    double v = 2.0 * th_px * norm2(dx, dy) / area;
+    return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+}
+
+/* matching.rs:843-865. */
+double orc_binomial_tail(uint64_t n, uint64_t k, double p) {
+    if (k == 0) return 1.0;
+    if (k > n || p <= 0.0) return 0.0;
+    if (p >= 1.0) return 1.0;
+    double lp = log(p), lq = log1p(-p);
+    double log_c = 0.0, sum = 0.0;
+    for (uint64_t i = 0; i <= n; ++i) {
+        if (i > 0) log_c += log((double)(n - i + 1)) - log((double)i);
+        if (i >= k) sum += exp(log_c + (double)i * lp + (double)(n - i) * lq);
+    }
+    return sum < 1.0 ? sum : 1.0;
+}
+
+/* matching.rs:869-874 (FREE_FIT = 8, :821). */
+double orc_chance_inlier_probability(uint64_t n, uint64_t cnt, double p) {
+    if (cnt <= 8) return 1.0;
+    return orc_binomial_tail(n > 8 ? n - 8 : 0, cnt - 8, p);
+}
+
+/* matching.rs:878-906. */
+static size_t off_line_count(const double* p, size_t n, double th_px) {
+    if (n < 3) return 0;
+    uint64_t st = 0x6A09E667F3BCC909ULL;
+    size_t best = 0;
+    for (int it = 0; it < 300; ++it) {
+        uint64_t i = rnd(&st, n), j = rnd(&st, n);
+        double dx = p[2 * j] - p[2 * i], dy = p[2 * j + 1] - p[2 * i + 1];
+        double len = norm2(dx, dy);
+        if (i == j || isnan(len) || len <= 0.0) continue;
+        double nx = -dy / len, ny = dx / len;
+        size_t c = 0;
+        for (size_t q = 0; q < n; ++q) {
+            double ex = p[2 * q] - p[2 * i], ey = p[2 * q + 1] - p[2 * i + 1];
+            if (fabs(ex * nx + ey * ny) < th_px) ++c;
+        }
+        if (c > best) best = c;
+    }
+    return n - best;
+}
+
+/* two_view.rs:605-642: DLT on (already normalised) points. */
+static int homography_dlt_core(const double* a, const double* b, size_t n, double* H) {
+    if (n < 4) return 0;
+    double ata[81];
+    memset(ata, 0, sizeof ata);
+    for (size_t i = 0; i < n; ++i) {
+        double x = a[2 * i], y = a[2 * i + 1], u = b[2 * i], v = b[2 * i + 1];
+        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, u * x, u * y, u};
+        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, v * x, v * y, v};
+        for (int r = 0; r < 9; ++r)
+            for (int c = 0; c < 9; ++c)
+                ata[9 * r + c] = ata[9 * r + c] + (r1[r] * r1[c] + r2[r] * r2[c]);
+    }
+    double ev[9], vec[81];
+    orc_sym_eigen9(ata, ev, vec);
+    int k = imin9(ev);
+    for (int i = 0; i < 9; ++i) {
+        H[i] = vec[9 * k + i];
+        if (!isfinite(H[i])) return 0;
+    }
+    return 1;
+}
+
+/* matching.rs:251-266: Hartley-normalised pixel DLT. */
+static int homography_dlt(const double* x1, const double* x2, size_t n, double* H) {
+    if (n < 4) return 0;
+    double t1[9], t2[9];
+    orc_normalizer(x1, n, t1);
+    orc_normalizer(x2, n, t2);
+    double* a = (double*)malloc(sizeof(double) * 2 * n);
+    double* b = (double*)malloc(sizeof(double) * 2 * n);
+    for (size_t i = 0; i < n; ++i) {
+        double p[3] = {x1[2 * i], x1[2 * i + 1], 1.0}, v[3];
+        mv3(t1, p, v);
+        a[2 * i] = v[0] / v[2]; a[2 * i + 1] = v[1] / v[2];
+        double q[3] = {x2[2 * i], x2[2 * i + 1], 1.0};
+        mv3(t2, q, v);
+        b[2 * i] = v[0] / v[2]; b[2 * i + 1] = v[1] / v[2];
+    }
+    double hn[9];
+    int ok = homography_dlt_core(a, b, n, hn);
+    free(a);
+    free(b);
+    if (!ok) return 0;
+    double t2i[9], tmp[9], h[9];
+    if (!inv3(t2, t2i)) return 0;
+    mm3(t2i, hn, tmp);
+    mm3(tmp, t1, h);
+    double nn = fro3(h);
+    if (!(isfinite(nn) && nn > 0.0)) return 0;
+    for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
+    return 1;
+}
+
+/* matching.rs:269-287. */
+static int htr(const double* m, const double* x, double* out) {
+    double p[3] = {x[0], x[1], 1.0}, v[3];
+    mv3(m, p, v);
+    if (fabs(v[2]) < 1e-12) return 0;
+    out[0] = v[0] / v[2];
+    out[1] = v[1] / v[2];
+    return 1;
+}
+
+static double homography_error(const double* h, const double* hi, const double* p,
+                               const double* q) {
+    double a[2], b[2];
+    if (!htr(h, p, a) || !htr(hi, q, b)) return INFINITY;
+    double e1 = norm2(a[0] - q[0], a[1] - q[1]);
+    double e2 = norm2(b[0] - p[0], b[1] - p[1]);
+    return fmax(e1, e2);
+}
+
+static size_t count_h(const double* h, const double* hi, const double* x1, const double* x2,
+                      size_t n, double th) {
+    size_t c = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (homography_error(h, hi, x1 + 2 * i, x2 + 2 * i) < th) ++c;
+    return c;
+}
+
+/* matching.rs:300-369. Returns 1 with H, H^-1 and the count. */
+static int fit_homography(const double* x1, const double* x2, size_t n, double th_px, double* H,
+                          double* Hi, size_t* cnt_out) {
+    if (n < 4) return 0;
+    uint64_t st = 0x2545F4914F6CDD1DULL;
+    int have = 0;
+    double bh[9], bhi[9];
+    size_t bc = 0;
+    uint64_t iters = 500, it = 0;
+    double s1[8], s2[8];
+    while (it < iters) {
+        it++;
+        uint64_t idx[4];
+        int k = 0;
+        while (k < 4) {
+            uint64_t c = rnd(&st, n);
+            int dup = 0;
+            for (int e = 0; e < k; ++e) dup |= idx[e] == c;
+            if (!dup) idx[k++] = c;
+        }
+        for (int e = 0; e < 4; ++e) {
+            s1[2 * e] = x1[2 * idx[e]]; s1[2 * e + 1] = x1[2 * idx[e] + 1];
+            s2[2 * e] = x2[2 * idx[e]]; s2[2 * e + 1] = x2[2 * idx[e] + 1];
+        }
+        double h[9], hi[9];
+        if (!homography_dlt(s1, s2, 4, h)) continue;
+        if (!inv3(h, hi)) continue;
+        size_t c = count_h(h, hi, x1, x2, n, th_px);
+        if (!have || c > bc) {
+            have = 1;
+            memcpy(bh, h, sizeof bh);
+            memcpy(bhi, hi, sizeof bhi);
+            bc = c;
+            iters = orc_adaptive_iterations((double)c / (double)n, 4, 0.999, 500);
+        }
+    }
+    if (!have) return 0;
+    double ch[9], chi[9];
+    memcpy(ch, bh, sizeof ch);
+    memcpy(chi, bhi, sizeof chi);
+    static const double HM[4] = {4.0, 2.0, 1.0, 1.0};
+    double* sx1 = (double*)malloc(sizeof(double) * 2 * n);
+    double* sx2 = (double*)malloc(sizeof(double) * 2 * n);
+    for (int r = 0; r < 4; ++r) {
+        size_t ns = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (homography_error(ch, chi, x1 + 2 * i, x2 + 2 * i) < th_px * HM[r]) {
+                sx1[2 * ns] = x1[2 * i]; sx1[2 * ns + 1] = x1[2 * i + 1];
+                sx2[2 * ns] = x2[2 * i]; sx2[2 * ns + 1] = x2[2 * i + 1];
+                ns++;
+            }
+        }
+        double g[9], gi[9];
+        if (!homography_dlt(sx1, sx2, ns, g)) break;
+        if (!inv3(g, gi)) break;
+        memcpy(ch, g, sizeof ch);
+        memcpy(chi, gi, sizeof chi);
+        size_t c = count_h(g, gi, x1, x2, n, th_px);
+        if (c > bc) {
+            memcpy(bh, g, sizeof bh);
+            memcpy(bhi, gi, sizeof bhi);
+            bc = c;
+        }
+    }
+    free(sx1);
+    free(sx2);
+    memcpy(H, bh, sizeof bh);
+    memcpy(Hi, bhi, sizeof bhi);
+    *cnt_out = bc;
+    return 1;
+}
+
+uint64_t orc_homography_support(const double* x1, const double* x2, size_t n, double th_px) {
+    double h[9], hi[9];
+    size_t c = 0;
+    if (!fit_homography(x1, x2, n, th_px, h, hi, &c)) return 0;
+    return c;
+}
+
+/* matching.rs:405-409. */
+uint64_t orc_parallax_needed(uint64_t n) {
+    for (uint64_t k = 2; k <= n; ++k)
+        if (orc_binomial_tail(n, k, 0.001) < 1e-6) return k;
+    return n + 1;
+}
+
+/* matching.rs:425-492. */
+int orc_select_two_view_model(const double* x1, const double* x2, size_t n, const double* F,
+                              double sigma_px, orc_model_sel* out) {
+    memset(out, 0, sizeof *out);
+    if (n < 8 || !isfinite(sigma_px) || sigma_px <= 0.0 || !all_finite2(x1, n) ||
+        !all_finite2(x2, n))
+        return 0;
+    for (int i = 0; i < 9; ++i)
+        if (!isfinite(F[i])) return 0;
+    double* es = (double*)malloc(sizeof(double) * n);
+    size_t ne = 0;
+    for (size_t i = 0; i < n; ++i) {
+        double e = orc_sampson_error(F, x1 + 2 * i, x2 + 2 * i);
+        if (isfinite(e)) es[ne++] = e;
+    }
+    if (ne < 8) { free(es); return 0; }
+    qsort(es, ne, sizeof(double), cmp_double);
+    double sigma_hat = sqrt(es[ne / 2] / 0.455);
+    free(es);
+    double sigma = fmax(sigma_px, sigma_hat);
+    double s2 = sigma * sigma;
+    double h[9], hi[9];
+    size_t hc;
+    if (!fit_homography(x1, x2, n, 3.0 * sigma, h, hi, &hc)) return 0;
+    double rho_f = 0.0, rho_h = 0.0;
+    uint64_t parallax = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double* p = x1 + 2 * i;
+        const double* q = x2 + 2 * i;
+        double ef = orc_sampson_error(F, p, q) / s2;
+        double a[2], b[2], t1, t2;
+        t1 = htr(h, p, a) ? (a[0] - q[0]) * (a[0] - q[0]) + (a[1] - q[1]) * (a[1] - q[1]) : INFINITY;
+        t2 = htr(hi, q, b) ? (b[0] - p[0]) * (b[0] - p[0]) + (b[1] - p[1]) * (b[1] - p[1]) : INFINITY;
+        double eh = 0.25 * (t1 + t2) / s2;
+        rho_f += fmin(ef, 2.0 * (4.0 - 3.0));
+        rho_h += fmin(eh, 2.0 * (4.0 - 2.0));
+        if (eh > 13.8 && ef < 10.8) parallax++;
+    }
+    double nf = (double)n;
+    double l1 = log(4.0), l2 = log(4.0 * nf);
+    out->gric_f = rho_f + l1 * 3.0 * nf + l2 * 7.0;
+    out->gric_h = rho_h + l1 * 2.0 * nf + l2 * 8.0;
+    out->parallax = parallax;
+    out->parallax_needed = orc_parallax_needed(n);
+    out->model = (out->gric_h < out->gric_f && parallax < out->parallax_needed) ? 1 : 0;
+    out->sigma_px = sigma;
+    out->has_selection = 1;
+    return 1;
+}
+
+void orc_ransac_cfg_default(orc_ransac_cfg* c) {
+    c->threshold_px = 1.5;
+    c->min_inlier_ratio = 0.2;
+    c->max_iters = 2000;
+    c->confidence = 0.999;
+    c->seed = 1;
+    c->min_inliers = 15;
+    c->max_chance_prob = 1e-6;
+}
+
+static size_t inliers_of(const double* F, const double* x1, const double* x2, size_t n,
+                         double th2, uint8_t* mask) {
+    size_t c = 0;
+    for (size_t i = 0; i < n; ++i) {
+        mask[i] = orc_sampson_error(F, x1 + 2 * i, x2 + 2 * i) < th2;
+        c += mask[i];
+    }
+    return c;
+}
+
+static size_t select_pts(const double* F, const double* x1, const double* x2, size_t n,
+                         double t2, double* s1, double* s2) {
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (orc_sampson_error(F, x1 + 2 * i, x2 + 2 * i) < t2) {
+            s1[2 * k] = x1[2 * i]; s1[2 * k + 1] = x1[2 * i + 1];
+            s2[2 * k] = x2[2 * i]; s2[2 * k + 1] = x2[2 * i + 1];
+            k++;
+        }
+    }
+    return k;
+}
+
+static size_t gather_mask(const uint8_t* mask, const double* x1, const double* x2, size_t n,
+                          double* s1, double* s2) {
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (mask[i]) {
+            s1[2 * k] = x1[2 * i]; s1[2 * k + 1] = x1[2 * i + 1];
+            s2[2 * k] = x2[2 * i]; s2[2 * k + 1] = x2[2 * i + 1];
+            k++;
+        }
+    }
+    return k;
+}
+
+typedef struct {
+    double F[9];
+    uint8_t* mask;
+    size_t cnt;
+} hyp_t;
+
+void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
+                           uint64_t* lm_converged, uint64_t* lm_stalled_at_start) {
+    if (iters) *iters = g_iters;
+    if (lo_runs) *lo_runs = g_lo_runs;
+    if (lm_calls) *lm_calls = g_lm_calls;
+    if (lm_converged) *lm_converged = g_lm_conv;
+    if (lm_stalled_at_start) *lm_stalled_at_start = g_lm_stall0;
+}
+
+/* matching.rs:941-1104. */
+int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const orc_ransac_cfg* cfg,
+                           double* F_out, uint8_t* inl_out, orc_model_sel* sel) {
+    g_iters = g_lo_runs = g_lm_calls = g_lm_conv = g_lm_stall0 = 0;
+    if (sel) memset(sel, 0, sizeof *sel);
+    if (n < 8 || !all_finite2(x1, n) || !all_finite2(x2, n)) return 0;
+    const double th2 = cfg->threshold_px * cfg->threshold_px;
+    uint64_t st = cfg->seed ^ 0x9E3779B97F4A7C15ULL;
+    double* s1 = (double*)malloc(sizeof(double) * 2 * n);
+    double* s2 = (double*)malloc(sizeof(double) * 2 * n);
+    uint8_t* inl = (uint8_t*)malloc(n);
+    uint8_t* gi = (uint8_t*)malloc(n);
+    uint8_t* best_mask = (uint8_t*)malloc(n);
+    hyp_t pool[6];
+    for (int k = 0; k < 6; ++k) pool[k].mask = (uint8_t*)malloc(n);
+    int npool = 0;
+    int have_best = 0;
+    double bestF[9];
+    size_t best_cnt_v = 0;
+    uint64_t iters = cfg->max_iters, it = 0;
+    while (it < iters) {
+        it++;
+        uint64_t idx[8];
+        int k = 0;
+        while (k < 8) {
+            uint64_t c = rnd(&st, n);
+            int dup = 0;
+            for (int e = 0; e < k; ++e) dup |= idx[e] == c;
+            if (!dup) idx[k++] = c;
+        }
+        double a8[16], b8[16];
+        for (int e = 0; e < 8; ++e) {
+            a8[2 * e] = x1[2 * idx[e]]; a8[2 * e + 1] = x1[2 * idx[e] + 1];
+            b8[2 * e] = x2[2 * idx[e]]; b8[2 * e + 1] = x2[2 * idx[e] + 1];
+        }
+        double f[9];
+        if (!orc_fundamental_8pt(a8, b8, 8, f)) continue;
+        size_t cnt = inliers_of(f, x1, x2, n, th2, inl);
+        size_t best_cnt = have_best ? best_cnt_v : 0;
+        if (cnt >= 8 && 4 * cnt >= best_cnt) {
+            g_lo_runs++;
+            double f_lo[9], cur[9];
+            memcpy(f_lo, f, sizeof f_lo);
+            memcpy(cur, f, sizeof cur);
+            for (int r = 0; r < 6; ++r) {
+                double t2 = th2 * NM[r] * NM[r];
+                size_t ns = select_pts(cur, x1, x2, n, t2, s1, s2);
+                double g[9];
+                if (!orc_fundamental_8pt(s1, s2, ns, g)) break;
+                memcpy(cur, g, sizeof cur);
+                size_t gc = inliers_of(g, x1, x2, n, th2, gi);
+                if (gc > cnt) {
+                    memcpy(f_lo, g, sizeof f_lo);
+                    memcpy(inl, gi, n);
+                    cnt = gc;
+                }
+            }
+            /* matching.rs:1017-1028: Jaccard > 0.7 valley test, stable sort. */
+            int same = -1;
+            for (int j = 0; j < npool && same < 0; ++j) {
+                size_t both = 0, either = 0;
+                for (size_t i = 0; i < n; ++i) {
+                    both += pool[j].mask[i] && inl[i];
+                    either += pool[j].mask[i] || inl[i];
+                }
+                if (10 * both > 7 * either) same = j;
+            }
+            if (same >= 0) {
+                if (pool[same].cnt < cnt) {
+                    memcpy(pool[same].F, f_lo, sizeof f_lo);
+                    memcpy(pool[same].mask, inl, n);
+                    pool[same].cnt = cnt;
+                }
+            } else {
+                memcpy(pool[npool].F, f_lo, sizeof f_lo);
+                memcpy(pool[npool].mask, inl, n);
+                pool[npool].cnt = cnt;
+                npool++;
+            }
+            /* stable insertion sort by count, descending */
+            for (int a = 1; a < npool; ++a) {
+                hyp_t x = pool[a];
+                int b = a - 1;
+                while (b >= 0 && pool[b].cnt < x.cnt) {
+                    pool[b + 1] = pool[b];
+                    b--;
+                }
+                pool[b + 1] = x;
+            }
+            if (npool > 5) npool = 5; /* the dropped mask buffer stays at pool[5] */
+            if (cnt <= best_cnt) continue;
+            double w = (double)cnt / (double)n;
+            iters = orc_adaptive_iterations(w, 8, cfg->confidence, cfg->max_iters);
+            memcpy(bestF, f_lo, sizeof bestF);
+            memcpy(best_mask, inl, n);
+            best_cnt_v = cnt;
+            have_best = 1;
+        }
+    }
+    g_iters = it;
+    int result = 0;
+    if (!have_best) goto done;
+    {
+        double f[9];
+        memcpy(f, bestF, sizeof f);
+        memcpy(inl, best_mask, n);
+        size_t inl_cnt = best_cnt_v;
+        /* matching.rs:1041-1057 */
+        for (int p = 0; p < npool; ++p) {
+            double cur[9];
+            memcpy(cur, pool[p].F, sizeof cur);
+            for (int r = 0; r < 6; ++r) {
+                double t2 = th2 * NM[r] * NM[r];
+                size_t ns = select_pts(cur, x1, x2, n, t2, s1, s2);
+                double nxt[9];
+                orc_refine_sampson(cur, s1, s2, ns, 30, nxt, NULL, NULL);
+                memcpy(cur, nxt, sizeof cur);
+                size_t c = inliers_of(cur, x1, x2, n, th2, gi);
+                if (c > inl_cnt) {
+                    memcpy(f, cur, sizeof f);
+                    memcpy(inl, gi, n);
+                    inl_cnt = c;
+                }
+            }
+        }
+        /* matching.rs:1059-1069 */
+        for (int r = 0; r < 2; ++r) {
+            size_t ns = gather_mask(inl, x1, x2, n, s1, s2);
+            double g[9];
+            orc_refine_sampson(f, s1, s2, ns, 30, g, NULL, NULL);
+            size_t c = inliers_of(g, x1, x2, n, th2, gi);
+            if (c < ns) break;
+            memcpy(f, g, sizeof f);
+            memcpy(inl, gi, n);
+            inl_cnt = c;
+        }
+        size_t cnt = inl_cnt;
+        /* matching.rs:1071-1091 */
+        size_t minc = cfg->min_inliers > 8 ? cfg->min_inliers : 8;
+        if (cnt < minc || (double)cnt < cfg->min_inlier_ratio * (double)n) goto done;
+        if (cfg->max_chance_prob > 0.0) {
+            double p = orc_epipolar_band_probability(x2, n, cfg->threshold_px);
+            if (orc_chance_inlier_probability(n, cnt, p) > cfg->max_chance_prob) goto done;
+        }
+        size_t ns = gather_mask(inl, x1, x2, n, s1, s2);
+        if (nearly_collinear(s1, ns) || nearly_collinear(s2, ns)) goto done;
+        if (off_line_count(s1, ns, cfg->threshold_px) < 8 ||
+            off_line_count(s2, ns, cfg->threshold_px) < 8)
+            goto done;
+        /* matching.rs:1096-1097 */
+        orc_model_sel ms;
+        int has = orc_select_two_view_model(s1, s2, ns, f, cfg->threshold_px / 3.0, &ms);
+        if (sel) {
+            if (has) *sel = ms;
+            else memset(sel, 0, sizeof *sel);
+        }
+        memcpy(F_out, f, sizeof f);
+        memcpy(inl_out, inl, n);
+        result = 1;
+    }
+done:
+    free(s1);
+    free(s2);
+    free(inl);
+    free(gi);
+    free(best_mask);
+    for (int k = 0; k < 6; ++k) free(pool[k].mask);
+    return result;
+}

@@ -152,3 +152,189 @@ def candidate_pairs(views, temporal, cross, pow2_max):
     lib().orc_candidate_pairs(P(v), v.shape[0], min(temporal, mx), min(cross, mx),
                               min(pow2_max, mx), P(out), n)
     return [(int(i), int(j)) for i, j in out[:n]]
+
+
+# ---------------------------------------------------------------- RANSAC
+_SIGS_R = {
+    "orc_ransac_cfg_default": (None, [_vp]),
+    "orc_adaptive_iterations": (_u64, [_d, ctypes.c_int32, _d, _u64]),
+    "orc_binomial_tail": (_d, [_u64, _u64, _d]),
+    "orc_chance_inlier_probability": (_d, [_u64, _u64, _d]),
+    "orc_epipolar_band_probability": (_d, [_vp, _sz, _d]),
+    "orc_parallax_needed": (_u64, [_u64]),
+    "orc_fundamental_8pt": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "orc_sampson_error": (_d, [_vp, _vp, _vp]),
+    "orc_sampson_residual_jacobian": (None, [_vp, _d, _d, _vp, _vp, _vp, _vp]),
+    "orc_normalizer": (None, [_vp, _sz, _vp]),
+    "orc_refine_sampson": (ctypes.c_int, [_vp, _vp, _vp, _sz, _sz, _vp, _vp, _vp]),
+    "orc_homography_support": (_u64, [_vp, _vp, _sz, _d]),
+    "orc_select_two_view_model": (ctypes.c_int, [_vp, _vp, _sz, _vp, _d, _vp]),
+    "orc_ransac_fundamental": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
+    "orc_ransac_last_stats": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_sym_eigen9": (None, [_vp, _vp, _vp]),
+    "orc_two_cameras": (None, [_vp, _vp]),
+    "orc_project": (ctypes.c_int, [_vp, _vp, _vp]),
+    "orc_fundamental_from_cameras": (None, [_vp, _vp, _vp]),
+    "orc_correspondences": (None, [_sz, _d, _d, _u64, _vp, _vp, _vp]),
+    "orc_planar_correspondences": (None, [_sz, _d, _u64, _vp, _vp]),
+    "orc_ground_with_buildings": (None, [_sz, _d, _d, _u64, _vp, _vp]),
+    "orc_noisy_lines": (None, [_d, _u64, _vp, _vp]),
+    "orc_line_plus_general": (None, [_sz, _d, _u64, _vp, _vp, _vp, _vp]),
+}
+_SIGS.update(_SIGS_R)
+
+
+def default_cfg(**kw) -> RansacCfg:
+    c = RansacCfg()
+    lib().orc_ransac_cfg_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _xy(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 2))
+
+
+def adaptive_iterations(w, s, conf, mx):
+    return int(lib().orc_adaptive_iterations(w, s, conf, mx))
+
+
+def binomial_tail(n, k, p):
+    return float(lib().orc_binomial_tail(n, k, p))
+
+
+def parallax_needed(n):
+    return int(lib().orc_parallax_needed(n))
+
+
+def fundamental_8pt(x1, x2):
+    x1, x2 = _xy(x1), _xy(x2)
+    F = np.zeros(9)
+    n = min(x1.shape[0], x2.shape[0]) if x1.shape[0] == x2.shape[0] else 0
+    ok = lib().orc_fundamental_8pt(P(x1), P(x2), n, P(F)) if x1.shape[0] == x2.shape[0] else 0
+    return F.reshape(3, 3) if ok else None
+
+
+def sampson_error(F, p, q):
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    p = np.ascontiguousarray(p, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    return float(lib().orc_sampson_error(P(F), P(p), P(q)))
+
+
+def sampson_residual_jacobian(g, s1, s2, a, b):
+    g = np.ascontiguousarray(g, np.float64).reshape(9)
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    r = np.zeros(1)
+    j = np.zeros(9)
+    lib().orc_sampson_residual_jacobian(P(g), s1, s2, P(a), P(b), P(r), P(j))
+    return float(r[0]), j
+
+
+def normalizer(p):
+    p = _xy(p)
+    T = np.zeros(9)
+    lib().orc_normalizer(P(p), p.shape[0], P(T))
+    return T.reshape(3, 3)
+
+
+def homography_support(x1, x2, th):
+    x1, x2 = _xy(x1), _xy(x2)
+    return int(lib().orc_homography_support(P(x1), P(x2), x1.shape[0], th))
+
+
+def select_two_view_model(x1, x2, F, sigma):
+    x1, x2 = _xy(x1), _xy(x2)
+    if x1.shape[0] != x2.shape[0]:
+        return None
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    m = ModelSel()
+    ok = lib().orc_select_two_view_model(P(x1), P(x2), x1.shape[0], P(F), sigma, ctypes.byref(m))
+    return m if ok else None
+
+
+def ransac_fundamental(x1, x2, cfg=None):
+    """(F 3x3, inliers bool[n], ModelSel) or None."""
+    x1, x2 = _xy(x1), _xy(x2)
+    if x1.shape[0] != x2.shape[0]:
+        return None
+    cfg = cfg or default_cfg()
+    n = x1.shape[0]
+    F = np.zeros(9)
+    inl = np.zeros(max(n, 1), np.uint8)
+    sel = ModelSel()
+    ok = lib().orc_ransac_fundamental(P(x1), P(x2), n, ctypes.byref(cfg), P(F), P(inl),
+                                      ctypes.byref(sel))
+    if not ok:
+        return None
+    return F.reshape(3, 3), inl[:n].astype(bool), sel
+
+
+def ransac_last_stats():
+    v = [np.zeros(1, np.uint64) for _ in range(5)]
+    lib().orc_ransac_last_stats(*[P(x) for x in v])
+    return dict(zip(["iters", "lo_runs", "lm_calls", "lm_converged", "lm_stalled_at_start"],
+                    [int(x[0]) for x in v]))
+
+
+def sym_eigen9(A):
+    A = np.ascontiguousarray(A, np.float64).reshape(81)
+    ev = np.zeros(9)
+    vec = np.zeros(81)
+    lib().orc_sym_eigen9(P(A), P(ev), P(vec))
+    return ev, vec.reshape(9, 9).T  # columns are eigenvectors
+
+
+def two_cameras():
+    c1 = np.zeros(18)
+    c2 = np.zeros(18)
+    lib().orc_two_cameras(P(c1), P(c2))
+    return c1, c2
+
+
+def fundamental_from_cameras(c1, c2):
+    F = np.zeros(9)
+    lib().orc_fundamental_from_cameras(P(np.ascontiguousarray(c1)), P(np.ascontiguousarray(c2)),
+                                       P(F))
+    return F.reshape(3, 3)
+
+
+def correspondences(n, sigma, out, seed):
+    x1 = np.zeros((n, 2))
+    x2 = np.zeros((n, 2))
+    t = np.zeros(n, np.uint8)
+    lib().orc_correspondences(n, sigma, out, seed, P(x1), P(x2), P(t))
+    c1, c2 = two_cameras()
+    return x1, x2, t.astype(bool), c1, c2
+
+
+def planar_correspondences(n, sigma, seed):
+    x1 = np.zeros((n, 2))
+    x2 = np.zeros((n, 2))
+    lib().orc_planar_correspondences(n, sigma, seed, P(x1), P(x2))
+    return x1, x2
+
+
+def ground_with_buildings(n, bld, sigma, seed):
+    x1 = np.zeros((n, 2))
+    x2 = np.zeros((n, 2))
+    lib().orc_ground_with_buildings(n, bld, sigma, seed, P(x1), P(x2))
+    return x1, x2
+
+
+def noisy_lines(sigma, seed):
+    l1 = np.zeros((60, 2))
+    l2 = np.zeros((60, 2))
+    lib().orc_noisy_lines(sigma, seed, P(l1), P(l2))
+    return l1, l2
+
+
+def line_plus_general(k, sigma, seed):
+    x1 = np.zeros((60 + k, 2))
+    x2 = np.zeros((60 + k, 2))
+    h1 = np.zeros((100, 2))
+    h2 = np.zeros((100, 2))
+    lib().orc_line_plus_general(k, sigma, seed, P(x1), P(x2), P(h1), P(h2))
+    return x1, x2, h1, h2
