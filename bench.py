@@ -1,0 +1,214 @@
+"""Matched image pairs/sec of the pairwise matching path on one MI355X.
+
+One step = every candidate pair of the full synthetic scene (240 views =
+3 cameras x 80 positions, 1500-4096 unit 128-d descriptors per view, 3663
+pairs from candidate_pairs(5, 4, 16)) through ratio_match(0.8, mutual) ->
+keypoint hand-off -> ransac_fundamental(RansacConfig::default), as one plan
+(two captured hipGraphs) over device-resident inputs.
+
+The timed path ranks distances with BF16 MFMA and re-scores every decision
+its error bound cannot settle in FP32, so its match lists and inlier masks
+equal the exact path's; the plain FP32 GPU path is timed beside it, and the
+CPU oracle (the restated reference algorithm) is timed on a bounded sample of
+the same pairs on the host cores.
+
+Prints one JSON line (see the field notes in the repository documentation).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "skylens-stream-rs_amd"))
+
+METRIC = "matched image pairs/sec (ratio_match+RANSAC), BF16/FP8 vs FP32 GPU, exact matches"
+PEAK = {"bf16": 2500.0, "fp8": 5000.0}  # dense MFMA TFLOP/s, MI355X
+FP32_VALU_PEAK_TOPS = 157.3  # TFLOP/s (FMA = 2); the exact kernel issues 3 separate ops per term
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name):
+    import workloads
+
+    if name == "full":
+        sc = workloads.full_scene(seed=1)
+        desc = ("full scene: 240 views (3 cameras x 80 positions), features/view U[1500,4096], "
+                "60% noisy copies of a 20k pool, 3663 candidate pairs, ratio 0.8 mutual, "
+                "ransac_fundamental default config, pair outlier ratio 20-60%")
+    elif name == "uneven":
+        sc = workloads.uneven_scene(seed=4)
+        desc = ("uneven batch: 264 views, features/view log-uniform [200,8000], first 4000 "
+                "candidate pairs, inlier ratio 10-90%")
+    elif name == "fixed":
+        sc = workloads.fixed_scene(seed=3)
+        desc = "fixed scene: 240 views x 8192 features, 3663 candidate pairs"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return sc, desc
+
+
+def cpu_baseline(sc, pairs_idx, threads):
+    """The oracle (C restatement of the reference's CPU path) on a sample of
+    pairs: ratio_match then ransac_fundamental per pair, pairs spread over
+    `threads` host threads (ctypes releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    def one(p):
+        i, j = (int(v) for v in sc.pairs[p])
+        m = ob.ratio_match(sc.view_desc(i), sc.view_desc(j), 0.8, True)
+        mm = np.array(m, dtype=np.int64).reshape(-1, 2)
+        x1 = sc.view_kp(i)[mm[:, 0]].astype(np.float64)
+        x2 = sc.view_kp(j)[mm[:, 1]].astype(np.float64)
+        return ob.ransac_fundamental(x1, x2) is not None
+
+    t0 = time.time()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, pairs_idx))
+    return time.time() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="full", choices=["full", "uneven", "fixed"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is measured on "
+              "one MI355X and its multi-GPU sharding is a design note, not a run.", flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import skylens_hip
+
+    t0 = time.time()
+    sc, wl_desc = build_workload(args.workload)
+    P = len(sc.pairs)
+    na = np.array([sc.offsets[i + 1] - sc.offsets[i] for i, _ in sc.pairs], np.float64)
+    nb = np.array([sc.offsets[j + 1] - sc.offsets[j] for _, j in sc.pairs], np.float64)
+    gemm_flop = float(np.sum(2.0 * 128.0 * na * nb))
+    log(f"workload {args.workload}: {len(sc.views)} views, {P} pairs, "
+        f"{sc.offsets[-1]} features, built in {time.time() - t0:.1f}s")
+
+    ctx = skylens_hip.Context(0)
+    d_desc = torch.from_numpy(sc.desc).cuda()
+    d_kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
+    torch.cuda.synchronize()
+
+    def make_plan(prec):
+        p = skylens_hip.Plan(ctx, sc.offsets, d_desc.data_ptr(), d_kp.data_ptr(), sc.pairs, 0.8,
+                             True, prec, True, 0.0)
+        p.set_timing(True)
+        return p
+
+    def timed(plan, steps, warmup):
+        for _ in range(warmup):
+            plan.run()
+        plan.sync()
+        torch.cuda.synchronize()
+        kern, match_ms, ransac_ms = [], [], []
+        t = time.perf_counter()
+        for _ in range(steps):
+            plan.run()
+            plan.sync()
+            kern.append(plan.stage_ms(3))
+            match_ms.append(plan.stage_ms(1))
+            ransac_ms.append(plan.stage_ms(2))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        return el, float(np.mean(kern)), float(np.mean(match_ms)), float(np.mean(ransac_ms))
+
+    plan = make_plan(args.precision)
+    log(f"plan {args.precision} built")
+    el, kern_ms, match_ms, ransac_ms = timed(plan, args.steps, args.warmup)
+    res = plan.results()
+    ms_step = el / args.steps * 1e3
+    value = P * args.steps / el
+    n_found = int(np.sum(res.found))
+    n_matches = int(np.sum(res.match_len))
+    log(f"{args.precision}: {ms_step:.2f} ms/step ({match_ms:.2f} match, {ransac_ms:.2f} ransac, "
+        f"distance kernel {kern_ms:.2f}), {value:.0f} pairs/s, {n_matches} matches, "
+        f"{n_found}/{P} verified")
+
+    extra = {}
+    if args.precision != "fp32" and args.fp32_steps > 0:
+        plan32 = make_plan("fp32")
+        el32, k32, m32, r32 = timed(plan32, args.fp32_steps, 1)
+        r32res = plan32.results()
+        same = (np.array_equal(r32res.match_len, res.match_len)
+                and all(np.array_equal(r32res.matches(p), res.matches(p)) for p in range(P))
+                and np.array_equal(r32res.found, res.found)
+                and np.array_equal(r32res.F.view(np.uint64), res.F.view(np.uint64)))
+        v32 = P * args.fp32_steps / el32
+        extra = {"fp32_gpu_pairs_per_s": round(v32, 2), "speedup_vs_fp32_gpu": round(value / v32, 3),
+                 "fp32_ms_per_step": round(el32 / args.fp32_steps * 1e3, 2),
+                 "fp32_distance_kernel_ms": round(k32, 3),
+                 "identical_to_fp32_gpu": bool(same)}
+        log(f"fp32: {el32 / args.fp32_steps * 1e3:.2f} ms/step ({m32:.2f} match, {r32:.2f} ransac, "
+            f"exact kernel {k32:.2f}), {v32:.0f} pairs/s, identical={same}")
+        plan32.close()
+
+    achieved = gemm_flop / (kern_ms * 1e-3) / 1e12
+    peak = PEAK.get(args.precision, FP32_VALU_PEAK_TOPS)
+    roofline = {"bound": "mfma" if args.precision in PEAK else "valu", "kernel":
+                "mfma_rank_kernel" if args.precision in PEAK else "exact_state_kernel",
+                "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "flop_per_launch": gemm_flop, "launch_ms": round(kern_ms, 4)}
+    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.precision}.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(0)
+        # size the sample to ~cpu_seconds: 4 pairs to calibrate, then scale
+        cal = rng.choice(P, size=min(P, 2 * threads), replace=False)
+        t_cal = cpu_baseline(sc, cal, threads)
+        per_pair = t_cal / len(cal)
+        k = int(max(len(cal), min(P, args.cpu_seconds / max(per_pair, 1e-6))))
+        sample = rng.choice(P, size=k, replace=False)
+        t_cpu = cpu_baseline(sc, sample, threads)
+        cpu = {"value": round(k / t_cpu, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
+               "sample": f"{k} of the {P} pairs, the same synthetic data, pairs over {threads} "
+                         f"threads ({t_cpu:.1f} s)"}
+        log(f"cpu oracle: {k / t_cpu:.2f} pairs/s on {threads} threads")
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": wl_desc, "pairs": P, "views": len(sc.views),
+                   "features": int(sc.offsets[-1]), "precision": args.precision,
+                   "matches": n_matches, "verified_pairs": n_found,
+                   "match_ms": round(match_ms, 3), "ransac_ms": round(ransac_ms, 3), **extra},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -96,6 +96,98 @@ int skh_ratio_match_state(skh_ctx* ctx, const float* a, size_t na, const float* 
                           uint32_t* row_j1, float* row_d1, float* row_d2, uint32_t* col_i,
                           float* col_d);
 
+/* RansacConfig (matching.rs:762-793). */
+typedef struct {
+    double threshold_px;     /* inlier Sampson distance threshold (px, not squared) */
+    double min_inlier_ratio; /* below this inlier ratio -> None */
+    uint64_t max_iters;
+    double confidence;       /* adaptive termination confidence */
+    uint64_t seed;
+    uint64_t min_inliers;
+    double max_chance_prob;  /* <= 0 disables the significance gate */
+} skh_ransac_cfg;
+
+/* ModelSelection (matching.rs:380-394) plus a presence flag (the reference's
+ * Option). model: 0 = Fundamental, 1 = Homography. */
+typedef struct {
+    int32_t has_selection;
+    int32_t model;
+    double gric_f, gric_h;
+    uint64_t parallax, parallax_needed;
+    double sigma_px;
+} skh_model_sel;
+
+/* RansacConfig::default() (matching.rs:781-793): 1.5 px, 0.2, 2000, 0.999,
+ * seed 1, 15 inliers, 1e-6. */
+void skh_ransac_cfg_default(skh_ransac_cfg* cfg);
+
+/* ransac_fundamental (matching.rs:941-1104) on host buffers.
+ * x1, x2: n x 2 f64 (Vector2<f64> slices, nalgebra column layout [x, y]).
+ * On Some: *found = 1, F_rowmajor = F (unit Frobenius norm, rank 2; the Rust
+ * wrapper builds Matrix3::new from the row-major values), inliers[k] = 1 for
+ * inliers (n bytes), *sel = the model selection (has_selection = 0 for the
+ * reference's selection: None). On None: *found = 0. Blocks until done. */
+int skh_ransac_fundamental(skh_ctx* ctx, const double* x1, const double* x2, size_t n,
+                           const skh_ransac_cfg* cfg, double* F_rowmajor, uint8_t* inliers,
+                           skh_model_sel* sel, int* found);
+
+/* Hypotheses drawn by the context's last skh_ransac_fundamental call
+ * (diagnostics; the reference's adaptive stopping point). */
+uint64_t skh_ctx_last_ransac_iters(skh_ctx* ctx);
+
+/* ---- batched per-pair driver (benches/pipeline.rs:219-305) ----
+ * All pairs of a scene: ratio_match, the Feature -> pixel hand-off and
+ * ransac_fundamental, as one captured hipGraph on the context stream. The
+ * plan fixes the views, the pair list and the parameters; every run
+ * recomputes everything from the device-resident descriptors and keypoints.
+ *
+ * views: n_views feature ranges [view_offset[v], view_offset[v+1]) into
+ *   d_desc (total x 128 f32) and d_kp (total x 2 f32, Keypoint.x/.y), both
+ *   device pointers that must stay valid for the plan's lifetime.
+ * pairs: n_pairs (i, j) view indices (host), e.g. skh_candidate_pairs output.
+ * Each pair runs ratio_match(view i, view j, ratio, mutual) and, on its match
+ * list, ransac_fundamental with x1 = kp_i + px_offset, x2 = kp_j + px_offset
+ * (benches/pipeline.rs:276-283 uses 0.0; README.md:376-379 uses 0.5). */
+typedef struct {
+    uint32_t n_views;
+    const uint64_t* view_offset; /* host, n_views + 1 */
+    const float* d_desc;         /* device */
+    const float* d_kp;           /* device */
+    uint32_t n_pairs;
+    const uint32_t* pairs;       /* host, 2 * n_pairs */
+    float ratio;
+    int32_t mutual;
+    int32_t precision;           /* SKH_PREC_* */
+    int32_t run_ransac;          /* 0: matching only */
+    double px_offset;
+    skh_ransac_cfg ransac;
+} skh_batch_desc;
+
+typedef struct skh_plan skh_plan;
+
+int skh_plan_create(skh_ctx* ctx, const skh_batch_desc* desc, skh_plan** out);
+int skh_plan_destroy(skh_plan* plan);
+/* Enqueues one run (graph launch) on the context stream; asynchronous. */
+int skh_plan_run(skh_plan* plan);
+/* Waits for the plan's runs to finish. */
+int skh_plan_sync(skh_plan* plan);
+/* Per-pair match capacity offsets: match_base[p] (host, n_pairs + 1). The
+ * matches of pair p live at [match_base[p], match_base[p] + match_len[p]). */
+int skh_plan_match_base(skh_plan* plan, uint64_t* match_base);
+/* Copies results to host buffers (any may be NULL): match_len (n_pairs),
+ * match_ij (2 * match_base[n_pairs] uint32), inliers (match_base[n_pairs]
+ * bytes, aligned with match_ij), F (9 * n_pairs, row-major), found (n_pairs:
+ * 1 = Some, 0 = None), sel (n_pairs). Synchronises first. */
+int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* match_ij, uint8_t* inliers,
+                     double* F, int32_t* found, skh_model_sel* sel);
+/* Device time (ms) of the named stage in the plan's last run, measured with
+ * HIP events on the context stream when timing is enabled: stage 0 = whole
+ * run, 1 = matching, 2 = RANSAC, 3 = the distance kernel alone (MFMA ranking
+ * or the exact FP32 kernel). Returns -1 if unavailable. */
+double skh_plan_stage_ms(skh_plan* plan, int stage);
+/* 1 = record stage events in each run (costs nothing measurable), 0 = off. */
+int skh_plan_set_timing(skh_plan* plan, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,444 @@
+// Batched per-pair driver: all candidate pairs of a scene in one captured
+// hipGraph (the MI355X counterpart of benches/pipeline.rs:219-305, which runs
+// ratio_match pair by pair and ransac_fundamental over pairs in parallel).
+//
+// Scheduling: every pair contributes ceil(na / BM) row-block workgroups to the
+// distance kernel. Pairs are assigned to the 8 XCDs longest-first (greedy
+// LPT on na * nb), and the launch order interleaves the per-XCD lists so that
+// workgroup b lands on XCD b % 8 (the hardware's round-robin dispatch): all
+// row blocks of a pair share one XCD's L2, where the pair's b descriptors
+// stay resident. RANSAC runs one 64-lane workgroup per pair.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "skh_ctx.hpp"
+#include "skh_lowp.hpp"
+#include "skh_ransac.hpp"
+#include "skylens_hip.h"
+
+namespace skh {
+namespace {
+
+constexpr int kXcds = 8;
+
+// Gathers each pair's matched keypoints into the RANSAC input
+// (benches/pipeline.rs:276-283: f32 Keypoint coordinates widened to f64,
+// plus the caller's pixel offset).
+__global__ void handoff_kernel(const uint32_t* __restrict__ match_ij,
+                               const uint32_t* __restrict__ match_len,
+                               const uint64_t* __restrict__ match_base,
+                               const uint64_t* __restrict__ kp_off_a,
+                               const uint64_t* __restrict__ kp_off_b, const float2* __restrict__ kp,
+                               double px_offset, double2* __restrict__ x1, double2* __restrict__ x2) {
+    const uint32_t p = blockIdx.x;
+    const uint64_t base = match_base[p];
+    const uint32_t len = match_len[p];
+    for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
+        const uint32_t i = match_ij[2 * (base + k)], j = match_ij[2 * (base + k) + 1];
+        const float2 a = kp[kp_off_a[p] + i], b = kp[kp_off_b[p] + j];
+        x1[base + k] = make_double2((double)a.x + px_offset, (double)a.y + px_offset);
+        x2[base + k] = make_double2((double)b.x + px_offset, (double)b.y + px_offset);
+    }
+}
+
+}  // namespace
+
+struct Plan {
+    Ctx* ctx = nullptr;
+    skh_batch_desc desc{};
+    std::vector<uint64_t> view_off;
+    std::vector<uint32_t> pairs;
+    std::vector<uint64_t> match_base;  // n_pairs + 1
+    int tile = kTile128x128;
+    uint32_t tbits = 1;
+    bool lowp = false;
+    uint32_t n_blocks = 0;
+    uint64_t total_rows = 0, total_cols = 0, total_parts = 0, cap_total = 0;
+    uint32_t max_view_rows = 0;
+
+    DevBuf d_view_off, d_tasks_exact, d_tasks_lowp, d_blocks, d_stats, d_q, d_nrm;
+    DevBuf d_row_k1, d_row_j1, d_row_k2, d_row_d1, d_row_d2, d_col_key, d_col_part, d_col_top2;
+    DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
+        d_kp_b;
+    DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters;
+
+    hipGraph_t g_match = nullptr, g_ransac = nullptr;
+    hipGraphExec_t x_match = nullptr, x_ransac = nullptr;
+    // ev[0..2]: run start, matching end, RANSAC end (recorded between graph
+    // launches); ev[3..4]: around the distance kernel, recorded inside the graph.
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool timing = false;
+    bool timed_run = false;
+
+    std::vector<DevBuf*> bufs() {
+        return {&d_view_off, &d_tasks_exact, &d_tasks_lowp, &d_blocks, &d_stats,  &d_q,
+                &d_nrm,      &d_row_k1,      &d_row_j1,     &d_row_k2, &d_row_d1, &d_row_d2,
+                &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len,
+                &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
+                &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
+                &d_rtasks,   &d_iters};
+    }
+
+    ~Plan() {
+        if (x_match) (void)hipGraphExecDestroy(x_match);
+        if (x_ransac) (void)hipGraphExecDestroy(x_ransac);
+        if (g_match) (void)hipGraphDestroy(g_match);
+        if (g_ransac) (void)hipGraphDestroy(g_ransac);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (DevBuf* b : bufs())
+            if (b->ptr) (void)hipFree(b->ptr);
+    }
+
+    int alloc(DevBuf& b, size_t bytes, const char* what) {
+        bytes = bytes ? bytes : 16;
+        hipError_t e = hipMalloc(&b.ptr, bytes);
+        if (e != hipSuccess) return ctx->hip(e, what);
+        b.cap = bytes;
+        return SKH_OK;
+    }
+
+    int upload(DevBuf& b, const void* src, size_t bytes, const char* what) {
+        int rc = alloc(b, bytes, what);
+        if (rc) return rc;
+        if (!bytes) return SKH_OK;
+        return ctx->hip(hipMemcpy(b.ptr, src, bytes, hipMemcpyHostToDevice), what);
+    }
+
+    uint32_t rows(uint32_t v) const { return (uint32_t)(view_off[v + 1] - view_off[v]); }
+
+    int build();
+    void enqueue_match(hipStream_t s);
+    void enqueue_ransac(hipStream_t s);
+};
+
+int Plan::build() {
+    const uint32_t P = desc.n_pairs, V = desc.n_views;
+    lowp = desc.precision == SKH_PREC_BF16;
+    for (uint32_t v = 0; v < V; ++v) max_view_rows = std::max(max_view_rows, rows(v));
+    if (lowp && max_view_rows > 131072) lowp = false;  // tile ids would erode the key precision
+    tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile128x128;
+    const uint32_t bm = lowp ? lowp_block_rows(tile) : 64u;
+    tbits = tile_bits(max_view_rows);
+
+    // per-pair bases
+    std::vector<PairTask> pt(P);
+    std::vector<LowpTask> lt(P);
+    std::vector<uint64_t> kp_a(P), kp_b(P);
+    match_base.assign(P + 1, 0);
+    uint64_t rb = 0, cb = 0, pb = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+        const uint32_t na = rows(i), nb = rows(j);
+        const float* a = desc.d_desc + view_off[i] * kDescLen;
+        const float* b = desc.d_desc + view_off[j] * kDescLen;
+        pt[p] = PairTask{a, b, na, nb, (uint32_t)rb, (uint32_t)cb};
+        LowpTask& l = lt[p];
+        l.a = a;
+        l.b = b;
+        l.na = na;
+        l.nb = nb;
+        l.row_base = (uint32_t)rb;
+        l.col_base = (uint32_t)cb;
+        l.part_base = pb;
+        kp_a[p] = view_off[i];
+        kp_b[p] = view_off[j];
+        rb += na;
+        cb += nb;
+        pb += (uint64_t)((na + bm - 1) / bm) * nb;
+        const uint32_t cap = desc.mutual ? std::min(na, nb) : na;
+        match_base[p + 1] = match_base[p] + cap;
+    }
+    total_rows = rb;
+    total_cols = cb;
+    total_parts = pb;
+    cap_total = match_base[P];
+    if (rb >= 0xffffffffull || cb >= 0xffffffffull)
+        return ctx->fail(SKH_EINVAL, "batch too large for 32-bit row/column bases");
+
+    int rc;
+    // quantised views (lowp)
+    if ((rc = upload(d_view_off, view_off.data(), 8ull * (V + 1), "view_off"))) return rc;
+    const uint64_t total_feat = view_off[V];
+    if (lowp) {
+        if ((rc = alloc(d_q, 2ull * kDescLen * total_feat, "q")) ||
+            (rc = alloc(d_nrm, 4ull * total_feat, "nrm")) ||
+            (rc = alloc(d_stats, sizeof(ViewStats) * V, "stats")))
+            return rc;
+        for (uint32_t p = 0; p < P; ++p) {
+            const uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+            lt[p].qa = d_q.as<uint16_t>() + view_off[i] * kDescLen;
+            lt[p].qb = d_q.as<uint16_t>() + view_off[j] * kDescLen;
+            lt[p].nrm_a = d_nrm.as<uint32_t>() + view_off[i];
+            lt[p].nrm_b = d_nrm.as<uint32_t>() + view_off[j];
+            lt[p].sa = d_stats.as<ViewStats>() + i;
+            lt[p].sb = d_stats.as<ViewStats>() + j;
+        }
+    }
+
+    // XCD-aware, longest-first row-block order
+    std::vector<uint32_t> order(P);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return (uint64_t)pt[x].na * pt[x].nb > (uint64_t)pt[y].na * pt[y].nb;
+    });
+    std::vector<std::vector<RowBlock>> lists(kXcds);
+    std::vector<uint64_t> load(kXcds, 0);
+    for (uint32_t p : order) {
+        if (pt[p].na == 0) continue;
+        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += (uint64_t)pt[p].na * std::max<uint32_t>(pt[p].nb, 1);
+        for (uint32_t r0 = 0; r0 < pt[p].na; r0 += bm) lists[x].push_back(RowBlock{p, r0});
+    }
+    size_t longest = 0;
+    for (auto& l : lists) longest = std::max(longest, l.size());
+    std::vector<RowBlock> blocks;
+    blocks.reserve(longest * kXcds);
+    for (size_t s = 0; s < longest; ++s)
+        for (int x = 0; x < kXcds; ++x)
+            blocks.push_back(s < lists[x].size() ? lists[x][s] : RowBlock{kNone, 0u});
+    n_blocks = (uint32_t)blocks.size();
+
+    if ((rc = upload(d_tasks_exact, pt.data(), sizeof(PairTask) * P, "tasks")) ||
+        (rc = upload(d_tasks_lowp, lt.data(), sizeof(LowpTask) * P, "ltasks")) ||
+        (rc = upload(d_blocks, blocks.data(), sizeof(RowBlock) * blocks.size(), "blocks")) ||
+        (rc = alloc(d_row_j1, 4ull * rb, "row_j1")) ||
+        (rc = alloc(d_row_match, 4ull * rb, "row_match")) ||
+        (rc = upload(d_match_base, match_base.data(), 8ull * (P + 1), "match_base")) ||
+        (rc = alloc(d_match_ij, 8ull * cap_total, "match_ij")) ||
+        (rc = alloc(d_match_len, 4ull * P, "match_len")) ||
+        (rc = upload(d_kp_a, kp_a.data(), 8ull * P, "kp_a")) ||
+        (rc = upload(d_kp_b, kp_b.data(), 8ull * P, "kp_b")))
+        return rc;
+    if (lowp) {
+        if ((rc = alloc(d_row_k1, 4ull * rb, "row_k1")) || (rc = alloc(d_row_k2, 4ull * rb, "row_k2")) ||
+            (rc = alloc(d_col_part, 8ull * pb, "col_part")) ||
+            (rc = alloc(d_col_top2, 8ull * cb, "col_top2")) ||
+            (rc = alloc(d_queue, 16ull * rb, "queue")) || (rc = alloc(d_queue_len, 4, "queue_len")))
+            return rc;
+    } else {
+        if ((rc = alloc(d_row_d1, 4ull * rb, "row_d1")) || (rc = alloc(d_row_d2, 4ull * rb, "row_d2")) ||
+            (rc = alloc(d_col_key, 8ull * cb, "col_key")))
+            return rc;
+    }
+
+    if (desc.run_ransac) {
+        std::vector<RansacTask> rt(P);
+        std::vector<uint64_t> sbase(P + 1, 0);
+        for (uint32_t p = 0; p < P; ++p) {
+            const uint32_t cap = (uint32_t)(match_base[p + 1] - match_base[p]);
+            sbase[p + 1] = sbase[p] + ransac_scratch_words(std::max<uint32_t>(cap, 1));
+        }
+        if ((rc = alloc(d_x1, 16ull * cap_total, "x1")) || (rc = alloc(d_x2, 16ull * cap_total, "x2")) ||
+            (rc = alloc(d_inl, cap_total, "inliers")) || (rc = alloc(d_F, 72ull * P, "F")) ||
+            (rc = alloc(d_found, 4ull * P, "found")) ||
+            (rc = alloc(d_sel, sizeof(skh_model_sel) * P, "sel")) ||
+            (rc = alloc(d_scratch, 4ull * sbase[P], "scratch")) ||
+            (rc = alloc(d_iters, 8ull * P, "iters")))
+            return rc;
+        for (uint32_t p = 0; p < P; ++p) {
+            RansacTask& t = rt[p];
+            t.x1 = d_x1.as<double2>() + match_base[p];
+            t.x2 = d_x2.as<double2>() + match_base[p];
+            t.n = (uint32_t)(match_base[p + 1] - match_base[p]);
+            t.F = d_F.as<double>() + 9ull * p;
+            t.inliers = d_inl.as<uint8_t>() + match_base[p];
+            t.found = d_found.as<int32_t>() + p;
+            t.sel = d_sel.as<skh_model_sel>() + p;
+            t.scratch = d_scratch.as<uint32_t>() + sbase[p];
+            t.iters_out = d_iters.as<uint64_t>() + p;
+            t.n_dev = d_match_len.as<uint32_t>() + p;
+        }
+        if ((rc = upload(d_rtasks, rt.data(), sizeof(RansacTask) * P, "rtasks"))) return rc;
+    }
+
+    for (hipEvent_t& e : ev)
+        if ((rc = ctx->hip(hipEventCreate(&e), "event"))) return rc;
+    // Capture the two stage graphs.
+    hipStream_t s = ctx->stream;
+    if ((rc = ctx->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
+        return rc;
+    enqueue_match(s);
+    if ((rc = ctx->hip(hipStreamEndCapture(s, &g_match), "end capture")) ||
+        (rc = ctx->hip(hipGraphInstantiate(&x_match, g_match, nullptr, nullptr, 0), "instantiate")))
+        return rc;
+    if (desc.run_ransac) {
+        if ((rc = ctx->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
+            return rc;
+        enqueue_ransac(s);
+        if ((rc = ctx->hip(hipStreamEndCapture(s, &g_ransac), "end capture")) ||
+            (rc = ctx->hip(hipGraphInstantiate(&x_ransac, g_ransac, nullptr, nullptr, 0),
+                           "instantiate")))
+            return rc;
+    }
+    return SKH_OK;
+}
+
+void Plan::enqueue_match(hipStream_t s) {
+    const uint32_t P = desc.n_pairs;
+    if (lowp) {
+        LowpOut o{};
+        o.row_k1 = d_row_k1.as<float>();
+        o.row_j1 = d_row_j1.as<uint32_t>();
+        o.row_k2 = d_row_k2.as<float>();
+        o.col_part = d_col_part.as<float2>();
+        o.col_top2 = d_col_top2.as<float2>();
+        o.row_match = d_row_match.as<uint32_t>();
+        o.queue = d_queue.as<uint4>();
+        o.queue_len = d_queue_len.as<uint32_t>();
+        launch_stats_reset(d_stats.as<ViewStats>(), desc.n_views, s);
+        launch_fill_u32(o.queue_len, 1, 0u, s);
+        launch_quantize_views(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views, max_view_rows,
+                              d_q.as<uint16_t>(), d_nrm.as<uint32_t>(), d_stats.as<ViewStats>(), s);
+        const LowpTask* lt = d_tasks_lowp.as<LowpTask>();
+        (void)hipEventRecord(ev[3], s);
+        launch_mfma_rank(tile, lt, d_blocks.as<RowBlock>(), n_blocks, o, tbits, s);
+        (void)hipEventRecord(ev[4], s);
+        launch_colmerge(lt, P, o, lowp_block_rows(tile), s);
+        launch_decide(lt, P, o, desc.ratio, desc.mutual, tbits, s);
+        launch_rescore(lt, o, desc.ratio, desc.mutual, tbits, (uint32_t)std::max(64, 4 * ctx->num_cu),
+                       s);
+    } else {
+        launch_fill_u64(d_col_key.as<unsigned long long>(), total_cols, kColEmpty, s);
+        (void)hipEventRecord(ev[3], s);
+        launch_exact_state(d_tasks_exact.as<PairTask>(), d_blocks.as<RowBlock>(), n_blocks,
+                           d_row_j1.as<uint32_t>(), d_row_d1.as<float>(), d_row_d2.as<float>(),
+                           d_col_key.as<unsigned long long>(), s);
+        (void)hipEventRecord(ev[4], s);
+        launch_exact_decide(d_tasks_exact.as<PairTask>(), P, d_row_j1.as<uint32_t>(),
+                            d_row_d1.as<float>(), d_row_d2.as<float>(),
+                            d_col_key.as<unsigned long long>(), desc.ratio, desc.mutual,
+                            d_row_match.as<uint32_t>(), s);
+    }
+    // out_base of the compaction = match_base (uint64 -> the kernel takes uint32 bases)
+    launch_compact(d_tasks_exact.as<PairTask>(), P, d_row_match.as<uint32_t>(),
+                          d_match_base.as<uint64_t>(), d_match_ij.as<uint32_t>(),
+                          d_match_len.as<uint32_t>(), s);
+}
+
+void Plan::enqueue_ransac(hipStream_t s) {
+    const uint32_t P = desc.n_pairs;
+    if (!P) return;
+    hipLaunchKernelGGL(handoff_kernel, dim3(P), dim3(256), 0, s, d_match_ij.as<uint32_t>(),
+                       d_match_len.as<uint32_t>(), d_match_base.as<uint64_t>(),
+                       d_kp_a.as<uint64_t>(), d_kp_b.as<uint64_t>(),
+                       reinterpret_cast<const float2*>(desc.d_kp), desc.px_offset,
+                       d_x1.as<double2>(), d_x2.as<double2>());
+    launch_ransac(d_rtasks.as<RansacTask>(), P, desc.ransac, s);
+}
+
+}  // namespace skh
+
+using namespace skh;
+
+struct skh_plan : skh::Plan {};
+
+extern "C" int skh_plan_create(skh_ctx* ctx, const skh_batch_desc* desc, skh_plan** out) {
+    if (!ctx || !desc || !out) return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *out = nullptr;
+    if (!desc->view_offset || (!desc->pairs && desc->n_pairs) || !desc->d_desc ||
+        (desc->run_ransac && !desc->d_kp) || !skh_precision_supported(desc->precision))
+        return ctx->fail(SKH_EINVAL, "skh_plan_create: invalid argument");
+    if (desc->precision == SKH_PREC_FP8)
+        return ctx->fail(SKH_EINVAL, "skh_plan_create: FP8 batches are not built");
+    for (uint32_t v = 0; v < desc->n_views; ++v)
+        if (desc->view_offset[v + 1] < desc->view_offset[v])
+            return ctx->fail(SKH_EINVAL, "skh_plan_create: view offsets must not decrease");
+    for (uint32_t p = 0; p < 2 * desc->n_pairs; ++p)
+        if (desc->pairs[p] >= desc->n_views)
+            return ctx->fail(SKH_EINVAL, "skh_plan_create: pair %u names a missing view", p / 2);
+    if (hipSetDevice(ctx->device) != hipSuccess) return SKH_ENODEV;
+    skh_plan* plan = new skh_plan();
+    plan->ctx = ctx;
+    plan->desc = *desc;
+    plan->view_off.assign(desc->view_offset, desc->view_offset + desc->n_views + 1);
+    plan->pairs.assign(desc->pairs, desc->pairs + 2 * desc->n_pairs);
+    plan->desc.view_offset = nullptr;
+    plan->desc.pairs = nullptr;
+    int rc = plan->build();
+    if (rc) {
+        delete plan;
+        return rc;
+    }
+    *out = plan;
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_destroy(skh_plan* plan) {
+    if (!plan) return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(plan->ctx->mu);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    delete plan;
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_run(skh_plan* plan) {
+    if (!plan) return SKH_EINVAL;
+    Ctx* ctx = plan->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc;
+    hipStream_t s = ctx->stream;
+    const bool t = plan->timing;
+    if (t && (rc = ctx->hip(hipEventRecord(plan->ev[0], s), "event"))) return rc;
+    if ((rc = ctx->hip(hipGraphLaunch(plan->x_match, s), "graph launch"))) return rc;
+    if (t && (rc = ctx->hip(hipEventRecord(plan->ev[1], s), "event"))) return rc;
+    if (plan->x_ransac && (rc = ctx->hip(hipGraphLaunch(plan->x_ransac, s), "graph launch")))
+        return rc;
+    if (t && (rc = ctx->hip(hipEventRecord(plan->ev[2], s), "event"))) return rc;
+    plan->timed_run = t;
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_sync(skh_plan* plan) {
+    if (!plan) return SKH_EINVAL;
+    return plan->ctx->hip(hipStreamSynchronize(plan->ctx->stream), "sync");
+}
+
+extern "C" int skh_plan_match_base(skh_plan* plan, uint64_t* match_base) {
+    if (!plan || !match_base) return SKH_EINVAL;
+    std::memcpy(match_base, plan->match_base.data(), 8 * plan->match_base.size());
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* match_ij,
+                                uint8_t* inliers, double* F, int32_t* found, skh_model_sel* sel) {
+    if (!plan) return SKH_EINVAL;
+    Ctx* ctx = plan->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc;
+    if ((rc = ctx->hip(hipStreamSynchronize(ctx->stream), "sync"))) return rc;
+    const uint32_t P = plan->desc.n_pairs;
+    auto get = [&](void* dst, const DevBuf& b, size_t bytes, const char* what) -> int {
+        if (!dst || !bytes) return SKH_OK;
+        if (!b.ptr) return ctx->fail(SKH_EINVAL, "%s: not computed by this plan", what);
+        return ctx->hip(hipMemcpy(dst, b.ptr, bytes, hipMemcpyDeviceToHost), what);
+    };
+    if ((rc = get(match_len, plan->d_match_len, 4ull * P, "match_len")) ||
+        (rc = get(match_ij, plan->d_match_ij, 8ull * plan->cap_total, "match_ij")) ||
+        (rc = get(inliers, plan->d_inl, plan->cap_total, "inliers")) ||
+        (rc = get(F, plan->d_F, 72ull * P, "F")) || (rc = get(found, plan->d_found, 4ull * P, "found")) ||
+        (rc = get(sel, plan->d_sel, sizeof(skh_model_sel) * P, "sel")))
+        return rc;
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_set_timing(skh_plan* plan, int on) {
+    if (!plan) return SKH_EINVAL;
+    plan->timing = on != 0;
+    return SKH_OK;
+}
+
+extern "C" double skh_plan_stage_ms(skh_plan* plan, int stage) {
+    if (!plan || stage < 0 || stage > 3) return -1.0;
+    if (stage < 3 && !plan->timed_run) return -1.0;
+    float ms = 0.f;
+    hipEvent_t a = stage == 3 ? plan->ev[3] : (stage == 2 ? plan->ev[1] : plan->ev[0]);
+    hipEvent_t b = stage == 3 ? plan->ev[4] : (stage == 1 ? plan->ev[1] : plan->ev[2]);
+    if (hipEventSynchronize(b) != hipSuccess) return -1.0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+    return ms;
+}

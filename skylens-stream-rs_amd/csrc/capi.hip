@@ -14,8 +14,6 @@
 
 using namespace skh;
 
-struct skh_ctx : skh::Ctx {};
-
 namespace skh {
 
 int Ctx::fail(int code, const char* fmt, ...) {
@@ -96,13 +94,13 @@ int Ctx::ratio_match_dev(const float* d_a, size_t na, const float* d_b, size_t n
     if (precision == SKH_PREC_FP32) {
         if ((rc = exact_state_single(d_a, na, d_b, nb))) return rc;
         if ((rc = ensure(row_match, sizeof(uint32_t) * (na ? na : 1), "row_match")) ||
-            (rc = ensure(out_base, sizeof(uint32_t), "out_base")))
+            (rc = ensure(out_base, sizeof(uint64_t), "out_base")))
             return rc;
-        launch_fill_u32(out_base.as<uint32_t>(), 1, 0u, stream);
+        launch_fill_u64(reinterpret_cast<unsigned long long*>(out_base.ptr), 1, 0ull, stream);
         launch_exact_decide(tasks.as<PairTask>(), 1, row_j1.as<uint32_t>(), row_d1.as<float>(),
                             row_d2.as<float>(), col_key.as<unsigned long long>(), ratio, mutual,
                             row_match.as<uint32_t>(), stream);
-        launch_compact(tasks.as<PairTask>(), 1, row_match.as<uint32_t>(), out_base.as<uint32_t>(),
+        launch_compact(tasks.as<PairTask>(), 1, row_match.as<uint32_t>(), out_base.as<uint64_t>(),
                        d_out_ij, d_out_len, stream);
         return hip(hipGetLastError(), "launch decide/compact");
     }

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(NT) void exact_state_kernel(const PairTask* __restr
     __shared__ float Bs[kDescLen][BN];
 
     const RowBlock blk = blocks[blockIdx.x];
+    if (blk.pair == kNone) return;  // padding of the XCD-interleaved work list
     const PairTask t = tasks[blk.pair];
     const int tid = threadIdx.x;
     const int tx = tid & 15;  // column group: cols tx*4 .. tx*4+3 of a chunk
@@ -236,7 +237,7 @@ __global__ void exact_decide_kernel(const PairTask* __restrict__ tasks, const ui
 // One workgroup per pair: ballot-based stream compaction in ascending i.
 __global__ __launch_bounds__(256) void compact_kernel(const PairTask* __restrict__ tasks,
                                                       const uint32_t* __restrict__ row_match,
-                                                      const uint32_t* __restrict__ out_base,
+                                                      const uint64_t* __restrict__ out_base,
                                                       uint32_t* __restrict__ out_ij,
                                                       uint32_t* __restrict__ out_len) {
     __shared__ uint32_t wave_cnt[4];
@@ -300,7 +301,7 @@ void launch_exact_decide(const PairTask* d_tasks, uint32_t n_pairs, const uint32
 }
 
 void launch_compact(const PairTask* d_tasks, uint32_t n_pairs, const uint32_t* row_match,
-                    const uint32_t* out_base, uint32_t* out_ij, uint32_t* out_len,
+                    const uint64_t* out_base, uint32_t* out_ij, uint32_t* out_len,
                     hipStream_t s) {
     if (n_pairs == 0) return;
     hipLaunchKernelGGL(compact_kernel, dim3(n_pairs), dim3(256), 0, s, d_tasks, row_match,

@@ -57,11 +57,27 @@ __device__ __forceinline__ void atomic_max_pos(float* p, float v) {
     atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
 }
 
+__device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* __restrict__ q,
+                             uint32_t* __restrict__ nrm, ViewStats* __restrict__ st);
+
 // One thread per row: bf16 quantisation, packed norm, bound statistics.
 __global__ void quantize_kernel(const float* __restrict__ x, uint32_t n, uint16_t* __restrict__ q,
                                 uint32_t* __restrict__ nrm, ViewStats* __restrict__ st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i < n) quantize_row(x, i, q, nrm, st);
+}
+
+// All views of a scene: blockIdx.y = view, rows [off[v], off[v+1]).
+__global__ void quantize_views_kernel(const float* __restrict__ x, const uint64_t* __restrict__ off,
+                                      uint16_t* __restrict__ q, uint32_t* __restrict__ nrm,
+                                      ViewStats* __restrict__ st) {
+    const uint32_t v = blockIdx.y;
+    const uint64_t r = off[v] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < off[v + 1]) quantize_row(x, (uint32_t)r, q, nrm, st + v);
+}
+
+__device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* __restrict__ q,
+                             uint32_t* __restrict__ nrm, ViewStats* __restrict__ st) {
     const float* r = x + (size_t)i * kDescLen;
     float s = 0.f, res = 0.f, qq = 0.f;
     bool bad = false;
@@ -151,6 +167,7 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
     __shared__ float2 Cs[4][BN];
 
     const RowBlock blk = blocks[blockIdx.x];
+    if (blk.pair == kNone) return;  // padding of the XCD-interleaved work list
     const LowpTask t = tasks[blk.pair];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -549,6 +566,14 @@ void launch_quantize(const float* x, uint32_t n, uint16_t* q, uint32_t* nrm, Vie
                      hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(quantize_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, n, q, nrm, st);
+}
+
+void launch_quantize_views(const float* x, const uint64_t* d_off, uint32_t n_views,
+                           uint32_t max_rows, uint16_t* q, uint32_t* nrm, ViewStats* st,
+                           hipStream_t s) {
+    if (!n_views || !max_rows) return;
+    hipLaunchKernelGGL(quantize_views_kernel, dim3((max_rows + 255) / 256, n_views), dim3(256), 0,
+                       s, x, d_off, q, nrm, st);
 }
 
 uint32_t lowp_block_rows(int tile) { return tile_rb(tile) * 128u; }

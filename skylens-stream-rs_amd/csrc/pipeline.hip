@@ -40,7 +40,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
         (rc = ensure(col_top2, sizeof(float2) * nb, "col_top2")) ||
         (rc = ensure(row_match, 4 * na, "row_match")) ||
         (rc = ensure(queue, sizeof(uint4) * na, "queue")) ||
-        (rc = ensure(queue_len, 4, "queue_len")) || (rc = ensure(out_base, 4, "out_base")) ||
+        (rc = ensure(queue_len, 4, "queue_len")) || (rc = ensure(out_base, 8, "out_base")) ||
         (rc = ensure(tasks, sizeof(PairTask), "tasks")))
         return rc;
 
@@ -82,7 +82,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
 
     launch_stats_reset(st, 2, stream);
     launch_fill_u32(o.queue_len, 1, 0u, stream);
-    launch_fill_u32(out_base.as<uint32_t>(), 1, 0u, stream);
+    launch_fill_u64(reinterpret_cast<unsigned long long*>(out_base.ptr), 1, 0ull, stream);
     launch_quantize(d_a, (uint32_t)na, q_a.as<uint16_t>(), nrm_a.as<uint32_t>(), st, stream);
     launch_quantize(d_b, (uint32_t)nb, q_b.as<uint16_t>(), nrm_b.as<uint32_t>(), st + 1, stream);
     launch_mfma_rank(tile, ltasks.as<LowpTask>(), blocks.as<RowBlock>(), nblk, o, tbits, stream);
@@ -90,7 +90,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     launch_decide(ltasks.as<LowpTask>(), 1, o, ratio, mutual, tbits, stream);
     launch_rescore(ltasks.as<LowpTask>(), o, ratio, mutual, tbits, (uint32_t)std::max(64, 4 * num_cu),
                    stream);
-    launch_compact(tasks.as<PairTask>(), 1, o.row_match, out_base.as<uint32_t>(), d_out_ij,
+    launch_compact(tasks.as<PairTask>(), 1, o.row_match, out_base.as<uint64_t>(), d_out_ij,
                    d_out_len, stream);
     return hip(hipGetLastError(), "launch lowp");
 }

@@ -1,0 +1,976 @@
+// ransac_fundamental on gfx950: one 64-lane workgroup per image pair.
+//
+// Reference: crates/core/src/matching.rs:941-1104 (and the helpers it calls).
+// The sequential semantics (adaptive iteration count, local-optimisation
+// trigger, 5-entry Jaccard pool, stable sort) are replayed exactly while the
+// heavy work runs across lanes:
+//   * hypotheses in batches of 64: lane 0 draws the seeded samples
+//     (matching.rs:956-979), every lane solves one 8-point F (FP64) and counts
+//     its Sampson inliers;
+//   * local optimisation for hypothesis k depends only on F_k and the data
+//     (matching.rs:993-1015), so every hypothesis that *could* trigger it runs
+//     it in its own lane: cnt >= 8 and 4 cnt >= max(best count at batch
+//     start, raw counts >= 8 earlier in the batch), a superset of the real
+//     triggers because the best count is never below an earlier raw count;
+//   * lane 0 replays the batch in order with the precomputed results; pool
+//     Jaccard tests use inlier bitsets;
+//   * the pool's Sampson-LM trajectories (matching.rs:1041-1057) run in
+//     parallel lanes and are merged in pool order; the final refits, gates
+//     and GRIC model selection follow the reference.
+// All FP64 routines match the host restatement operation for operation.
+#pragma clang fp contract(off)
+
+#include "ransac_dev.hpp"
+#include "skh_ransac.hpp"
+
+namespace skh {
+namespace {
+
+using namespace rdev;
+
+constexpr int NL = 64;  // lanes per pair
+__constant__ double kNM[6] = {3.0, 2.0, 1.5, 1.0, 1.0, 1.0};
+
+__host__ __device__ __forceinline__ uint32_t words(uint32_t n) { return (n + 31) / 32; }
+
+// The first four of eight gathered samples (a homography hypothesis).
+struct Pts4 {
+    const Pts8* b;
+    static constexpr uint32_t n = 4;
+    __device__ bool take(uint32_t) const { return true; }
+    __device__ double2 p(uint32_t i) const { return b->a[i]; }
+    __device__ double2 q(uint32_t i) const { return b->b[i]; }
+};
+
+// ---------------------------------------------------------------- LM refine
+
+template <class S>
+__device__ double lm_cost(const double* g, double s1, double s2, const double* t1,
+                          const double* t2, const S& s) {
+    const double q1 = s1 * s1, q2 = s2 * s2;
+    double cost = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 p = s.p(i), q = s.q(i);
+        double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+        mv3(t1, pa, a);
+        mv3(t2, pb, b);
+        double ga[3], gtb[3];
+        mv3(g, a, ga);
+        mtv3(g, b, gtb);
+        double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+        double e;
+        if (den > 0.0)
+            e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
+        else
+            e = 0.0;
+        cost = cost + e * e;
+    }
+    return cost;
+}
+
+// Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs.
+__device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+            const double factor = -L[ut(j, k)];
+#pragma unroll
+            for (int i = j; i < 9; ++i) L[ut(i, j)] = L[ut(i, j)] + factor * L[ut(i, k)];
+        }
+        const double diag = L[ut(j, j)];
+        if (!(diag > 0.0)) return false;
+        const double den = sqrt(diag);
+        L[ut(j, j)] = den;
+#pragma unroll
+        for (int i = j + 1; i < 9; ++i) L[ut(i, j)] = L[ut(i, j)] / den;
+    }
+    double y[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) y[i] = rhs[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        y[i] = y[i] / L[ut(i, i)];
+#pragma unroll
+        for (int k = i + 1; k < 9; ++k) y[k] = y[k] - L[ut(k, i)] * y[i];
+    }
+#pragma unroll
+    for (int i = 8; i >= 0; --i) {
+        double dot = 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 9; ++k) dot = dot + L[ut(k, i)] * y[k];
+        y[i] = (y[i] - dot) / L[ut(i, i)];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) x[i] = y[i];
+    return true;
+}
+
+// matching.rs:665-760 over the selected points (len of them).
+template <class S>
+__device__ void refine_sampson(const double* F, const S& s, uint32_t len, int iters, double* out) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = F[i];
+    if (len < 8) return;
+    double t1[9], t2[9], t1i[9], t2i[9];
+    normalizer(s, false, t1, len);
+    normalizer(s, true, t2, len);
+    if (!inv3(t1, t1i) || !inv3(t2, t2i)) return;
+    const double s1 = t1[0], s2 = t2[0];
+    double t2it[9], tmp[9], g0[9], g[9];
+    tr3(t2i, t2it);
+    mm3(t2it, F, tmp);
+    mm3(tmp, t1i, g0);
+    if (!rank2_unit(g0, g)) return;
+    double cost = lm_cost(g, s1, s2, t1, t2, s);
+    double lambda = 1e-3;
+    const double q1 = s1 * s1, q2 = s2 * s2;
+    for (int it = 0; it < iters; ++it) {
+        double jtj[45], jtr[9];
+#pragma unroll
+        for (int i = 0; i < 45; ++i) jtj[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) jtr[i] = 0.0;
+        for (uint32_t i = 0; i < s.n; ++i) {
+            if (!s.take(i)) continue;
+            double2 p = s.p(i), q = s.q(i);
+            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+            mv3(t1, pa, a);
+            mv3(t2, pb, b);
+            // matching.rs:591-618
+            double ga[3], gtb[3];
+            mv3(g, a, ga);
+            mtv3(g, b, gtb);
+            double e = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
+            double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+            double ri, ji[9];
+            if (den <= 0.0) {
+                ri = 0.0;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) ji[k] = 0.0;
+            } else {
+                double inv = 1.0 / sqrt(den);
+                double c = e * inv * inv * inv;
+                double u[3] = {q2 * ga[0], q2 * ga[1], 0.0};
+                double v[3] = {q1 * gtb[0], q1 * gtb[1], 0.0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l)
+                        ji[3 * k + l] = b[k] * a[l] * inv - c * (u[k] * a[l] + b[k] * v[l]);
+                ri = e * inv;
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+#pragma unroll
+                for (int r = c; r < 9; ++r) jtj[ut(c, r)] = ji[c] * ji[r] + jtj[ut(c, r)];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) jtr[k] = jtr[k] + ji[k] * ri;
+        }
+        bool improved = false;
+        for (int tries = 0; tries < 8; ++tries) {
+            double m[45];
+#pragma unroll
+            for (int i = 0; i < 45; ++i) m[i] = jtj[i];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[ut(k, k)] = m[ut(k, k)] + lambda * (jtj[ut(k, k)] + 1e-12);
+            double nrhs[9], d[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) nrhs[k] = -jtr[k];
+            if (!cholesky_solve9(m, nrhs, d)) {
+                lambda *= 10.0;
+                continue;
+            }
+            double gs[9], cand[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
+            if (!rank2_unit(gs, cand)) {
+                lambda *= 10.0;
+                continue;
+            }
+            double c = lm_cost(cand, s1, s2, t1, t2, s);
+            if (c < cost) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) g[k] = cand[k];
+                double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
+                cost = c;
+                lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;
+                improved = rel > 1e-10;
+                break;
+            }
+            lambda *= 10.0;
+        }
+        if (!improved) break;
+    }
+    double t2t[9], fp[9];
+    tr3(t2, t2t);
+    mm3(t2t, g, tmp);
+    mm3(tmp, t1, fp);
+    double nn = fro3(fp);
+    if (isfinite(nn) && nn > 0.0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = fp[i] / nn;
+    }
+}
+
+// ------------------------------------------------------------- mask helpers
+
+// Writes the bitset {i : sampson(F, i) < t2} (w words) and returns its size.
+__device__ uint32_t mask_of(const double* F, const double2* x1, const double2* x2, uint32_t n,
+                            double t2, uint32_t* bits) {
+    uint32_t cnt = 0;
+    for (uint32_t w0 = 0; w0 < n; w0 += 32) {
+        uint32_t wbits = 0;
+        const uint32_t e = n - w0 < 32 ? n - w0 : 32;
+        for (uint32_t b = 0; b < e; ++b)
+            if (sampson_error(F, x1[w0 + b], x2[w0 + b]) < t2) wbits |= 1u << b;
+        bits[w0 >> 5] = wbits;
+        cnt += __popc(wbits);
+    }
+    return cnt;
+}
+
+__device__ uint32_t count_of(const double* F, const double2* x1, const double2* x2, uint32_t n,
+                             double t2) {
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < n; ++i) cnt += sampson_error(F, x1[i], x2[i]) < t2 ? 1u : 0u;
+    return cnt;
+}
+
+// ------------------------------------------------------------- model choice
+
+__device__ __forceinline__ uint64_t ipow_adaptive(double w, int sample, double confidence,
+                                                  uint64_t max_iters) {
+    // matching.rs:799-811; powi as binary exponentiation
+    uint64_t floor_ = 50 < max_iters ? 50 : max_iters;
+    double a = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+    unsigned ub = (unsigned)sample;
+    double r = 1.0;
+    for (;;) {
+        if (ub & 1) r *= a;
+        ub /= 2;
+        if (ub == 0) break;
+        a *= a;
+    }
+    if (r >= 1.0) return floor_;
+    double denom = log1p(-r);
+    const double cmax = 1.0 - 1e-15;
+    double cc = confidence < 0.0 ? 0.0 : (confidence > cmax ? cmax : confidence);
+    double need = log1p(-cc) / denom;
+    if (!isfinite(need) || need >= (double)max_iters) return max_iters;
+    double ce = ceil(need);
+    uint64_t v = ce <= 0.0 ? 0 : (uint64_t)ce;
+    if (v < floor_) v = floor_;
+    if (v > max_iters) v = max_iters;
+    return v;
+}
+
+__device__ double binomial_tail(uint64_t n, uint64_t k, double p) {
+    if (k == 0) return 1.0;
+    if (k > n || p <= 0.0) return 0.0;
+    if (p >= 1.0) return 1.0;
+    double lp = log(p), lq = log1p(-p);
+    double log_c = 0.0, sum = 0.0;
+    for (uint64_t i = 0; i <= n; ++i) {
+        if (i > 0) log_c += log((double)(n - i + 1)) - log((double)i);
+        if (i >= k) sum += exp(log_c + (double)i * lp + (double)(n - i) * lq);
+    }
+    return sum < 1.0 ? sum : 1.0;
+}
+
+__device__ uint64_t parallax_needed(uint64_t n) {
+    for (uint64_t k = 2; k <= n; ++k)
+        if (binomial_tail(n, k, 0.001) < 1e-6) return k;
+    return n + 1;
+}
+
+// two_view.rs:605-642 on Hartley-normalised points; matching.rs:251-266.
+template <class S>
+__device__ bool homography_dlt(const S& s, uint32_t len, double* H) {
+    if (len < 4) return false;
+    double t1[9], t2[9];
+    normalizer(s, false, t1, len);
+    normalizer(s, true, t2, len);
+    double ata[45];
+#pragma unroll
+    for (int i = 0; i < 45; ++i) ata[i] = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 pp = s.p(i), qq = s.q(i);
+        double p3[3] = {pp.x, pp.y, 1.0}, v[3];
+        mv3(t1, p3, v);
+        const double x = v[0] / v[2], y = v[1] / v[2];
+        double q3[3] = {qq.x, qq.y, 1.0};
+        mv3(t2, q3, v);
+        const double u = v[0] / v[2], w = v[1] / v[2];
+        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, u * x, u * y, u};
+        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, w * x, w * y, w};
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int c = r; c < 9; ++c)
+                ata[ut(r, c)] = ata[ut(r, c)] + (r1[r] * r1[c] + r2[r] * r2[c]);
+    }
+    double vv[81];
+    sym_eigen9(ata, vv);
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < 9; ++i)
+        if (ata[ut(i, i)] < ata[ut(k, k)]) k = i;
+    double hn[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc = (j == k) ? vv[9 * i + j] : acc;
+        hn[i] = acc;
+        if (!isfinite(acc)) return false;
+    }
+    double t2i[9], tmp[9], h[9];
+    if (!inv3(t2, t2i)) return false;
+    mm3(t2i, hn, tmp);
+    mm3(tmp, t1, h);
+    double nn = fro3(h);
+    if (!(isfinite(nn) && nn > 0.0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
+    return true;
+}
+
+__device__ __forceinline__ bool htr(const double* m, double2 x, double2& out) {
+    double p[3] = {x.x, x.y, 1.0}, v[3];
+    mv3(m, p, v);
+    if (fabs(v[2]) < 1e-12) return false;
+    out.x = v[0] / v[2];
+    out.y = v[1] / v[2];
+    return true;
+}
+
+__device__ __forceinline__ double homography_error(const double* h, const double* hi, double2 p,
+                                                   double2 q) {
+    double2 a, b;
+    if (!htr(h, p, a) || !htr(hi, q, b)) return INFINITY;
+    return fmax(norm2(a.x - q.x, a.y - q.y), norm2(b.x - p.x, b.y - p.y));
+}
+
+template <class S>
+__device__ uint32_t count_h(const double* h, const double* hi, const S& s, double th) {
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        c += homography_error(h, hi, s.p(i), s.q(i)) < th ? 1u : 0u;
+    }
+    return c;
+}
+
+struct SelState {
+    double F[NL][9];
+    int cnt[NL];
+    uint32_t idx[NL][8];
+    double best[9], besti[9];
+    int have, bcnt;
+    uint64_t iters, it;
+};
+
+// matching.rs:300-369 over the points of `s` (ns of them), all lanes.
+template <class S>
+__device__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, double* Hi,
+                               uint32_t* cnt_out, SelState& sh, uint32_t* selbits,
+                               const double2* x1, const double2* x2) {
+    const int lane = threadIdx.x;
+    if (ns < 4) return false;
+    // matching.rs:314-320: draws index the selected points in order, so
+    // lanes translate draw ranks to point indices.
+    __shared__ uint32_t rank2idx_dummy;
+    (void)rank2idx_dummy;
+    if (lane == 0) {
+        sh.have = 0;
+        sh.bcnt = 0;
+        sh.iters = 500;
+        sh.it = 0;
+    }
+    __syncthreads();
+    uint64_t st = 0x2545F4914F6CDD1Dull;
+    while (sh.it < sh.iters) {
+        if (lane == 0) {
+            for (int h = 0; h < NL; ++h) {
+                int k = 0;
+                while (k < 4) {
+                    uint32_t c = lcg_draw(st, ns);
+                    bool dup = false;
+                    for (int e = 0; e < k; ++e) dup |= sh.idx[h][e] == c;
+                    if (!dup) sh.idx[h][k++] = c;
+                }
+            }
+        }
+        __syncthreads();
+        {
+            // the draws are ranks among the selected points
+            Pts8 p4;
+            for (int e = 0; e < 4; ++e) {
+                uint32_t r = sh.idx[lane][e], i = 0, seen = 0;
+                for (;; ++i) {
+                    if (s.take(i)) {
+                        if (seen == r) break;
+                        ++seen;
+                    }
+                }
+                p4.a[e] = s.p(i);
+                p4.b[e] = s.q(i);
+            }
+            Pts4 v4{&p4};
+            double h[9], hi[9];
+            int c = -1;
+            if (homography_dlt(v4, 4, h) && inv3(h, hi)) {
+                c = (int)count_h(h, hi, s, th);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) sh.F[lane][i] = h[i];
+            }
+            sh.cnt[lane] = c;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int h = 0; h < NL && sh.it < sh.iters; ++h) {
+                sh.it++;
+                const int c = sh.cnt[h];
+                if (c < 0) continue;
+                if (!sh.have || c > sh.bcnt) {
+                    sh.have = 1;
+                    sh.bcnt = c;
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) sh.best[i] = sh.F[h][i];
+                    sh.iters = ipow_adaptive((double)c / (double)ns, 4, 0.999, 500);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!sh.have) return false;
+    bool ok = true;
+    if (lane == 0) {
+        double bh[9], bhi[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) bh[i] = sh.best[i];
+        inv3(bh, bhi);
+        uint32_t bc = (uint32_t)sh.bcnt;
+        double ch[9], chi[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            ch[i] = bh[i];
+            chi[i] = bhi[i];
+        }
+        const double HM[4] = {4.0, 2.0, 1.0, 1.0};
+        const uint32_t W = words(s.n);
+        for (int r = 0; r < 4; ++r) {
+            uint32_t nsel = 0;
+            for (uint32_t w0 = 0; w0 < W; ++w0) selbits[w0] = 0u;
+            for (uint32_t i = 0; i < s.n; ++i) {
+                if (!s.take(i)) continue;
+                if (homography_error(ch, chi, s.p(i), s.q(i)) < th * HM[r]) {
+                    selbits[i >> 5] |= 1u << (i & 31);
+                    ++nsel;
+                }
+            }
+            PtsMask sel{x1, x2, s.n, selbits};
+            double g[9], gi[9];
+            if (!homography_dlt(sel, nsel, g)) break;
+            if (!inv3(g, gi)) break;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                ch[i] = g[i];
+                chi[i] = gi[i];
+            }
+            uint32_t c = count_h(g, gi, s, th);
+            if (c > bc) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    bh[i] = g[i];
+                    bhi[i] = gi[i];
+                }
+                bc = c;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            H[i] = bh[i];
+            Hi[i] = bhi[i];
+        }
+        *cnt_out = bc;
+    }
+    __syncthreads();
+    return ok;
+}
+
+struct Shared {
+    uint32_t idx[NL][8];
+    double hF[NL][9];
+    int hCnt[NL];
+    double loF[NL][9];
+    int loCnt[NL];
+    double poolF[6][9];
+    int poolCnt[6];
+    int poolSlot[6];
+    int npool;
+    double bestF[9];
+    int bestCnt, haveBest;
+    uint64_t iters, it;
+    double stF[5][6][9];
+    int stCnt[5][6];
+    double f[9];
+    int inlCnt;
+    int result;
+    uint32_t draws[600];
+    int lineBest[NL];
+    double es_med;
+    int es_n;
+    double redA[NL], redB[NL];
+};
+
+__device__ bool hyp8(const RansacTask& t, const uint32_t* idx, double* F) {
+    Pts8 s;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        s.a[e] = t.x1[idx[e]];
+        s.b[e] = t.x2[idx[e]];
+    }
+    return fundamental_8pt(s, 8, F);
+}
+
+// matching.rs:993-1015 for one hypothesis; returns F_lo, its count, and its
+// inlier bitset in `inl` (sel is scratch).
+__device__ uint32_t local_opt(const RansacTask& t, const double* F0, uint32_t cnt0, double th2,
+                              double* f_lo, uint32_t* inl, uint32_t* sel) {
+    const uint32_t n = t.n;
+    double cur[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        f_lo[i] = F0[i];
+        cur[i] = F0[i];
+    }
+    uint32_t cnt = cnt0;
+    for (int r = 0; r < 6; ++r) {
+        const double t2 = th2 * kNM[r] * kNM[r];
+        const uint32_t ns = mask_of(cur, t.x1, t.x2, n, t2, sel);
+        PtsMask s{t.x1, t.x2, n, sel};
+        double g[9];
+        if (!fundamental_8pt(s, ns, g)) break;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) cur[i] = g[i];
+        const uint32_t gc = count_of(g, t.x1, t.x2, n, th2);
+        if (gc > cnt) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) f_lo[i] = g[i];
+            cnt = gc;
+        }
+    }
+    mask_of(f_lo, t.x1, t.x2, n, th2, inl);
+    return cnt;
+}
+
+__global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict__ tasks,
+                                                    const RansacCfgDev cfg) {
+    __shared__ Shared sh;
+    __shared__ SelState hs;
+    RansacTask t = tasks[blockIdx.x];
+    if (t.n_dev) t.n = min(*t.n_dev, t.n);
+    const int lane = threadIdx.x;
+    const uint32_t n = t.n;
+    const uint32_t W = words(n);
+    uint32_t* lo_masks = t.scratch;                    // NL * W
+    uint32_t* sel_masks = t.scratch + (size_t)NL * W;  // NL * W
+    uint32_t* pool_masks = t.scratch + (size_t)2 * NL * W;  // 6 * W
+    uint32_t* fin_mask = t.scratch + (size_t)(2 * NL + 6) * W;  // W
+    uint32_t* fin_sel = fin_mask + W;                            // W
+
+    // matching.rs:946-949
+    bool valid = n >= 8;
+    for (uint32_t i = lane; valid && i < n; i += NL) {
+        double2 a = t.x1[i], b = t.x2[i];
+        if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(b.x) || !isfinite(b.y)) valid = false;
+    }
+    valid = __syncthreads_and(valid);
+    if (!valid) {
+        if (lane == 0) *t.found = 0;
+        return;
+    }
+    const double th2 = cfg.threshold_px * cfg.threshold_px;
+    uint64_t st = cfg.seed ^ 0x9E3779B97F4A7C15ull;
+    if (lane == 0) {
+        sh.npool = 0;
+        sh.bestCnt = 0;
+        sh.haveBest = 0;
+        sh.iters = cfg.max_iters;
+        sh.it = 0;
+        sh.result = 0;
+        for (int k = 0; k < 6; ++k) sh.poolSlot[k] = k;
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------- hypothesis batches
+    while (sh.it < sh.iters) {
+        if (lane == 0) {
+            for (int h = 0; h < NL; ++h) {
+                int k = 0;
+                while (k < 8) {
+                    uint32_t c = lcg_draw(st, n);
+                    bool dup = false;
+                    for (int e = 0; e < k; ++e) dup |= sh.idx[h][e] == c;
+                    if (!dup) sh.idx[h][k++] = c;
+                }
+            }
+        }
+        __syncthreads();
+        double F[9];
+        int cnt = -1;
+        if (hyp8(t, sh.idx[lane], F)) cnt = (int)count_of(F, t.x1, t.x2, n, th2);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sh.hF[lane][i] = F[i];
+        sh.hCnt[lane] = cnt;
+        // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
+        int v = cnt >= 8 ? cnt : 0;
+        int incl = v;
+#pragma unroll
+        for (int off = 1; off < NL; off <<= 1) {
+            int o = __shfl_up(incl, off);
+            if (lane >= off) incl = incl > o ? incl : o;
+        }
+        int excl = __shfl_up(incl, 1);
+        if (lane == 0) excl = 0;
+        const int bound = sh.bestCnt > excl ? sh.bestCnt : excl;
+        const bool cand = cnt >= 8 && 4 * cnt >= bound;
+        if (cand) {
+            double flo[9];
+            sh.loCnt[lane] = (int)local_opt(t, F, (uint32_t)cnt, th2, flo,
+                                            lo_masks + (size_t)lane * W,
+                                            sel_masks + (size_t)lane * W);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sh.loF[lane][i] = flo[i];
+        } else {
+            sh.loCnt[lane] = -1;
+        }
+        __syncthreads();
+        // replay in order (lane 0)
+        if (lane == 0) {
+            for (int h = 0; h < NL && sh.it < sh.iters; ++h) {
+                sh.it++;
+                const int c0 = sh.hCnt[h];
+                if (c0 < 0) continue;
+                const int best_cnt = sh.haveBest ? sh.bestCnt : 0;
+                if (!(c0 >= 8 && 4 * c0 >= best_cnt)) continue;
+                // sh.loCnt[h] >= 0 here: the candidate bound never exceeds best_cnt
+                const int c = sh.loCnt[h];
+                if (c < 0) {  // cannot happen: candidates are a superset of triggers
+                    sh.result = -1;
+                    sh.it = sh.iters;
+                    break;
+                }
+                const uint32_t* inl = lo_masks + (size_t)h * W;
+                int same = -1;
+                for (int j = 0; j < sh.npool && same < 0; ++j) {
+                    const uint32_t* pm = pool_masks + (size_t)sh.poolSlot[j] * W;
+                    uint32_t both = 0, either = 0;
+                    for (uint32_t w0 = 0; w0 < W; ++w0) {
+                        both += __popc(pm[w0] & inl[w0]);
+                        either += __popc(pm[w0] | inl[w0]);
+                    }
+                    if (10 * both > 7 * either) same = j;
+                }
+                int target = -1;
+                if (same >= 0) {
+                    if (sh.poolCnt[same] < c) target = same;
+                } else {
+                    target = sh.npool++;
+                }
+                if (target >= 0) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) sh.poolF[target][i] = sh.loF[h][i];
+                    sh.poolCnt[target] = c;
+                    uint32_t* dst = pool_masks + (size_t)sh.poolSlot[target] * W;
+                    for (uint32_t w0 = 0; w0 < W; ++w0) dst[w0] = inl[w0];
+                }
+                // stable insertion sort, count descending
+                for (int a = 1; a < sh.npool; ++a) {
+                    double xf[9];
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) xf[i] = sh.poolF[a][i];
+                    const int xc = sh.poolCnt[a], xs = sh.poolSlot[a];
+                    int b = a - 1;
+                    while (b >= 0 && sh.poolCnt[b] < xc) {
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = sh.poolF[b][i];
+                        sh.poolCnt[b + 1] = sh.poolCnt[b];
+                        sh.poolSlot[b + 1] = sh.poolSlot[b];
+                        --b;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = xf[i];
+                    sh.poolCnt[b + 1] = xc;
+                    sh.poolSlot[b + 1] = xs;
+                }
+                if (sh.npool > 5) sh.npool = 5;  // the dropped slot stays at index 5
+                if (c <= best_cnt) continue;
+                sh.iters = ipow_adaptive((double)c / (double)n, 8, cfg.confidence, cfg.max_iters);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) sh.bestF[i] = sh.loF[h][i];
+                sh.bestCnt = c;
+                sh.haveBest = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && t.iters_out) *t.iters_out = sh.it;
+    if (sh.result < 0) {
+        if (lane == 0) *t.found = -1;
+        return;
+    }
+    if (!sh.haveBest) {
+        if (lane == 0) *t.found = 0;
+        return;
+    }
+
+    // ------------------------------------------- pool LM (matching.rs:1041-1057)
+    if (lane < sh.npool) {
+        double cur[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) cur[i] = sh.poolF[lane][i];
+        uint32_t* sel = sel_masks + (size_t)lane * W;
+        for (int r = 0; r < 6; ++r) {
+            const double t2 = th2 * kNM[r] * kNM[r];
+            const uint32_t ns = mask_of(cur, t.x1, t.x2, n, t2, sel);
+            PtsMask s{t.x1, t.x2, n, sel};
+            double nxt[9];
+            refine_sampson(cur, s, ns, 30, nxt);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                cur[i] = nxt[i];
+                sh.stF[lane][r][i] = nxt[i];
+            }
+            sh.stCnt[lane][r] = (int)count_of(cur, t.x1, t.x2, n, th2);
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double f[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) f[i] = sh.bestF[i];
+        int ic = sh.bestCnt;
+        for (int p = 0; p < sh.npool; ++p)
+            for (int r = 0; r < 6; ++r)
+                if (sh.stCnt[p][r] > ic) {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) f[i] = sh.stF[p][r][i];
+                    ic = sh.stCnt[p][r];
+                }
+        // matching.rs:1059-1069
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t ns = mask_of(f, t.x1, t.x2, n, th2, fin_sel);
+            PtsMask s{t.x1, t.x2, n, fin_sel};
+            double g[9];
+            refine_sampson(f, s, ns, 30, g);
+            const uint32_t c = count_of(g, t.x1, t.x2, n, th2);
+            if (c < ns) break;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) f[i] = g[i];
+            ic = (int)c;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sh.f[i] = f[i];
+        sh.inlCnt = ic;
+        mask_of(f, t.x1, t.x2, n, th2, fin_mask);
+        // matching.rs:1071-1080
+        int ok = 1;
+        const uint64_t minc = cfg.min_inliers > 8 ? cfg.min_inliers : 8;
+        if ((uint64_t)ic < minc || (double)ic < cfg.min_inlier_ratio * (double)n) ok = 0;
+        if (ok && cfg.max_chance_prob > 0.0) {
+            double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY;
+            for (uint32_t i = 0; i < n; ++i) {
+                double2 q = t.x2[i];
+                lox = fmin(lox, q.x);
+                loy = fmin(loy, q.y);
+                hix = fmax(hix, q.x);
+                hiy = fmax(hiy, q.y);
+            }
+            double dx = hix - lox, dy = hiy - loy;
+            double area = dx * dy;
+            double p = 1.0;
+            if (isfinite(area) && area > 0.0) {
+                p = 2.0 * cfg.threshold_px * norm2(dx, dy) / area;
+                p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
+            }
+            double chance = (uint64_t)ic <= 8 ? 1.0
+                                              : binomial_tail(n > 8 ? n - 8 : 0, (uint64_t)ic - 8, p);
+            if (chance > cfg.max_chance_prob) ok = 0;
+        }
+        if (ok) {
+            PtsMask s{t.x1, t.x2, n, fin_mask};
+            const uint32_t ns = (uint32_t)ic;
+            if (nearly_collinear(s, false, ns) || nearly_collinear(s, true, ns)) ok = 0;
+        }
+        sh.result = ok;
+        sh.es_n = (int)ic;
+    }
+    __syncthreads();
+    if (!sh.result) {
+        if (lane == 0) *t.found = 0;
+        return;
+    }
+
+    // ---------------------------- off_line_count (matching.rs:878-906), both images
+    const uint32_t ns = (uint32_t)sh.es_n;
+    bool line_ok = true;
+    for (int side = 0; side < 2; ++side) {
+        if (ns < 3) {
+            line_ok = false;  // off_line_count = 0 < 8
+            break;
+        }
+        if (lane == 0) {
+            uint64_t ls = 0x6A09E667F3BCC909ull;
+            for (int k = 0; k < 600; ++k) sh.draws[k] = lcg_draw(ls, ns);
+        }
+        __syncthreads();
+        // rank -> point index among the inliers
+        int best = 0;
+        for (int it = lane; it < 300; it += NL) {
+            const uint32_t ri = sh.draws[2 * it], rj = sh.draws[2 * it + 1];
+            uint32_t pi = 0, pj = 0, seen = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!((fin_mask[i >> 5] >> (i & 31)) & 1u)) continue;
+                if (seen == ri) pi = i;
+                if (seen == rj) pj = i;
+                ++seen;
+            }
+            const double2 a = side ? t.x2[pi] : t.x1[pi];
+            const double2 b = side ? t.x2[pj] : t.x1[pj];
+            const double dx = b.x - a.x, dy = b.y - a.y;
+            const double len = norm2(dx, dy);
+            if (ri == rj || isnan(len) || len <= 0.0) continue;
+            const double nx = -dy / len, ny = dx / len;
+            int c = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!((fin_mask[i >> 5] >> (i & 31)) & 1u)) continue;
+                const double2 q = side ? t.x2[i] : t.x1[i];
+                const double ex = q.x - a.x, ey = q.y - a.y;
+                if (fabs(ex * nx + ey * ny) < cfg.threshold_px) ++c;
+            }
+            best = best > c ? best : c;
+        }
+        sh.lineBest[lane] = best;
+        __syncthreads();
+        if (lane == 0) {
+            int b = 0;
+            for (int l = 0; l < NL; ++l) b = b > sh.lineBest[l] ? b : sh.lineBest[l];
+            sh.result = (int)(ns - (uint32_t)b) >= 8;
+        }
+        __syncthreads();
+        if (!sh.result) {
+            line_ok = false;
+            break;
+        }
+    }
+    if (!line_ok) {
+        if (lane == 0) *t.found = 0;
+        return;
+    }
+
+    // --------------------------- select_two_view_model (matching.rs:425-492)
+    RansacSel selo{};
+    {
+        PtsMask s{t.x1, t.x2, n, fin_mask};
+        const double sigma_in = cfg.threshold_px / 3.0;
+        bool has = ns >= 8 && isfinite(sigma_in) && sigma_in > 0.0;
+        // F entries are finite (F is normalised); coordinates are finite.
+        double sigma_hat = 0.0;
+        if (has) {
+            // median of the finite Sampson errors over the inliers, rank ns_f / 2
+            int nf = 0;
+            for (uint32_t i = 0; i < n; ++i)
+                if (s.take(i) && isfinite(sampson_error(sh.f, t.x1[i], t.x2[i]))) ++nf;
+            if (nf < 8) has = false;
+            if (has) {
+                const int want = nf / 2;
+                double found_v = 0.0;
+                int found_c = 0;
+                for (uint32_t i = lane; i < n; i += NL) {
+                    if (!s.take(i)) continue;
+                    const double ei = sampson_error(sh.f, t.x1[i], t.x2[i]);
+                    if (!isfinite(ei)) continue;
+                    int below = 0, tie_before = 0;
+                    for (uint32_t j = 0; j < n; ++j) {
+                        if (!s.take(j)) continue;
+                        const double ej = sampson_error(sh.f, t.x1[j], t.x2[j]);
+                        if (!isfinite(ej)) continue;
+                        below += ej < ei;
+                        tie_before += (ej == ei && j < i);
+                    }
+                    if (below + tie_before == want) {
+                        found_v = ei;
+                        found_c = 1;
+                    }
+                }
+                sh.redA[lane] = found_v;
+                sh.lineBest[lane] = found_c;
+                __syncthreads();
+                if (lane == 0) {
+                    double m = 0.0;
+                    for (int l = 0; l < NL; ++l)
+                        if (sh.lineBest[l]) m = sh.redA[l];
+                    sh.es_med = m;
+                }
+                __syncthreads();
+                sigma_hat = sqrt(sh.es_med / 0.455);
+            }
+        }
+        if (has) {
+            const double sigma = fmax(sigma_in, sigma_hat);
+            const double s2 = sigma * sigma;
+            double H[9], Hi[9];
+            uint32_t hc = 0;
+            const bool hok = fit_homography(s, ns, 3.0 * sigma, H, Hi, &hc, hs, fin_sel, t.x1, t.x2);
+            if (hok && lane == 0) {
+                double rho_f = 0.0, rho_h = 0.0;
+                uint64_t parallax = 0;
+                for (uint32_t i = 0; i < n; ++i) {
+                    if (!s.take(i)) continue;
+                    const double2 p = t.x1[i], q = t.x2[i];
+                    const double ef = sampson_error(sh.f, p, q) / s2;
+                    double2 a, b;
+                    const double e1 = htr(H, p, a) ? (a.x - q.x) * (a.x - q.x) + (a.y - q.y) * (a.y - q.y) : INFINITY;
+                    const double e2 = htr(Hi, q, b) ? (b.x - p.x) * (b.x - p.x) + (b.y - p.y) * (b.y - p.y) : INFINITY;
+                    const double eh = 0.25 * (e1 + e2) / s2;
+                    rho_f += fmin(ef, 2.0 * (4.0 - 3.0));
+                    rho_h += fmin(eh, 2.0 * (4.0 - 2.0));
+                    if (eh > 13.8 && ef < 10.8) parallax++;
+                }
+                const double nf = (double)ns;
+                const double l1 = log(4.0), l2 = log(4.0 * nf);
+                selo.gric_f = rho_f + l1 * 3.0 * nf + l2 * 7.0;
+                selo.gric_h = rho_h + l1 * 2.0 * nf + l2 * 8.0;
+                selo.parallax = parallax;
+                selo.parallax_needed = parallax_needed(ns);
+                selo.model = (selo.gric_h < selo.gric_f && parallax < selo.parallax_needed) ? 1 : 0;
+                selo.sigma_px = sigma;
+                selo.has_selection = 1;
+            }
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) t.F[i] = sh.f[i];
+        *t.sel = selo;
+        *t.found = 1;
+    }
+    for (uint32_t i = lane; i < n; i += NL) t.inliers[i] = (fin_mask[i >> 5] >> (i & 31)) & 1u;
+}
+
+}  // namespace
+
+void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
+                   hipStream_t s) {
+    if (!n_tasks) return;
+    hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg);
+}
+
+uint32_t ransac_scratch_words(uint32_t n) { return (2 * NL + 8) * words(n); }
+
+}  // namespace skh

@@ -1,0 +1,427 @@
+// Device-side FP64 geometry of ransac_fundamental (gfx950, one lane per call).
+//
+// Reference: crates/core/src/matching.rs:217-1104 and
+// crates/core/src/two_view.rs:605-642. Every routine performs the same
+// floating-point operations in the same order as the host restatement, with
+// no FMA contraction, so a lane's F, counts and masks are bit-identical to
+// it. The eigen/SVD solvers are the project's own (cyclic Jacobi for the
+// symmetric 9x9 problem, one-sided Jacobi for the 3x3 rank-2 projection):
+// nalgebra's routines (the reference's) are not part of this build.
+//
+// Point sets are read through a selector: all points, a bitset over the
+// n correspondences, or eight gathered samples, always in index order.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace skh {
+namespace rdev {
+
+// ------------------------------------------------------------ point sources
+
+struct PtsAll {
+    const double2* x1;
+    const double2* x2;
+    uint32_t n;
+    __device__ bool take(uint32_t) const { return true; }
+    __device__ double2 p(uint32_t i) const { return x1[i]; }
+    __device__ double2 q(uint32_t i) const { return x2[i]; }
+};
+
+struct PtsMask {
+    const double2* x1;
+    const double2* x2;
+    uint32_t n;
+    const uint32_t* bits;
+    __device__ bool take(uint32_t i) const { return (bits[i >> 5] >> (i & 31)) & 1u; }
+    __device__ double2 p(uint32_t i) const { return x1[i]; }
+    __device__ double2 q(uint32_t i) const { return x2[i]; }
+};
+
+struct Pts8 {
+    double2 a[8];
+    double2 b[8];
+    static constexpr uint32_t n = 8;
+    __device__ bool take(uint32_t) const { return true; }
+    __device__ double2 p(uint32_t i) const { return a[i]; }
+    __device__ double2 q(uint32_t i) const { return b[i]; }
+};
+
+// ------------------------------------------------------------------ 3x3 ops
+
+struct M3 {
+    double m[9];
+};
+
+__device__ __forceinline__ void mv3(const double* m, const double* x, double* y) {
+    double a = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+    double b = m[3] * x[0] + m[4] * x[1] + m[5] * x[2];
+    double c = m[6] * x[0] + m[7] * x[1] + m[8] * x[2];
+    y[0] = a;
+    y[1] = b;
+    y[2] = c;
+}
+
+__device__ __forceinline__ void mtv3(const double* m, const double* x, double* y) {
+    double a = m[0] * x[0] + m[3] * x[1] + m[6] * x[2];
+    double b = m[1] * x[0] + m[4] * x[1] + m[7] * x[2];
+    double c = m[2] * x[0] + m[5] * x[1] + m[8] * x[2];
+    y[0] = a;
+    y[1] = b;
+    y[2] = c;
+}
+
+__device__ __forceinline__ void mm3(const double* A, const double* B, double* C) {
+    double t[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            t[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) C[i] = t[i];
+}
+
+__device__ __forceinline__ void tr3(const double* A, double* T) {
+    double t[9] = {A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) T[i] = t[i];
+}
+
+__device__ __forceinline__ double fro3(const double* m) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double col = m[c] * m[c] + m[3 + c] * m[3 + c] + m[6 + c] * m[6 + c];
+        s = s + col;
+    }
+    return sqrt(s);
+}
+
+__device__ __forceinline__ double norm2(double x, double y) { return sqrt(x * x + y * y); }
+
+__device__ __forceinline__ bool inv3(const double* m, double* out) {
+    double m11 = m[4] * m[8] - m[5] * m[7];
+    double m12 = m[3] * m[8] - m[5] * m[6];
+    double m13 = m[3] * m[7] - m[4] * m[6];
+    double det = m[0] * m11 - m[1] * m12 + m[2] * m13;
+    if (det == 0.0) return false;
+    double o[9];
+    o[0] = m11 / det;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+    o[3] = -m12 / det;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+    o[6] = m13 / det;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = o[i];
+    return true;
+}
+
+__device__ __forceinline__ uint32_t lcg_draw(uint64_t& st, uint32_t m) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (uint32_t)(st >> 33) % m;
+}
+
+// ---------------------------------------------------------- Jacobi eigen 9x9
+
+// Symmetric matrix stored as its upper triangle, index of (r, c), r <= c.
+__host__ __device__ constexpr int ut(int r, int c) {
+    return r <= c ? r * 9 - r * (r - 1) / 2 + (c - r) : c * 9 - c * (c - 1) / 2 + (r - c);
+}
+
+__device__ __forceinline__ void jacobi_params(double app, double aqq, double apq, int sweep,
+                                              double& c, double& s, double& t) {
+    c = 1.0;
+    s = 0.0;
+    t = 0.0;
+    if (apq == 0.0) return;
+    double g = 100.0 * fabs(apq);
+    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return;
+    double h = aqq - app;
+    double tt;
+    if (fabs(h) + g == fabs(h)) {
+        tt = apq / h;
+    } else {
+        double theta = 0.5 * h / apq;
+        tt = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+        if (theta < 0.0) tt = -tt;
+    }
+    double cc = 1.0 / sqrt(1.0 + tt * tt);
+    c = cc;
+    s = tt * cc;
+    t = tt;
+}
+
+// a: upper triangle (45); v: 9x9 row-major. Same rotation order and
+// arithmetic as the host's cyclic Jacobi.
+__device__ __forceinline__ void sym_eigen9(double* a, double* v) {
+#pragma unroll
+    for (int i = 0; i < 81; ++i) v[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) off = off + fabs(a[ut(p, q)]);
+        if (off == 0.0) break;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) {
+                double c, s, t;
+                const double apq = a[ut(p, q)];
+                jacobi_params(a[ut(p, p)], a[ut(q, q)], apq, sweep, c, s, t);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == p || k == q) continue;
+                    double akp = a[ut(k, p)], akq = a[ut(k, q)];
+                    a[ut(k, p)] = c * akp - s * akq;
+                    a[ut(k, q)] = s * akp + c * akq;
+                }
+                a[ut(p, p)] = a[ut(p, p)] - t * apq;
+                a[ut(q, q)] = a[ut(q, q)] + t * apq;
+                a[ut(p, q)] = 0.0;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    double vkp = v[9 * k + p], vkq = v[9 * k + q];
+                    v[9 * k + p] = c * vkp - s * vkq;
+                    v[9 * k + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- 3x3 rank 2
+
+__device__ __forceinline__ void rank2_recompose(const double* F, double* out) {
+    double w[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) w[i] = F[i];
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        bool rotated = false;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2;
+            double alpha = w[p] * w[p] + w[3 + p] * w[3 + p] + w[6 + p] * w[6 + p];
+            double beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
+            double gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
+            double c = 1.0, s = 0.0;
+            if (gamma != 0.0 && fabs(gamma) > 1e-15 * sqrt(alpha * beta)) {
+                double zeta = (beta - alpha) / (2.0 * gamma);
+                double t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                if (zeta < 0.0) t = -t;
+                c = 1.0 / sqrt(1.0 + t * t);
+                s = c * t;
+                rotated = true;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double wp = w[3 * k + p], wq = w[3 * k + q];
+                w[3 * k + p] = c * wp - s * wq;
+                w[3 * k + q] = s * wp + c * wq;
+                double vp = v[3 * k + p], vq = v[3 * k + q];
+                v[3 * k + p] = c * vp - s * vq;
+                v[3 * k + q] = s * vp + c * vq;
+            }
+        }
+        if (!rotated) break;
+    }
+    double sig[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sig[c] = w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c];
+    int m = 0;
+    if (sig[1] < sig[m]) m = 1;
+    if (sig[2] < sig[m]) m = 2;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                if (i == m) continue;
+                acc = acc + w[3 * r + i] * v[3 * c + i];
+            }
+            out[3 * r + c] = acc;
+        }
+}
+
+__device__ __forceinline__ bool rank2_unit(const double* f, double* out) {
+    double g[9];
+    rank2_recompose(f, g);
+    double n = fro3(g);
+    if (!(isfinite(n) && n > 0.0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = g[i] / n;
+    return true;
+}
+
+// ------------------------------------------------------------- geometry ops
+
+template <class S>
+__device__ void normalizer(const S& s, bool second, double* T, uint32_t len) {
+    double n = (double)len;
+    double cx = 0.0, cy = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 v = second ? s.q(i) : s.p(i);
+        cx = cx + v.x;
+        cy = cy + v.y;
+    }
+    cx = cx / n;
+    cy = cy / n;
+    double d = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 v = second ? s.q(i) : s.p(i);
+        d = d + norm2(v.x - cx, v.y - cy);
+    }
+    d = d / n;
+    double sc = d > 0.0 ? 1.4142135623730951 / d : 1.0;
+    T[0] = sc; T[1] = 0.0; T[2] = -sc * cx;
+    T[3] = 0.0; T[4] = sc; T[5] = -sc * cy;
+    T[6] = 0.0; T[7] = 0.0; T[8] = 1.0;
+}
+
+template <class S>
+__device__ bool nearly_collinear(const S& s, bool second, uint32_t len) {
+    double n = (double)(len > 0 ? len : 1);
+    double cx = 0.0, cy = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 v = second ? s.q(i) : s.p(i);
+        cx = cx + v.x;
+        cy = cy + v.y;
+    }
+    cx = cx / n;
+    cy = cy / n;
+    double sxx = 0.0, sxy = 0.0, syy = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 v = second ? s.q(i) : s.p(i);
+        double dx = v.x - cx, dy = v.y - cy;
+        sxx += dx * dx;
+        sxy += dx * dy;
+        syy += dy * dy;
+    }
+    sxx = sxx / n;
+    sxy = sxy / n;
+    syy = syy / n;
+    double tr = sxx + syy;
+    double dd = sxx - syy;
+    double disc = sqrt(dd * dd + 4.0 * sxy * sxy);
+    double hi = 0.5 * (tr + disc);
+    double lo = 0.5 * (tr - disc);
+    if (!(lo >= 0.0)) lo = 0.0;
+    double slo = sqrt(lo), shi = sqrt(hi);
+    return !(shi > 0.0 && slo > 0.02 * shi);
+}
+
+template <class S>
+__device__ uint32_t count_taken(const S& s) {
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < s.n; ++i) k += s.take(i) ? 1u : 0u;
+    return k;
+}
+
+template <class S>
+__device__ bool all_finite(const S& s) {
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 a = s.p(i), b = s.q(i);
+        if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(b.x) || !isfinite(b.y)) return false;
+    }
+    return true;
+}
+
+// matching.rs:512-557 over the selected points (len of them).
+template <class S>
+__device__ bool fundamental_8pt(const S& s, uint32_t len, double* F) {
+    if (len < 8 || !all_finite(s)) return false;
+    if (nearly_collinear(s, false, len) || nearly_collinear(s, true, len)) return false;
+    double t1[9], t2[9];
+    normalizer(s, false, t1, len);
+    normalizer(s, true, t2, len);
+    double ata[45];
+#pragma unroll
+    for (int i = 0; i < 45; ++i) ata[i] = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 p = s.p(i), q = s.q(i);
+        double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0};
+        double a[3], b[3];
+        mv3(t1, pa, a);
+        mv3(t2, pb, b);
+        double row[9] = {b[0] * a[0], b[0] * a[1], b[0], b[1] * a[0], b[1] * a[1], b[1],
+                         a[0], a[1], 1.0};
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int c = r; c < 9; ++c) ata[ut(r, c)] = ata[ut(r, c)] + row[r] * row[c];
+    }
+    double v[81];
+    sym_eigen9(ata, v);
+    double ev[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) ev[k] = ata[ut(k, k)];
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < 9; ++i)
+        if (ev[i] < ev[k]) k = i;
+    // second smallest and largest of the nine eigenvalues
+    double e1 = INFINITY, e0 = INFINITY, e8 = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double x = ev[i];
+        if (x < e0) {
+            e1 = e0;
+            e0 = x;
+        } else if (x < e1) {
+            e1 = x;
+        }
+        if (x > e8) e8 = x;
+    }
+    if (e1 <= 1e-10 * e8) return false;
+    double fn[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[9 * i + j] : acc;
+        fn[i] = acc;
+    }
+    double r2[9];
+    rank2_recompose(fn, r2);
+    double t2t[9], tmp[9], f[9];
+    tr3(t2, t2t);
+    mm3(t2t, r2, tmp);
+    mm3(tmp, t1, f);
+    double nn = fro3(f);
+    if (!(isfinite(nn) && nn > 0.0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = f[i] / nn;
+    return true;
+}
+
+// matching.rs:561-573.
+__device__ __forceinline__ double sampson_error(const double* F, double2 p, double2 q) {
+    double a[3] = {p.x, p.y, 1.0}, b[3] = {q.x, q.y, 1.0};
+    double fa[3], ftb[3];
+    mv3(F, a, fa);
+    mtv3(F, b, ftb);
+    double e = b[0] * fa[0] + b[1] * fa[1] + b[2] * fa[2];
+    double den = fa[0] * fa[0] + fa[1] * fa[1] + ftb[0] * ftb[0] + ftb[1] * ftb[1];
+    if (den <= 0.0) return INFINITY;
+    return e * e / den;
+}
+
+}  // namespace rdev
+}  // namespace skh

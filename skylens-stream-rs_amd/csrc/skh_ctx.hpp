@@ -26,6 +26,7 @@ struct Ctx {
     int device = 0;
     int num_cu = 0;
     int tile_override = -1;  // SKH_TILE_AUTO
+    uint64_t last_iters = 0;  // hypotheses drawn by the last single-pair RANSAC
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string last_error;
@@ -37,6 +38,8 @@ struct Ctx {
     // low-precision state
     DevBuf q_a, q_b, nrm_a, nrm_b, stats, ltasks, row_k1, row_k2, col_part, col_top2, queue,
         queue_len;
+    // single-pair RANSAC
+    DevBuf r_x1, r_x2, r_out, r_scratch, r_task;
 
     std::vector<RowBlock> h_blocks;
 
@@ -44,7 +47,7 @@ struct Ctx {
         return {&in_a,   &in_b,   &out_ij,    &out_len,  &tasks,    &blocks, &row_j1,
                 &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a,    &q_b,
                 &nrm_a,  &nrm_b,  &stats,     &ltasks,   &row_k1,   &row_k2, &col_part,
-                &col_top2, &queue, &queue_len};
+                &col_top2, &queue, &queue_len, &r_x1, &r_x2, &r_out, &r_scratch, &r_task};
     }
 
     ~Ctx();
@@ -61,3 +64,6 @@ struct Ctx {
 };
 
 }  // namespace skh
+
+// The opaque C handle is the context itself.
+struct skh_ctx : skh::Ctx {};

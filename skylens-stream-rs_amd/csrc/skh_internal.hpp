@@ -58,7 +58,7 @@ void launch_exact_decide(const PairTask* d_tasks, uint32_t n_pairs, const uint32
 // Compacts accepted rows of every pair into (i, j) uint32 pairs at
 // out_ij + 2 * (out_base[p]) in ascending i, and the count into out_len[p].
 void launch_compact(const PairTask* d_tasks, uint32_t n_pairs, const uint32_t* row_match,
-                    const uint32_t* out_base, uint32_t* out_ij, uint32_t* out_len,
+                    const uint64_t* out_base, uint32_t* out_ij, uint32_t* out_len,
                     hipStream_t s);
 
 }  // namespace skh

@@ -53,6 +53,9 @@ uint32_t lowp_block_rows(int tile);
 void launch_stats_reset(ViewStats* st, uint32_t n, hipStream_t s);
 void launch_quantize(const float* x, uint32_t n, uint16_t* q, uint32_t* nrm, ViewStats* st,
                      hipStream_t s);
+void launch_quantize_views(const float* x, const uint64_t* d_off, uint32_t n_views,
+                           uint32_t max_rows, uint16_t* q, uint32_t* nrm, ViewStats* st,
+                           hipStream_t s);
 void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
                       const LowpOut& out, uint32_t tbits, hipStream_t s);
 void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,

@@ -1,0 +1,36 @@
+// Shared declarations of the device ransac_fundamental.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "skylens_hip.h"
+
+namespace skh {
+
+using RansacCfgDev = skh_ransac_cfg;
+using RansacSel = skh_model_sel;
+
+// One pair's RANSAC problem. x1/x2: n correspondences (double2, device).
+// Outputs: F (row-major, 9 doubles), inliers (n bytes), found (1 = Some,
+// 0 = None), sel. scratch: ransac_scratch_words(n) words of device memory.
+struct RansacTask {
+    const double2* x1;
+    const double2* x2;
+    uint32_t n;
+    uint32_t pad;
+    double* F;
+    uint8_t* inliers;
+    int32_t* found;
+    RansacSel* sel;
+    uint32_t* scratch;
+    uint64_t* iters_out;  // hypotheses drawn (diagnostics), may be null
+    const uint32_t* n_dev;  // if set, the device-side count (<= n) replaces n
+};
+
+uint32_t ransac_scratch_words(uint32_t n);
+void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
+                   hipStream_t s);
+
+}  // namespace skh

@@ -63,7 +63,66 @@ EXPORTS = [
       ctypes.c_int, _vp, _vp]),
     ("skh_ratio_match_state", ctypes.c_int,
      [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
+    ("skh_ransac_cfg_default", None, [_vp]),
+    ("skh_ransac_fundamental", ctypes.c_int,
+     [_vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int)]),
+    ("skh_ctx_last_ransac_iters", ctypes.c_uint64, [_vp]),
+    ("skh_plan_create", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    ("skh_plan_destroy", ctypes.c_int, [_vp]),
+    ("skh_plan_run", ctypes.c_int, [_vp]),
+    ("skh_plan_sync", ctypes.c_int, [_vp]),
+    ("skh_plan_match_base", ctypes.c_int, [_vp, _vp]),
+    ("skh_plan_results", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("skh_plan_stage_ms", ctypes.c_double, [_vp, ctypes.c_int]),
+    ("skh_plan_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
 ]
+
+
+class RansacConfig(ctypes.Structure):
+    """matching.rs:762-793 (skh_ransac_cfg)."""
+    _fields_ = [("threshold_px", ctypes.c_double), ("min_inlier_ratio", ctypes.c_double),
+                ("max_iters", ctypes.c_uint64), ("confidence", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("min_inliers", ctypes.c_uint64),
+                ("max_chance_prob", ctypes.c_double)]
+
+    @classmethod
+    def default(cls, **kw) -> "RansacConfig":
+        c = cls()
+        load_library().skh_ransac_cfg_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class ModelSelectionC(ctypes.Structure):
+    _fields_ = [("has_selection", ctypes.c_int32), ("model", ctypes.c_int32),
+                ("gric_f", ctypes.c_double), ("gric_h", ctypes.c_double),
+                ("parallax", ctypes.c_uint64), ("parallax_needed", ctypes.c_uint64),
+                ("sigma_px", ctypes.c_double)]
+
+
+@dataclass
+class ModelSelection:
+    """matching.rs:380-394."""
+    model: str  # "Fundamental" | "Homography"
+    gric_f: float
+    gric_h: float
+    parallax: int
+    parallax_needed: int
+    sigma_px: float
+
+
+@dataclass
+class FundamentalFit:
+    """matching.rs:911-930: F (3x3, unit Frobenius norm, rank 2), inlier flags,
+    the model selection over the inliers, and the chosen model."""
+    f: np.ndarray
+    inliers: np.ndarray
+    selection: Optional[ModelSelection]
+    model: str
+
+    def is_planar(self) -> bool:
+        return self.model == "Homography"
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -171,6 +230,37 @@ class Context:
         self._check(rc, "skh_ratio_match_state")
         return j1, d1, d2, ci, cd
 
+    # -- ransac_fundamental ----------------------------------------------
+    def ransac_fundamental(self, x1, x2, cfg: Optional[RansacConfig] = None
+                           ) -> Optional[FundamentalFit]:
+        x1 = np.ascontiguousarray(np.asarray(x1, dtype=np.float64).reshape(-1, 2))
+        x2 = np.ascontiguousarray(np.asarray(x2, dtype=np.float64).reshape(-1, 2))
+        if x1.shape[0] != x2.shape[0]:
+            return None  # matching.rs:947: length mismatch -> None
+        cfg = cfg or RansacConfig.default()
+        n = x1.shape[0]
+        F = np.zeros(9, np.float64)
+        inl = np.zeros(max(n, 1), np.uint8)
+        sel = ModelSelectionC()
+        found = ctypes.c_int(0)
+        rc = self.lib.skh_ransac_fundamental(self.handle, _ptr(x1), _ptr(x2), n, ctypes.byref(cfg),
+                                             _ptr(F), _ptr(inl), ctypes.byref(sel),
+                                             ctypes.byref(found))
+        self._check(rc, "skh_ransac_fundamental")
+        if not found.value:
+            return None
+        ms = None
+        if sel.has_selection:
+            ms = ModelSelection("Homography" if sel.model else "Fundamental", sel.gric_f,
+                                sel.gric_h, int(sel.parallax), int(sel.parallax_needed),
+                                sel.sigma_px)
+        return FundamentalFit(F.reshape(3, 3), inl[:n].astype(bool), ms,
+                              ms.model if ms else "Fundamental")
+
+    @property
+    def last_ransac_iters(self) -> int:
+        return int(self.lib.skh_ctx_last_ransac_iters(self.handle))
+
     def ratio_match_dev(self, a_ptr: int, na: int, b_ptr: int, nb: int, ratio: float,
                         mutual: bool, precision: str, out_ij_ptr: int, out_len_ptr: int):
         """Device-pointer form: inputs/outputs already in HBM, async on `stream`."""
@@ -178,6 +268,115 @@ class Context:
                                           int(bool(mutual)), PRECISIONS[precision], out_ij_ptr,
                                           out_len_ptr)
         self._check(rc, "skh_ratio_match_dev")
+
+
+class BatchDesc(ctypes.Structure):
+    """skh_batch_desc."""
+    _fields_ = [("n_views", ctypes.c_uint32), ("view_offset", _vp), ("d_desc", _vp),
+                ("d_kp", _vp), ("n_pairs", ctypes.c_uint32), ("pairs", _vp),
+                ("ratio", ctypes.c_float), ("mutual", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("run_ransac", ctypes.c_int32),
+                ("px_offset", ctypes.c_double), ("ransac", RansacConfig)]
+
+
+@dataclass
+class BatchResult:
+    match_base: np.ndarray   # (P + 1,) uint64 capacity offsets
+    match_len: np.ndarray    # (P,) uint32
+    match_ij: np.ndarray     # (cap, 2) uint32
+    inliers: Optional[np.ndarray]
+    F: Optional[np.ndarray]  # (P, 3, 3)
+    found: Optional[np.ndarray]
+    sel: Optional[np.ndarray]
+
+    def matches(self, p: int) -> np.ndarray:
+        b = int(self.match_base[p])
+        return self.match_ij[b:b + int(self.match_len[p])]
+
+    def inlier_mask(self, p: int) -> np.ndarray:
+        b = int(self.match_base[p])
+        return self.inliers[b:b + int(self.match_len[p])].astype(bool)
+
+
+class Plan:
+    """A batched matching + RANSAC plan over device-resident views (skh_plan).
+
+    `desc_ptr`/`kp_ptr` are device addresses (e.g. torch tensor data_ptr())
+    that must outlive the plan; `view_offset` and `pairs` are host arrays.
+    """
+
+    def __init__(self, ctx: Context, view_offset, desc_ptr: int, kp_ptr: int, pairs,
+                 ratio: float = 0.8, mutual: bool = True, precision: str = "bf16",
+                 run_ransac: bool = True, px_offset: float = 0.0,
+                 cfg: Optional[RansacConfig] = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._off = np.ascontiguousarray(view_offset, dtype=np.uint64)
+        self._pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint32).reshape(-1, 2))
+        d = BatchDesc()
+        d.n_views = len(self._off) - 1
+        d.view_offset = self._off.ctypes.data
+        d.d_desc = desc_ptr
+        d.d_kp = kp_ptr
+        d.n_pairs = self._pairs.shape[0]
+        d.pairs = _ptr(self._pairs)
+        d.ratio = ratio
+        d.mutual = int(bool(mutual))
+        d.precision = PRECISIONS[precision]
+        d.run_ransac = int(bool(run_ransac))
+        d.px_offset = px_offset
+        d.ransac = cfg or RansacConfig.default()
+        self.n_pairs = d.n_pairs
+        self.run_ransac = bool(run_ransac)
+        h = _vp()
+        rc = self.lib.skh_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h))
+        ctx._check(rc, "skh_plan_create")
+        self.handle = h
+        self.match_base = np.zeros(self.n_pairs + 1, np.uint64)
+        ctx._check(self.lib.skh_plan_match_base(self.handle, _ptr(self.match_base)),
+                   "skh_plan_match_base")
+
+    def run(self):
+        self.ctx._check(self.lib.skh_plan_run(self.handle), "skh_plan_run")
+
+    def sync(self):
+        self.ctx._check(self.lib.skh_plan_sync(self.handle), "skh_plan_sync")
+
+    def set_timing(self, on: bool):
+        self.ctx._check(self.lib.skh_plan_set_timing(self.handle, int(on)), "skh_plan_set_timing")
+
+    def stage_ms(self, stage: int) -> float:
+        return float(self.lib.skh_plan_stage_ms(self.handle, stage))
+
+    def results(self) -> BatchResult:
+        P = self.n_pairs
+        cap = int(self.match_base[-1])
+        ml = np.zeros(max(P, 1), np.uint32)
+        ij = np.zeros((max(cap, 1), 2), np.uint32)
+        inl = F = found = sel = None
+        if self.run_ransac:
+            inl = np.zeros(max(cap, 1), np.uint8)
+            F = np.zeros((max(P, 1), 9), np.float64)
+            found = np.zeros(max(P, 1), np.int32)
+            sel = (ModelSelectionC * max(P, 1))()
+        rc = self.lib.skh_plan_results(
+            self.handle, _ptr(ml), _ptr(ij), _ptr(inl) if inl is not None else None,
+            _ptr(F) if F is not None else None, _ptr(found) if found is not None else None,
+            ctypes.addressof(sel) if sel is not None else None)
+        self.ctx._check(rc, "skh_plan_results")
+        return BatchResult(self.match_base, ml[:P], ij, inl, None if F is None else F[:P].reshape(P, 3, 3),
+                           None if found is None else found[:P], sel)
+
+    def close(self):
+        if self.handle:
+            self.lib.skh_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _desc(x) -> np.ndarray:
@@ -229,3 +428,8 @@ def candidate_pairs(views: Sequence[Tuple[int, int]], temporal: int, cross: int,
 def ratio_match(a, b, ratio: float, mutual: bool, precision: str = "fp32") -> List[Tuple[int, int]]:
     """matching.rs:167-215: (a index, b index) in a-index order."""
     return default_context().ratio_match(a, b, ratio, mutual, precision)
+
+
+def ransac_fundamental(x1, x2, cfg: Optional[RansacConfig] = None) -> Optional[FundamentalFit]:
+    """matching.rs:941-1104: Some(FundamentalFit) or None."""
+    return default_context().ransac_fundamental(x1, x2, cfg)
