@@ -182,9 +182,13 @@ int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* match_ij, ui
                      double* F, int32_t* found, skh_model_sel* sel);
 /* Device time (ms) of the named stage in the plan's last run, measured with
  * HIP events on the context stream when timing is enabled: stage 0 = whole
- * run, 1 = matching, 2 = RANSAC, 3 = the distance kernel alone (MFMA ranking
- * or the exact FP32 kernel). Returns -1 if unavailable. */
+ * run, 1 = matching, 2 = RANSAC. Returns -1 if unavailable. */
 double skh_plan_stage_ms(skh_plan* plan, int stage);
+/* Average duration (ms) of the distance kernel alone (MFMA ranking, or the
+ * exact FP32 kernel), launched `reps` times back to back between HIP events
+ * on the context stream, on the inputs of the plan's last run (call after a
+ * run). Synchronises. Returns -1 on error. */
+double skh_plan_time_distance(skh_plan* plan, int reps);
 /* 1 = record stage events in each run (costs nothing measurable), 0 = off. */
 int skh_plan_set_timing(skh_plan* plan, int on);
 

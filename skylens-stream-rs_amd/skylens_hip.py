@@ -75,6 +75,7 @@ EXPORTS = [
     ("skh_plan_results", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("skh_plan_stage_ms", ctypes.c_double, [_vp, ctypes.c_int]),
     ("skh_plan_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("skh_plan_time_distance", ctypes.c_double, [_vp, ctypes.c_int]),
 ]
 
 
@@ -138,6 +139,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
             raise ImportError(
                 f"libskylens_hip.so not found at {path}: build it first "
                 "(python -c 'import __graft_entry__ as g; g.build()')")
+        # torch ships its own libamdhip64.so.7 under the same SONAME as
+        # /opt/rocm's. Whichever loads first serves the whole process, and
+        # torch's HIP stack fails to start on the other one, so torch (when
+        # installed) is loaded first and this library binds to its runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         for name, res, args in EXPORTS:
             fn = getattr(lib, name)
@@ -347,6 +356,10 @@ class Plan:
 
     def stage_ms(self, stage: int) -> float:
         return float(self.lib.skh_plan_stage_ms(self.handle, stage))
+
+    def time_distance(self, reps: int = 3) -> float:
+        """Average ms of the distance kernel alone (HIP events, plan stream)."""
+        return float(self.lib.skh_plan_time_distance(self.handle, int(reps)))
 
     def results(self) -> BatchResult:
         P = self.n_pairs

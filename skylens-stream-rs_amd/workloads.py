@@ -181,12 +181,12 @@ def _candidate_pairs(views, temporal=5, cross=4, pow2_max=16) -> np.ndarray:
 
 
 def scene(seed: int, positions: int, counts: np.ndarray, outlier_lo: float, outlier_hi: float,
-          max_pairs: Optional[int] = None) -> Scene:
+          max_pairs: Optional[int] = None, pool_size: int = POOL) -> Scene:
     rng = np.random.default_rng(seed)
     W, H, F = 960, 540, 750.0
-    pool = unit_rows(rng, POOL)
-    X = np.stack([rng.uniform(-40, positions + 40, POOL), rng.uniform(-30, 30, POOL),
-                  rng.uniform(60, 80, POOL)], 1)
+    pool = unit_rows(rng, pool_size)
+    X = np.stack([rng.uniform(-40, positions + 40, pool_size), rng.uniform(-30, 30, pool_size),
+                  rng.uniform(60, 80, pool_size)], 1)
     views = [(c, p) for p in range(positions) for c in range(3)]
     yaw = {0: 0.0, 1: 15.0, 2: -15.0}
     V = len(views)

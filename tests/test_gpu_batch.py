@@ -15,12 +15,13 @@ import skylens_hip
 pytestmark = pytest.mark.gpu
 
 
-def _scene(seed, positions, lo, hi):
+def _scene(seed, positions, lo, hi, pool=1500):
     from workloads import scene
 
     rng = np.random.default_rng(seed)
     counts = rng.integers(lo, hi, 3 * positions)
-    return scene(seed, positions, counts, 0.106, 0.368)
+    # a small descriptor pool so that small views still share enough features
+    return scene(seed, positions, counts, 0.106, 0.368, pool_size=pool)
 
 
 def _oracle_pair(sc, p, px_offset, mutual):
