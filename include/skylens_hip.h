@@ -191,6 +191,12 @@ double skh_plan_stage_ms(skh_plan* plan, int stage);
 double skh_plan_time_distance(skh_plan* plan, int reps);
 /* 1 = record stage events in each run (costs nothing measurable), 0 = off. */
 int skh_plan_set_timing(skh_plan* plan, int on);
+/* RANSAC diagnostics of the last run (any pointer may be NULL): iters[p] =
+ * hypotheses drawn; phase[11 * p + k] = device wall-clock ticks of phase k
+ * (0 sampling, 1 hypotheses, 2 local optimisation, 3 replay, 4 pool LM,
+ * 5 final refits and gates, 6 line test, 7 model selection, 8 batches,
+ * 9 LO lanes, 10 total); *ticks_per_ms = the wall clock's rate. */
+int skh_plan_ransac_profile(skh_plan* plan, uint64_t* iters, uint64_t* phase, double* ticks_per_ms);
 
 #ifdef __cplusplus
 }

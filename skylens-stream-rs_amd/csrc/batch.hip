@@ -63,7 +63,7 @@ struct Plan {
     DevBuf d_row_k1, d_row_j1, d_row_k2, d_row_d1, d_row_d2, d_col_key, d_col_part, d_col_top2;
     DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
         d_kp_b;
-    DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters;
+    DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters, d_prof;
 
     hipGraph_t g_match = nullptr, g_ransac = nullptr;
     hipGraphExec_t x_match = nullptr, x_ransac = nullptr;
@@ -79,7 +79,7 @@ struct Plan {
                 &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len,
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
-                &d_rtasks,   &d_iters};
+                &d_rtasks,   &d_iters,       &d_prof};
     }
 
     ~Plan() {
@@ -238,7 +238,8 @@ int Plan::build() {
             (rc = alloc(d_found, 4ull * P, "found")) ||
             (rc = alloc(d_sel, sizeof(skh_model_sel) * P, "sel")) ||
             (rc = alloc(d_scratch, 4ull * sbase[P], "scratch")) ||
-            (rc = alloc(d_iters, 8ull * P, "iters")))
+            (rc = alloc(d_iters, 8ull * P, "iters")) ||
+            (rc = alloc(d_prof, 8ull * 11 * P, "prof")))
             return rc;
         for (uint32_t p = 0; p < P; ++p) {
             RansacTask& t = rt[p];
@@ -252,6 +253,7 @@ int Plan::build() {
             t.scratch = d_scratch.as<uint32_t>() + sbase[p];
             t.iters_out = d_iters.as<uint64_t>() + p;
             t.n_dev = d_match_len.as<uint32_t>() + p;
+            t.prof = d_prof.as<uint64_t>() + 11ull * p;
         }
         if ((rc = upload(d_rtasks, rt.data(), sizeof(RansacTask) * P, "rtasks"))) return rc;
     }
@@ -437,6 +439,30 @@ extern "C" int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* m
         (rc = get(F, plan->d_F, 72ull * P, "F")) || (rc = get(found, plan->d_found, 4ull * P, "found")) ||
         (rc = get(sel, plan->d_sel, sizeof(skh_model_sel) * P, "sel")))
         return rc;
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_ransac_profile(skh_plan* plan, uint64_t* iters, uint64_t* phase,
+                                       double* ticks_per_ms) {
+    if (!plan || !plan->d_prof.ptr) return SKH_EINVAL;
+    Ctx* ctx = plan->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const uint32_t P = plan->desc.n_pairs;
+    int rc;
+    if ((rc = ctx->hip(hipStreamSynchronize(ctx->stream), "sync"))) return rc;
+    if (iters && (rc = ctx->hip(hipMemcpy(iters, plan->d_iters.ptr, 8ull * P, hipMemcpyDeviceToHost),
+                                "iters")))
+        return rc;
+    if (phase && (rc = ctx->hip(hipMemcpy(phase, plan->d_prof.ptr, 88ull * P, hipMemcpyDeviceToHost),
+                                "prof")))
+        return rc;
+    if (ticks_per_ms) {
+        int khz = 0;
+        if ((rc = ctx->hip(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device),
+                           "clock rate")))
+            return rc;
+        *ticks_per_ms = (double)khz;
+    }
     return SKH_OK;
 }
 

@@ -502,12 +502,26 @@ __device__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, do
     return ok;
 }
 
-struct Shared {
+// Hypothesis-batch state; its LDS is reused by the homography fit and the
+// line test after the hypothesis loop.
+struct HypMem {
     uint32_t idx[NL][8];
-    double hF[NL][9];
     int hCnt[NL];
     double loF[NL][9];
     int loCnt[NL];
+};
+union PhaseMem {
+    HypMem hyp;
+    SelState hom;
+    uint32_t draws[600];
+};
+
+// Correspondences and per-lane bitsets live in LDS for pairs up to this size.
+constexpr uint32_t kLdsPts = 448;
+constexpr uint32_t kLdsWords = (kLdsPts + 31) / 32;
+constexpr uint32_t kMaskSets = 3 * NL + 8;  // lo, alt, sel per lane + pool(6) + final(2)
+
+struct Shared {
     double poolF[6][9];
     int poolCnt[6];
     int poolSlot[6];
@@ -520,7 +534,6 @@ struct Shared {
     double f[9];
     int inlCnt;
     int result;
-    uint32_t draws[600];
     int lineBest[NL];
     double es_med;
     int es_n;
@@ -537,51 +550,93 @@ __device__ bool hyp8(const RansacTask& t, const uint32_t* idx, double* F) {
     return fundamental_8pt(s, 8, F);
 }
 
-// matching.rs:993-1015 for one hypothesis; returns F_lo, its count, and its
-// inlier bitset in `inl` (sel is scratch).
-__device__ uint32_t local_opt(const RansacTask& t, const double* F0, uint32_t cnt0, double th2,
-                              double* f_lo, uint32_t* inl, uint32_t* sel) {
-    const uint32_t n = t.n;
-    double cur[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        f_lo[i] = F0[i];
-        cur[i] = F0[i];
+// One Sampson pass of F: the bitset {< th2} into `at` (and its size), and,
+// when `sel` is set, the bitset {< t2} into `sel` (size in *ns).
+__device__ uint32_t masks2(const double* F, const double2* x1, const double2* x2, uint32_t n,
+                           double th2, uint32_t* at, double t2, uint32_t* sel, uint32_t* ns) {
+    uint32_t cnt = 0, cs = 0;
+    for (uint32_t w0 = 0; w0 < n; w0 += 32) {
+        uint32_t wa = 0, ws = 0;
+        const uint32_t e = n - w0 < 32 ? n - w0 : 32;
+        for (uint32_t b = 0; b < e; ++b) {
+            const double d = sampson_error(F, x1[w0 + b], x2[w0 + b]);
+            if (d < th2) wa |= 1u << b;
+            if (d < t2) ws |= 1u << b;
+        }
+        at[w0 >> 5] = wa;
+        cnt += __popc(wa);
+        if (sel) {
+            sel[w0 >> 5] = ws;
+            cs += __popc(ws);
+        }
     }
+    if (ns) *ns = cs;
+    return cnt;
+}
+
+// matching.rs:993-1015 for one hypothesis; returns F_lo, its count, and its
+// inlier bitset in `inl`. `alt` and `sel` are scratch bitsets. The count of
+// stage r and the selection of stage r + 1 come from one Sampson pass (the
+// host evaluates the same errors twice).
+__device__ uint32_t local_opt(const RansacTask& t, const double* F0, uint32_t cnt0, double th2,
+                              double* f_lo, uint32_t* inl, uint32_t* alt, uint32_t* sel,
+                              uint32_t W) {
+    const uint32_t n = t.n;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f_lo[i] = F0[i];
     uint32_t cnt = cnt0;
+    uint32_t* cur_inl = inl;
+    uint32_t* spare = alt;
+    uint32_t ns = 0;
+    masks2(F0, t.x1, t.x2, n, th2, cur_inl, th2 * kNM[0] * kNM[0], sel, &ns);
     for (int r = 0; r < 6; ++r) {
-        const double t2 = th2 * kNM[r] * kNM[r];
-        const uint32_t ns = mask_of(cur, t.x1, t.x2, n, t2, sel);
         PtsMask s{t.x1, t.x2, n, sel};
         double g[9];
         if (!fundamental_8pt(s, ns, g)) break;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) cur[i] = g[i];
-        const uint32_t gc = count_of(g, t.x1, t.x2, n, th2);
+        const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
+        const uint32_t gc = masks2(g, t.x1, t.x2, n, th2, spare, t2n, r < 5 ? sel : nullptr, &ns);
         if (gc > cnt) {
 #pragma unroll
             for (int i = 0; i < 9; ++i) f_lo[i] = g[i];
             cnt = gc;
+            uint32_t* tmp = cur_inl;
+            cur_inl = spare;
+            spare = tmp;
         }
     }
-    mask_of(f_lo, t.x1, t.x2, n, th2, inl);
+    if (cur_inl != inl)
+        for (uint32_t w0 = 0; w0 < W; ++w0) inl[w0] = cur_inl[w0];
     return cnt;
 }
 
 __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict__ tasks,
                                                     const RansacCfgDev cfg) {
     __shared__ Shared sh;
-    __shared__ SelState hs;
+    __shared__ PhaseMem pm;
+    __shared__ double2 lds_x1[kLdsPts], lds_x2[kLdsPts];
+    __shared__ uint32_t lds_masks[kMaskSets * kLdsWords];
     RansacTask t = tasks[blockIdx.x];
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
     const int lane = threadIdx.x;
     const uint32_t n = t.n;
     const uint32_t W = words(n);
-    uint32_t* lo_masks = t.scratch;                    // NL * W
-    uint32_t* sel_masks = t.scratch + (size_t)NL * W;  // NL * W
-    uint32_t* pool_masks = t.scratch + (size_t)2 * NL * W;  // 6 * W
-    uint32_t* fin_mask = t.scratch + (size_t)(2 * NL + 6) * W;  // W
-    uint32_t* fin_sel = fin_mask + W;                            // W
+    // Small pairs run from LDS (generic pointers, so every routine is shared).
+    if (n <= kLdsPts) {
+        for (uint32_t i = lane; i < n; i += NL) {
+            lds_x1[i] = t.x1[i];
+            lds_x2[i] = t.x2[i];
+        }
+        t.x1 = lds_x1;
+        t.x2 = lds_x2;
+    }
+    uint32_t* masks = W <= kLdsWords ? lds_masks : t.scratch;
+    uint32_t* lo_masks = masks;                                 // NL * W
+    uint32_t* alt_masks = masks + (size_t)NL * W;               // NL * W
+    uint32_t* sel_masks = masks + (size_t)2 * NL * W;           // NL * W
+    uint32_t* pool_masks = masks + (size_t)3 * NL * W;          // 6 * W
+    uint32_t* fin_mask = masks + (size_t)(3 * NL + 6) * W;      // W
+    uint32_t* fin_sel = fin_mask + W;                           // W
+    __syncthreads();
 
     // matching.rs:946-949
     bool valid = n >= 8;
@@ -607,26 +662,47 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
     }
     __syncthreads();
 
+    // Phase clock (lane 0, constant-rate wall clock): sampling, hypotheses,
+    // local optimisation, replay, pool LM, final refits + gates, line test,
+    // model selection, batches, LO lanes, total.
+    uint64_t pc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_prev = t_start;
+    auto lap = [&](int k) {
+        const uint64_t now = wall_clock64();
+        pc[k] += now - t_prev;
+        t_prev = now;
+    };
+    auto flush_prof = [&]() {
+        if (lane == 0 && t.prof) {
+            pc[10] = wall_clock64() - t_start;
+            for (int k = 0; k < 11; ++k) t.prof[k] = pc[k];
+        }
+    };
+
     // ---------------------------------------------------- hypothesis batches
     while (sh.it < sh.iters) {
+        lap(3);
+        pc[8] += 1;
         if (lane == 0) {
             for (int h = 0; h < NL; ++h) {
                 int k = 0;
                 while (k < 8) {
                     uint32_t c = lcg_draw(st, n);
                     bool dup = false;
-                    for (int e = 0; e < k; ++e) dup |= sh.idx[h][e] == c;
-                    if (!dup) sh.idx[h][k++] = c;
+                    for (int e = 0; e < k; ++e) dup |= pm.hyp.idx[h][e] == c;
+                    if (!dup) pm.hyp.idx[h][k++] = c;
                 }
             }
         }
         __syncthreads();
+        lap(0);
         double F[9];
         int cnt = -1;
-        if (hyp8(t, sh.idx[lane], F)) cnt = (int)count_of(F, t.x1, t.x2, n, th2);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) sh.hF[lane][i] = F[i];
-        sh.hCnt[lane] = cnt;
+        if (hyp8(t, pm.hyp.idx[lane], F)) cnt = (int)count_of(F, t.x1, t.x2, n, th2);
+        __syncthreads();
+        lap(1);
+        pm.hyp.hCnt[lane] = cnt;
         // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
         int v = cnt >= 8 ? cnt : 0;
         int incl = v;
@@ -639,27 +715,30 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         if (lane == 0) excl = 0;
         const int bound = sh.bestCnt > excl ? sh.bestCnt : excl;
         const bool cand = cnt >= 8 && 4 * cnt >= bound;
+        pc[9] += __popcll(__ballot(cand));
         if (cand) {
             double flo[9];
-            sh.loCnt[lane] = (int)local_opt(t, F, (uint32_t)cnt, th2, flo,
-                                            lo_masks + (size_t)lane * W,
-                                            sel_masks + (size_t)lane * W);
+            pm.hyp.loCnt[lane] = (int)local_opt(t, F, (uint32_t)cnt, th2, flo,
+                                                lo_masks + (size_t)lane * W,
+                                                alt_masks + (size_t)lane * W,
+                                                sel_masks + (size_t)lane * W, W);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) sh.loF[lane][i] = flo[i];
+            for (int i = 0; i < 9; ++i) pm.hyp.loF[lane][i] = flo[i];
         } else {
-            sh.loCnt[lane] = -1;
+            pm.hyp.loCnt[lane] = -1;
         }
         __syncthreads();
+        lap(2);
         // replay in order (lane 0)
         if (lane == 0) {
             for (int h = 0; h < NL && sh.it < sh.iters; ++h) {
                 sh.it++;
-                const int c0 = sh.hCnt[h];
+                const int c0 = pm.hyp.hCnt[h];
                 if (c0 < 0) continue;
                 const int best_cnt = sh.haveBest ? sh.bestCnt : 0;
                 if (!(c0 >= 8 && 4 * c0 >= best_cnt)) continue;
-                // sh.loCnt[h] >= 0 here: the candidate bound never exceeds best_cnt
-                const int c = sh.loCnt[h];
+                // loCnt[h] >= 0 here: the candidate bound never exceeds best_cnt
+                const int c = pm.hyp.loCnt[h];
                 if (c < 0) {  // cannot happen: candidates are a superset of triggers
                     sh.result = -1;
                     sh.it = sh.iters;
@@ -668,11 +747,11 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
                 const uint32_t* inl = lo_masks + (size_t)h * W;
                 int same = -1;
                 for (int j = 0; j < sh.npool && same < 0; ++j) {
-                    const uint32_t* pm = pool_masks + (size_t)sh.poolSlot[j] * W;
+                    const uint32_t* pmk = pool_masks + (size_t)sh.poolSlot[j] * W;
                     uint32_t both = 0, either = 0;
                     for (uint32_t w0 = 0; w0 < W; ++w0) {
-                        both += __popc(pm[w0] & inl[w0]);
-                        either += __popc(pm[w0] | inl[w0]);
+                        both += __popc(pmk[w0] & inl[w0]);
+                        either += __popc(pmk[w0] | inl[w0]);
                     }
                     if (10 * both > 7 * either) same = j;
                 }
@@ -684,7 +763,7 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
                 }
                 if (target >= 0) {
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) sh.poolF[target][i] = sh.loF[h][i];
+                    for (int i = 0; i < 9; ++i) sh.poolF[target][i] = pm.hyp.loF[h][i];
                     sh.poolCnt[target] = c;
                     uint32_t* dst = pool_masks + (size_t)sh.poolSlot[target] * W;
                     for (uint32_t w0 = 0; w0 < W; ++w0) dst[w0] = inl[w0];
@@ -712,20 +791,23 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
                 if (c <= best_cnt) continue;
                 sh.iters = ipow_adaptive((double)c / (double)n, 8, cfg.confidence, cfg.max_iters);
 #pragma unroll
-                for (int i = 0; i < 9; ++i) sh.bestF[i] = sh.loF[h][i];
+                for (int i = 0; i < 9; ++i) sh.bestF[i] = pm.hyp.loF[h][i];
                 sh.bestCnt = c;
                 sh.haveBest = 1;
             }
         }
         __syncthreads();
     }
+    lap(3);
     if (lane == 0 && t.iters_out) *t.iters_out = sh.it;
     if (sh.result < 0) {
         if (lane == 0) *t.found = -1;
+        flush_prof();
         return;
     }
     if (!sh.haveBest) {
         if (lane == 0) *t.found = 0;
+        flush_prof();
         return;
     }
 
@@ -735,9 +817,9 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
 #pragma unroll
         for (int i = 0; i < 9; ++i) cur[i] = sh.poolF[lane][i];
         uint32_t* sel = sel_masks + (size_t)lane * W;
+        uint32_t* cnt_bits = alt_masks + (size_t)lane * W;
+        uint32_t ns = mask_of(cur, t.x1, t.x2, n, th2 * kNM[0] * kNM[0], sel);
         for (int r = 0; r < 6; ++r) {
-            const double t2 = th2 * kNM[r] * kNM[r];
-            const uint32_t ns = mask_of(cur, t.x1, t.x2, n, t2, sel);
             PtsMask s{t.x1, t.x2, n, sel};
             double nxt[9];
             refine_sampson(cur, s, ns, 30, nxt);
@@ -746,10 +828,14 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
                 cur[i] = nxt[i];
                 sh.stF[lane][r][i] = nxt[i];
             }
-            sh.stCnt[lane][r] = (int)count_of(cur, t.x1, t.x2, n, th2);
+            // this stage's inlier count and the next stage's selection, one pass
+            const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
+            sh.stCnt[lane][r] =
+                (int)masks2(cur, t.x1, t.x2, n, th2, cnt_bits, t2n, r < 5 ? sel : nullptr, &ns);
         }
     }
     __syncthreads();
+    lap(4);
     if (lane == 0) {
         double f[9];
 #pragma unroll
@@ -811,8 +897,10 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         sh.es_n = (int)ic;
     }
     __syncthreads();
+    lap(5);
     if (!sh.result) {
         if (lane == 0) *t.found = 0;
+        flush_prof();
         return;
     }
 
@@ -826,13 +914,13 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         }
         if (lane == 0) {
             uint64_t ls = 0x6A09E667F3BCC909ull;
-            for (int k = 0; k < 600; ++k) sh.draws[k] = lcg_draw(ls, ns);
+            for (int k = 0; k < 600; ++k) pm.draws[k] = lcg_draw(ls, ns);
         }
         __syncthreads();
         // rank -> point index among the inliers
         int best = 0;
         for (int it = lane; it < 300; it += NL) {
-            const uint32_t ri = sh.draws[2 * it], rj = sh.draws[2 * it + 1];
+            const uint32_t ri = pm.draws[2 * it], rj = pm.draws[2 * it + 1];
             uint32_t pi = 0, pj = 0, seen = 0;
             for (uint32_t i = 0; i < n; ++i) {
                 if (!((fin_mask[i >> 5] >> (i & 31)) & 1u)) continue;
@@ -868,8 +956,10 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
             break;
         }
     }
+    lap(6);
     if (!line_ok) {
         if (lane == 0) *t.found = 0;
+        flush_prof();
         return;
     }
 
@@ -926,7 +1016,8 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
             const double s2 = sigma * sigma;
             double H[9], Hi[9];
             uint32_t hc = 0;
-            const bool hok = fit_homography(s, ns, 3.0 * sigma, H, Hi, &hc, hs, fin_sel, t.x1, t.x2);
+            const bool hok =
+                fit_homography(s, ns, 3.0 * sigma, H, Hi, &hc, pm.hom, fin_sel, t.x1, t.x2);
             if (hok && lane == 0) {
                 double rho_f = 0.0, rho_h = 0.0;
                 uint64_t parallax = 0;
@@ -961,6 +1052,8 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         *t.found = 1;
     }
     for (uint32_t i = lane; i < n; i += NL) t.inliers[i] = (fin_mask[i >> 5] >> (i & 31)) & 1u;
+    lap(7);
+    flush_prof();
 }
 
 }  // namespace
@@ -971,6 +1064,9 @@ void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev
     hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg);
 }
 
-uint32_t ransac_scratch_words(uint32_t n) { return (2 * NL + 8) * words(n); }
+uint32_t ransac_scratch_words(uint32_t n) {
+    // pairs whose bitsets fit in LDS need no global scratch
+    return words(n) <= kLdsWords ? 1u : kMaskSets * words(n);
+}
 
 }  // namespace skh

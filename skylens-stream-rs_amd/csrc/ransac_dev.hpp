@@ -342,14 +342,73 @@ __device__ bool all_finite(const S& s) {
     return true;
 }
 
+// Collinearity tests and Hartley normalisers of both images in two passes.
+// The host computes nearly_collinear(x1), nearly_collinear(x2),
+// normalizer(x1), normalizer(x2) one after another; each of their sums sees
+// the same operands in the same order here, so every value is identical.
+template <class S>
+__device__ bool collinear_and_normalizers(const S& s, uint32_t len, double* t1, double* t2) {
+    double c1x = 0.0, c1y = 0.0, c2x = 0.0, c2y = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        const double2 a = s.p(i), b = s.q(i);
+        c1x = c1x + a.x;
+        c1y = c1y + a.y;
+        c2x = c2x + b.x;
+        c2y = c2y + b.y;
+    }
+    const double nn = (double)len, nc = (double)(len > 0 ? len : 1);
+    // axis_spread divides by max(len, 1), the normaliser by len: equal for len >= 1
+    c1x = c1x / nc;
+    c1y = c1y / nc;
+    c2x = c2x / nc;
+    c2y = c2y / nc;
+    double s1xx = 0.0, s1xy = 0.0, s1yy = 0.0, s2xx = 0.0, s2xy = 0.0, s2yy = 0.0;
+    double d1 = 0.0, d2 = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        const double2 a = s.p(i), b = s.q(i);
+        const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
+        s1xx += ax * ax;
+        s1xy += ax * ay;
+        s1yy += ay * ay;
+        s2xx += bx * bx;
+        s2xy += bx * by;
+        s2yy += by * by;
+        d1 = d1 + norm2(ax, ay);
+        d2 = d2 + norm2(bx, by);
+    }
+    double sp[2][3] = {{s1xx, s1xy, s1yy}, {s2xx, s2xy, s2yy}};
+#pragma unroll
+    for (int im = 0; im < 2; ++im) {
+        double sxx = sp[im][0] / nc, sxy = sp[im][1] / nc, syy = sp[im][2] / nc;
+        double tr = sxx + syy;
+        double dd = sxx - syy;
+        double disc = sqrt(dd * dd + 4.0 * sxy * sxy);
+        double hi = 0.5 * (tr + disc);
+        double lo = 0.5 * (tr - disc);
+        if (!(lo >= 0.0)) lo = 0.0;
+        double slo = sqrt(lo), shi = sqrt(hi);
+        if (!(shi > 0.0 && slo > 0.02 * shi)) return true;
+    }
+    const double m1 = d1 / nn, m2 = d2 / nn;
+    const double sc1 = m1 > 0.0 ? 1.4142135623730951 / m1 : 1.0;
+    const double sc2 = m2 > 0.0 ? 1.4142135623730951 / m2 : 1.0;
+    t1[0] = sc1; t1[1] = 0.0; t1[2] = -sc1 * c1x;
+    t1[3] = 0.0; t1[4] = sc1; t1[5] = -sc1 * c1y;
+    t1[6] = 0.0; t1[7] = 0.0; t1[8] = 1.0;
+    t2[0] = sc2; t2[1] = 0.0; t2[2] = -sc2 * c2x;
+    t2[3] = 0.0; t2[4] = sc2; t2[5] = -sc2 * c2y;
+    t2[6] = 0.0; t2[7] = 0.0; t2[8] = 1.0;
+    return false;
+}
+
 // matching.rs:512-557 over the selected points (len of them).
 template <class S>
 __device__ bool fundamental_8pt(const S& s, uint32_t len, double* F) {
     if (len < 8 || !all_finite(s)) return false;
-    if (nearly_collinear(s, false, len) || nearly_collinear(s, true, len)) return false;
     double t1[9], t2[9];
-    normalizer(s, false, t1, len);
-    normalizer(s, true, t2, len);
+    if (collinear_and_normalizers(s, len, t1, t2)) return false;
     double ata[45];
 #pragma unroll
     for (int i = 0; i < 45; ++i) ata[i] = 0.0;

@@ -27,6 +27,7 @@ struct RansacTask {
     uint32_t* scratch;
     uint64_t* iters_out;  // hypotheses drawn (diagnostics), may be null
     const uint32_t* n_dev;  // if set, the device-side count (<= n) replaces n
+    uint64_t* prof;         // if set, 11 phase clocks (wall_clock64 ticks)
 };
 
 uint32_t ransac_scratch_words(uint32_t n);

@@ -76,7 +76,11 @@ EXPORTS = [
     ("skh_plan_stage_ms", ctypes.c_double, [_vp, ctypes.c_int]),
     ("skh_plan_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("skh_plan_time_distance", ctypes.c_double, [_vp, ctypes.c_int]),
+    ("skh_plan_ransac_profile", ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
 ]
+
+RANSAC_PHASES = ["sampling", "hypotheses", "local_opt", "replay", "pool_lm", "final_gates",
+                 "line_test", "model_selection", "batches", "lo_lanes", "total"]
 
 
 class RansacConfig(ctypes.Structure):
@@ -356,6 +360,15 @@ class Plan:
 
     def stage_ms(self, stage: int) -> float:
         return float(self.lib.skh_plan_stage_ms(self.handle, stage))
+
+    def ransac_profile(self):
+        """(iters per pair, phase ticks (P, 11), ticks per ms) of the last run."""
+        it = np.zeros(max(self.n_pairs, 1), np.uint64)
+        ph = np.zeros((max(self.n_pairs, 1), 11), np.uint64)
+        tpm = ctypes.c_double(0.0)
+        self.ctx._check(self.lib.skh_plan_ransac_profile(self.handle, _ptr(it), _ptr(ph),
+                                                         ctypes.byref(tpm)), "ransac_profile")
+        return it[: self.n_pairs], ph[: self.n_pairs], tpm.value
 
     def time_distance(self, reps: int = 3) -> float:
         """Average ms of the distance kernel alone (HIP events, plan stream)."""
