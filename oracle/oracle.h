@@ -100,6 +100,8 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
                            double* F, uint8_t* inliers, orc_model_sel* sel);
 /* Diagnostics of the last orc_ransac_fundamental call on this thread:
  * hypotheses drawn, LO runs, LM calls. */
+/* LM iterations and cost evaluations of the last orc_ransac_fundamental call. */
+void orc_lm_work(uint64_t* iters, uint64_t* evals);
 void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
                            uint64_t* lm_converged, uint64_t* lm_stalled_at_start);
 

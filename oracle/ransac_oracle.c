@@ -14,10 +14,11 @@
  *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958);
  *   - Cholesky: right-looking column factorisation, forward/back solves;
  *   - 3x3 inverse: cofactor expansion.
- * Rotations are applied unconditionally with (c, s, t) = (1, 0, 0) where a
- * rotation is skipped, so a GPU lane executing the same operations produces
- * the same bits. The F bits therefore pin the HIP path to this oracle, not to
- * the Rust binary (parity with the reference is at the level of its tests).
+ * A rotation that is not due (zero or negligible element) is skipped and the
+ * element set to zero; the GPU performs the same operations in the same
+ * order and produces the same bits. The F bits therefore pin the HIP path to
+ * this oracle, not to the Rust binary (parity with the reference is at the
+ * level of its tests).
  *
  * Matrices are row-major double[9] / double[81]. All sums run in the
  * reference's order (first element first).
@@ -110,8 +111,8 @@ static inline uint64_t rnd(uint64_t* st, uint64_t m) {
 /* ------------------------------------------------------------ eigen / SVD */
 
 /* One Jacobi rotation of the symmetric 9x9 `a` (and eigenvector matrix `v`)
- * annihilating a[p][q]. (c, s, t) = (1, 0, 0) leaves everything as is but the
- * explicit zero of a[p][q]. */
+ * annihilating a[p][q]. Only called for a due rotation: the sweep skips the
+ * others and just zeroes a[p][q]. */
 static inline void jacobi_rotate9(double* a, double* v, int p, int q, double c, double s, double t) {
     const double apq = a[9 * p + q];
     for (int k = 0; k < 9; ++k) {
@@ -170,8 +171,13 @@ void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor) {
         for (int p = 0; p < 8; ++p)
             for (int q = p + 1; q < 9; ++q) {
                 double c, s, t;
-                jacobi_params(a[9 * p + p], a[9 * q + q], a[9 * p + q], sweep, &c, &s, &t);
-                jacobi_rotate9(a, v, p, q, c, s, t);
+                if (jacobi_params(a[9 * p + p], a[9 * q + q], a[9 * p + q], sweep, &c, &s, &t)) {
+                    jacobi_rotate9(a, v, p, q, c, s, t);
+                } else {
+                    /* no rotation: a zero stays, a negligible element is dropped */
+                    a[9 * p + q] = 0.0;
+                    a[9 * q + p] = 0.0;
+                }
             }
     }
     for (int k = 0; k < 9; ++k) {
@@ -430,6 +436,7 @@ static int cholesky_solve9(const double* m_in, const double* rhs, double* x) {
 }
 
 static __thread uint64_t g_lm_calls, g_lm_conv, g_lm_stall0, g_iters, g_lo_runs;
+static __thread uint64_t g_lm_iters, g_lm_evals; /* LM iterations and cost evaluations */
 
 /* matching.rs:665-760. stop: 0 = converged, 1 = stalled, 2 = cap. */
 int orc_refine_sampson(const double* F, const double* x1, const double* x2, size_t n,
@@ -460,6 +467,7 @@ int orc_refine_sampson(const double* F, const double* x1, const double* x2, size
     uint64_t taken = 0;
     int stop = 2;
     for (size_t it = 0; it < iters; ++it) {
+        g_lm_iters++;
         double jtj[81], jtr[9];
         memset(jtj, 0, sizeof jtj);
         memset(jtr, 0, sizeof jtr);
@@ -484,6 +492,7 @@ int orc_refine_sampson(const double* F, const double* x1, const double* x2, size
             for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
             if (!rank2_unit(gs, cand)) { lambda *= 10.0; continue; }
             double c = sampson_cost(cand, s1, s2, a, b, n);
+            g_lm_evals++;
             if (c < cost) {
                 memcpy(g, cand, sizeof g);
                 double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
@@ -874,6 +883,11 @@ typedef struct {
     size_t cnt;
 } hyp_t;
 
+void orc_lm_work(uint64_t* iters, uint64_t* evals) {
+    *iters = g_lm_iters;
+    *evals = g_lm_evals;
+}
+
 void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
                            uint64_t* lm_converged, uint64_t* lm_stalled_at_start) {
     if (iters) *iters = g_iters;
@@ -887,6 +901,7 @@ void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_call
 int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const orc_ransac_cfg* cfg,
                            double* F_out, uint8_t* inl_out, orc_model_sel* sel) {
     g_iters = g_lo_runs = g_lm_calls = g_lm_conv = g_lm_stall0 = 0;
+    g_lm_iters = g_lm_evals = 0;
     if (sel) memset(sel, 0, sizeof *sel);
     if (n < 8 || !all_finite2(x1, n) || !all_finite2(x2, n)) return 0;
     const double th2 = cfg->threshold_px * cfg->threshold_px;

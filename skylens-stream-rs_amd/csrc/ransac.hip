@@ -42,200 +42,59 @@ struct Pts4 {
     __device__ double2 q(uint32_t i) const { return b->b[i]; }
 };
 
-// ---------------------------------------------------------------- LM refine
 
-template <class S>
-__device__ double lm_cost(const double* g, double s1, double s2, const double* t1,
-                          const double* t2, const S& s) {
-    const double q1 = s1 * s1, q2 = s2 * s2;
-    double cost = 0.0;
-    for (uint32_t i = 0; i < s.n; ++i) {
-        if (!s.take(i)) continue;
-        double2 p = s.p(i), q = s.q(i);
-        double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
-        mv3(t1, pa, a);
-        mv3(t2, pb, b);
-        double ga[3], gtb[3];
-        mv3(g, a, ga);
-        mtv3(g, b, gtb);
-        double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-        double e;
-        if (den > 0.0)
-            e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
-        else
-            e = 0.0;
-        cost = cost + e * e;
-    }
-    return cost;
-}
+// ---------------------------------------------------------------- Cholesky
 
-// Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs.
+// Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs
+// (right-looking Cholesky, then forward and back substitution). Fully
+// expanded at compile time: every index is a constant, so L stays in
+// registers. A non-positive pivot makes it return false (the remaining
+// columns are computed but unused).
 __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-#pragma unroll
-        for (int k = 0; k < j; ++k) {
+    bool ok = true;
+    sfor<0, 9>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        sfor<0, j>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
             const double factor = -L[ut(j, k)];
-#pragma unroll
-            for (int i = j; i < 9; ++i) L[ut(i, j)] = L[ut(i, j)] + factor * L[ut(i, k)];
-        }
+            sfor<j, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                L[ut(i, j)] = L[ut(i, j)] + factor * L[ut(i, k)];
+            });
+        });
         const double diag = L[ut(j, j)];
-        if (!(diag > 0.0)) return false;
+        ok = ok && diag > 0.0;
         const double den = sqrt(diag);
         L[ut(j, j)] = den;
-#pragma unroll
-        for (int i = j + 1; i < 9; ++i) L[ut(i, j)] = L[ut(i, j)] / den;
-    }
+        sfor<j + 1, 9>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            L[ut(i, j)] = L[ut(i, j)] / den;
+        });
+    });
+    if (!ok) return false;
     double y[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) y[i] = rhs[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
+    sfor<0, 9>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
         y[i] = y[i] / L[ut(i, i)];
-#pragma unroll
-        for (int k = i + 1; k < 9; ++k) y[k] = y[k] - L[ut(k, i)] * y[i];
-    }
-#pragma unroll
-    for (int i = 8; i >= 0; --i) {
+        sfor<i + 1, 9>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            y[k] = y[k] - L[ut(k, i)] * y[i];
+        });
+    });
+    sfor<0, 9>([&](auto ic) {
+        constexpr int i = 8 - decltype(ic)::value;
         double dot = 0.0;
-#pragma unroll
-        for (int k = i + 1; k < 9; ++k) dot = dot + L[ut(k, i)] * y[k];
+        sfor<i + 1, 9>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            dot = dot + L[ut(k, i)] * y[k];
+        });
         y[i] = (y[i] - dot) / L[ut(i, i)];
-    }
+    });
 #pragma unroll
     for (int i = 0; i < 9; ++i) x[i] = y[i];
     return true;
-}
-
-// matching.rs:665-760 over the selected points (len of them).
-template <class S>
-__device__ void refine_sampson(const double* F, const S& s, uint32_t len, int iters, double* out) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) out[i] = F[i];
-    if (len < 8) return;
-    double t1[9], t2[9], t1i[9], t2i[9];
-    normalizer(s, false, t1, len);
-    normalizer(s, true, t2, len);
-    if (!inv3(t1, t1i) || !inv3(t2, t2i)) return;
-    const double s1 = t1[0], s2 = t2[0];
-    double t2it[9], tmp[9], g0[9], g[9];
-    tr3(t2i, t2it);
-    mm3(t2it, F, tmp);
-    mm3(tmp, t1i, g0);
-    if (!rank2_unit(g0, g)) return;
-    double cost = lm_cost(g, s1, s2, t1, t2, s);
-    double lambda = 1e-3;
-    const double q1 = s1 * s1, q2 = s2 * s2;
-    for (int it = 0; it < iters; ++it) {
-        double jtj[45], jtr[9];
-#pragma unroll
-        for (int i = 0; i < 45; ++i) jtj[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) jtr[i] = 0.0;
-        for (uint32_t i = 0; i < s.n; ++i) {
-            if (!s.take(i)) continue;
-            double2 p = s.p(i), q = s.q(i);
-            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
-            mv3(t1, pa, a);
-            mv3(t2, pb, b);
-            // matching.rs:591-618
-            double ga[3], gtb[3];
-            mv3(g, a, ga);
-            mtv3(g, b, gtb);
-            double e = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
-            double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-            double ri, ji[9];
-            if (den <= 0.0) {
-                ri = 0.0;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) ji[k] = 0.0;
-            } else {
-                double inv = 1.0 / sqrt(den);
-                double c = e * inv * inv * inv;
-                double u[3] = {q2 * ga[0], q2 * ga[1], 0.0};
-                double v[3] = {q1 * gtb[0], q1 * gtb[1], 0.0};
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int l = 0; l < 3; ++l)
-                        ji[3 * k + l] = b[k] * a[l] * inv - c * (u[k] * a[l] + b[k] * v[l]);
-                ri = e * inv;
-            }
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-#pragma unroll
-                for (int r = c; r < 9; ++r) jtj[ut(c, r)] = ji[c] * ji[r] + jtj[ut(c, r)];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) jtr[k] = jtr[k] + ji[k] * ri;
-        }
-        bool improved = false;
-        for (int tries = 0; tries < 8; ++tries) {
-            double m[45];
-#pragma unroll
-            for (int i = 0; i < 45; ++i) m[i] = jtj[i];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) m[ut(k, k)] = m[ut(k, k)] + lambda * (jtj[ut(k, k)] + 1e-12);
-            double nrhs[9], d[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) nrhs[k] = -jtr[k];
-            if (!cholesky_solve9(m, nrhs, d)) {
-                lambda *= 10.0;
-                continue;
-            }
-            double gs[9], cand[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
-            if (!rank2_unit(gs, cand)) {
-                lambda *= 10.0;
-                continue;
-            }
-            double c = lm_cost(cand, s1, s2, t1, t2, s);
-            if (c < cost) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) g[k] = cand[k];
-                double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
-                cost = c;
-                lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;
-                improved = rel > 1e-10;
-                break;
-            }
-            lambda *= 10.0;
-        }
-        if (!improved) break;
-    }
-    double t2t[9], fp[9];
-    tr3(t2, t2t);
-    mm3(t2t, g, tmp);
-    mm3(tmp, t1, fp);
-    double nn = fro3(fp);
-    if (isfinite(nn) && nn > 0.0) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) out[i] = fp[i] / nn;
-    }
-}
-
-// ------------------------------------------------------------- mask helpers
-
-// Writes the bitset {i : sampson(F, i) < t2} (w words) and returns its size.
-__device__ uint32_t mask_of(const double* F, const double2* x1, const double2* x2, uint32_t n,
-                            double t2, uint32_t* bits) {
-    uint32_t cnt = 0;
-    for (uint32_t w0 = 0; w0 < n; w0 += 32) {
-        uint32_t wbits = 0;
-        const uint32_t e = n - w0 < 32 ? n - w0 : 32;
-        for (uint32_t b = 0; b < e; ++b)
-            if (sampson_error(F, x1[w0 + b], x2[w0 + b]) < t2) wbits |= 1u << b;
-        bits[w0 >> 5] = wbits;
-        cnt += __popc(wbits);
-    }
-    return cnt;
-}
-
-__device__ uint32_t count_of(const double* F, const double2* x1, const double2* x2, uint32_t n,
-                             double t2) {
-    uint32_t cnt = 0;
-    for (uint32_t i = 0; i < n; ++i) cnt += sampson_error(F, x1[i], x2[i]) < t2 ? 1u : 0u;
-    return cnt;
 }
 
 // ------------------------------------------------------------- model choice
@@ -375,7 +234,7 @@ struct SelState {
 
 // matching.rs:300-369 over the points of `s` (ns of them), all lanes.
 template <class S>
-__device__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, double* Hi,
+__device__ __noinline__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, double* Hi,
                                uint32_t* cnt_out, SelState& sh, uint32_t* selbits,
                                const double2* x1, const double2* x2) {
     const int lane = threadIdx.x;
@@ -502,6 +361,381 @@ __device__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, do
     return ok;
 }
 
+// ------------------------------------------------------ wave-cooperative sums
+//
+// The workgroup is one 64-lane wave. A sum over a selected point set keeps
+// the host's order (ascending point index, one running sum per quantity):
+// the lanes compute the per-point terms of a 64-point chunk in parallel,
+// compact the selected points' terms into LDS in index order, and lane k
+// then adds term k of every staged point to its own running sum. Each sum
+// sees the same operands in the same order as the host loop, so every total
+// is bit-identical, while the per-point work runs on all 64 lanes. Values
+// that every lane computes from the same inputs are identical in all lanes.
+
+constexpr int kStageW = 10;  // doubles staged per point
+
+struct Coop {
+    double stage[NL][kStageW];
+};
+__shared__ Coop s_cp;
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    return (1ull << threadIdx.x) - 1ull;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Bits [base, base + 64) of an n-bit bitset stored in 32-bit words.
+__device__ __forceinline__ uint64_t bits64(const uint32_t* bits, uint32_t base, uint32_t n) {
+    const uint32_t w = base >> 5;
+    uint64_t m = bits[w];
+    if (base + 32 < n) m |= (uint64_t)bits[w + 1] << 32;
+    return m;
+}
+
+// acc = acc + stage[j][u] for the cnt staged points, in order. The loads of
+// a group of eight are issued before its dependent adds.
+__device__ __forceinline__ double stage_sum(int u, uint32_t cnt, double acc) {
+    uint32_t j = 0;
+    for (; j + 8 <= cnt; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = s_cp.stage[j + e][u];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc = acc + v[e];
+    }
+    for (; j < cnt; ++j) acc = acc + s_cp.stage[j][u];
+    return acc;
+}
+
+// acc = acc + stage[j][u] * stage[j][w] for the cnt staged points, in order.
+__device__ __forceinline__ double stage_dot(int u, int w, uint32_t cnt,
+                                            double acc) {
+    uint32_t j = 0;
+    for (; j + 8 <= cnt; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = s_cp.stage[j + e][u] * s_cp.stage[j + e][w];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc = acc + v[e];
+    }
+    for (; j < cnt; ++j) acc = acc + s_cp.stage[j][u] * s_cp.stage[j][w];
+    return acc;
+}
+
+// (row, col) of upper-triangle slot k (ut order), k < 45.
+__device__ __forceinline__ void ut_rc(int k, int& r, int& c) {
+    r = 0;
+    while (k >= 9 - r) {
+        k -= 9 - r;
+        ++r;
+    }
+    c = r + k;
+}
+
+// Sampson errors of F over all n points, one pass: the bitset {d < th2}
+// into `at` and its size (returned); when `sel` is set, the bitset {d < t2}
+// into `sel` and its size into *ns.
+__device__ uint32_t coop_masks2(const double* F, const double2* x1, const double2* x2, uint32_t n,
+                                double th2, uint32_t* at, double t2, uint32_t* sel,
+                                uint32_t* ns) {
+    const int lane = threadIdx.x;
+    uint32_t cnt = 0, cs = 0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint32_t i = base + lane;
+        bool a = false, b = false;
+        if (i < n) {
+            const double d = sampson_error(F, x1[i], x2[i]);
+            a = d < th2;
+            b = d < t2;
+        }
+        const uint64_t ma = __ballot(a), mb = __ballot(b);
+        if (lane < 2 && base + 32u * lane < n) {
+            const uint32_t w = (base >> 5) + lane;
+            at[w] = lane ? (uint32_t)(ma >> 32) : (uint32_t)ma;
+            if (sel) sel[w] = lane ? (uint32_t)(mb >> 32) : (uint32_t)mb;
+        }
+        cnt += (uint32_t)__popcll(ma);
+        cs += (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (ns) *ns = cs;
+    return cnt;
+}
+
+__device__ __forceinline__ uint32_t coop_mask_of(const double* F, const double2* x1,
+                                                 const double2* x2, uint32_t n, double t2,
+                                                 uint32_t* bits) {
+    return coop_masks2(F, x1, x2, n, t2, bits, 0.0, nullptr, nullptr);
+}
+
+__device__ uint32_t coop_count_of(const double* F, const double2* x1, const double2* x2,
+                                  uint32_t n, double t2) {
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint32_t i = base + threadIdx.x;
+        const bool a = i < n && sampson_error(F, x1[i], x2[i]) < t2;
+        cnt += (uint32_t)__popcll(__ballot(a));
+    }
+    return cnt;
+}
+
+// collinear_and_normalizers over the bitset `sel` (len points): fills the
+// Hartley normalisers t1, t2 and returns whether either image is nearly
+// collinear.
+__device__ bool coop_norm(const double2* x1, const double2* x2, uint32_t n,
+                          const uint32_t* sel, uint32_t len, double* t1, double* t2) {
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 a = x1[base + lane], b = x2[base + lane];
+            s_cp.stage[slot][0] = a.x;
+            s_cp.stage[slot][1] = a.y;
+            s_cp.stage[slot][2] = b.x;
+            s_cp.stage[slot][3] = b.y;
+        }
+        __syncthreads();
+        acc = stage_sum(lane & 3, (uint32_t)__popcll(m), acc);
+        __syncthreads();
+    }
+    const double nc = (double)(len > 0 ? len : 1);
+    const double c1x = readlane_d(acc, 0) / nc, c1y = readlane_d(acc, 1) / nc;
+    const double c2x = readlane_d(acc, 2) / nc, c2y = readlane_d(acc, 3) / nc;
+    acc = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 a = x1[base + lane], b = x2[base + lane];
+            const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
+            s_cp.stage[slot][0] = ax * ax;
+            s_cp.stage[slot][1] = ax * ay;
+            s_cp.stage[slot][2] = ay * ay;
+            s_cp.stage[slot][3] = bx * bx;
+            s_cp.stage[slot][4] = bx * by;
+            s_cp.stage[slot][5] = by * by;
+            s_cp.stage[slot][6] = norm2(ax, ay);
+            s_cp.stage[slot][7] = norm2(bx, by);
+        }
+        __syncthreads();
+        acc = stage_sum(lane & 7, (uint32_t)__popcll(m), acc);
+        __syncthreads();
+    }
+    double q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) q[k] = readlane_d(acc, k);
+    return collinear_finish(c1x, c1y, c2x, c2y, q, len, t1, t2);
+}
+
+// A^T A of the normalised 8-point system over `sel` (matching.rs:522-538);
+// lane k < 45 returns upper-triangle entry k.
+__device__ double coop_ata(const double2* x1, const double2* x2, uint32_t n,
+                           const uint32_t* sel, const double* t1, const double* t2) {
+    const int lane = threadIdx.x;
+    int r, c;
+    ut_rc(lane < 45 ? lane : 0, r, c);
+    double acc = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 p = x1[base + lane], q = x2[base + lane];
+            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+            mv3(t1, pa, a);
+            mv3(t2, pb, b);
+            double* row = s_cp.stage[slot];
+            row[0] = b[0] * a[0];
+            row[1] = b[0] * a[1];
+            row[2] = b[0];
+            row[3] = b[1] * a[0];
+            row[4] = b[1] * a[1];
+            row[5] = b[1];
+            row[6] = a[0];
+            row[7] = a[1];
+            row[8] = 1.0;
+        }
+        __syncthreads();
+        acc = stage_dot(r, c, (uint32_t)__popcll(m), acc);
+        __syncthreads();
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- LM refine
+
+// Sum of squared Sampson residuals of the normalised g over `sel`
+// (matching.rs:620-641 summed as the host does).
+__device__ double coop_lm_cost(const double* g, double s1, double s2, const double* t1,
+                               const double* t2, const double2* x1, const double2* x2,
+                               uint32_t n, const uint32_t* sel) {
+    const int lane = threadIdx.x;
+    const double q1 = s1 * s1, q2 = s2 * s2;
+    double cost = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 p = x1[base + lane], q = x2[base + lane];
+            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+            mv3(t1, pa, a);
+            mv3(t2, pb, b);
+            double ga[3], gtb[3];
+            mv3(g, a, ga);
+            mtv3(g, b, gtb);
+            double den =
+                q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+            double e;
+            if (den > 0.0)
+                e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
+            else
+                e = 0.0;
+            s_cp.stage[slot][0] = e * e;
+        }
+        __syncthreads();
+        cost = stage_sum(0, (uint32_t)__popcll(m), cost);
+        __syncthreads();
+    }
+    return cost;
+}
+
+// J^T J (lanes 0..44, upper-triangle slots) and J^T r (lanes 45..53) of the
+// Sampson residuals over `sel` (matching.rs:585-618, :690-706).
+__device__ double coop_normal_eq(const double* g, double s1, double s2,
+                                 const double* t1, const double* t2, const double2* x1,
+                                 const double2* x2, uint32_t n, const uint32_t* sel) {
+    const int lane = threadIdx.x;
+    const double q1 = s1 * s1, q2 = s2 * s2;
+    int u = 0, w = 0;
+    if (lane < 45)
+        ut_rc(lane, u, w);
+    else if (lane < 54) {
+        u = lane - 45;
+        w = 9;
+    }
+    double acc = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 p = x1[base + lane], q = x2[base + lane];
+            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+            mv3(t1, pa, a);
+            mv3(t2, pb, b);
+            double ga[3], gtb[3];
+            mv3(g, a, ga);
+            mtv3(g, b, gtb);
+            double e = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
+            double den =
+                q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+            double* row = s_cp.stage[slot];
+            if (den <= 0.0) {
+#pragma unroll
+                for (int k = 0; k < 10; ++k) row[k] = 0.0;
+            } else {
+                double inv = 1.0 / sqrt(den);
+                double c = e * inv * inv * inv;
+                double uu[3] = {q2 * ga[0], q2 * ga[1], 0.0};
+                double vv[3] = {q1 * gtb[0], q1 * gtb[1], 0.0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l)
+                        row[3 * k + l] = b[k] * a[l] * inv - c * (uu[k] * a[l] + b[k] * vv[l]);
+                row[9] = e * inv;
+            }
+        }
+        __syncthreads();
+        acc = stage_dot(u, w, (uint32_t)__popcll(m), acc);
+        __syncthreads();
+    }
+    return acc;
+}
+
+// refine_sampson (matching.rs:661-760) over the bitset `sel` (len points),
+// all lanes; out receives the refined F (or F unchanged).
+__device__ __noinline__ void coop_refine_sampson(const double* F, const double2* x1,
+                                    const double2* x2, uint32_t n, const uint32_t* sel,
+                                    uint32_t len, int iters, double* out) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = F[i];
+    if (len < 8) return;
+    double t1[9], t2[9], t1i[9], t2i[9];
+    coop_norm(x1, x2, n, sel, len, t1, t2);
+    if (!inv3(t1, t1i) || !inv3(t2, t2i)) return;
+    const double s1 = t1[0], s2 = t2[0];
+    double t2it[9], tmp[9], g0[9], g[9];
+    tr3(t2i, t2it);
+    mm3(t2it, F, tmp);
+    mm3(tmp, t1i, g0);
+    if (!rank2_unit(g0, g)) return;
+    double cost = coop_lm_cost(g, s1, s2, t1, t2, x1, x2, n, sel);
+    double lambda = 1e-3;
+    for (int it = 0; it < iters; ++it) {
+        const double ne = coop_normal_eq(g, s1, s2, t1, t2, x1, x2, n, sel);
+        double jtj[45], jtr[9];
+#pragma unroll
+        for (int k = 0; k < 45; ++k) jtj[k] = readlane_d(ne, k);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) jtr[k] = readlane_d(ne, 45 + k);
+        bool improved = false;
+        for (int tries = 0; tries < 8; ++tries) {
+            double m[45];
+#pragma unroll
+            for (int i = 0; i < 45; ++i) m[i] = jtj[i];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[ut(k, k)] = m[ut(k, k)] + lambda * (jtj[ut(k, k)] + 1e-12);
+            double nrhs[9], d[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) nrhs[k] = -jtr[k];
+            if (!cholesky_solve9(m, nrhs, d)) {
+                lambda *= 10.0;
+                continue;
+            }
+            double gs[9], cand[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
+            if (!rank2_unit(gs, cand)) {
+                lambda *= 10.0;
+                continue;
+            }
+            double c = coop_lm_cost(cand, s1, s2, t1, t2, x1, x2, n, sel);
+            if (c < cost) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) g[k] = cand[k];
+                double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
+                cost = c;
+                lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;
+                improved = rel > 1e-10;
+                break;
+            }
+            lambda *= 10.0;
+        }
+        if (!improved) break;
+    }
+    double t2t[9], fp[9];
+    tr3(t2, t2t);
+    mm3(t2t, g, tmp);
+    mm3(tmp, t1, fp);
+    double nn = fro3(fp);
+    if (isfinite(nn) && nn > 0.0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = fp[i] / nn;
+    }
+}
+
 // Hypothesis-batch state; its LDS is reused by the homography fit and the
 // line test after the hypothesis loop.
 struct HypMem {
@@ -509,17 +743,26 @@ struct HypMem {
     int hCnt[NL];
     double loF[NL][9];
     int loCnt[NL];
+    uint32_t loNs[NL];
 };
 union PhaseMem {
     HypMem hyp;
     SelState hom;
     uint32_t draws[600];
 };
+__shared__ PhaseMem s_pm;
 
-// Correspondences and per-lane bitsets live in LDS for pairs up to this size.
-constexpr uint32_t kLdsPts = 448;
+// Correspondences and per-hypothesis bitsets live in LDS for pairs up to
+// this size; larger pairs read them from L2-resident device memory.
+#ifndef SKH_RANSAC_LDS_PTS
+#define SKH_RANSAC_LDS_PTS 384
+#endif
+#ifndef SKH_RANSAC_WAVES_PER_EU
+#define SKH_RANSAC_WAVES_PER_EU 1
+#endif
+constexpr uint32_t kLdsPts = SKH_RANSAC_LDS_PTS;
 constexpr uint32_t kLdsWords = (kLdsPts + 31) / 32;
-constexpr uint32_t kMaskSets = 3 * NL + 8;  // lo, alt, sel per lane + pool(6) + final(2)
+constexpr uint32_t kMaskSets = 3 * NL + 8;  // lo, alt, sel per hypothesis + pool(6) + final(2)
 
 struct Shared {
     double poolF[6][9];
@@ -537,84 +780,160 @@ struct Shared {
     int lineBest[NL];
     double es_med;
     int es_n;
-    double redA[NL], redB[NL];
+    double redA[NL];
 };
 
-__device__ bool hyp8(const RansacTask& t, const uint32_t* idx, double* F) {
-    Pts8 s;
+__device__ __forceinline__ void load9(const double* src, double* dst) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        s.a[e] = t.x1[idx[e]];
-        s.b[e] = t.x2[idx[e]];
-    }
-    return fundamental_8pt(s, 8, F);
+    for (int i = 0; i < 9; ++i) dst[i] = src[i];
 }
 
-// One Sampson pass of F: the bitset {< th2} into `at` (and its size), and,
-// when `sel` is set, the bitset {< t2} into `sel` (size in *ns).
-__device__ uint32_t masks2(const double* F, const double2* x1, const double2* x2, uint32_t n,
-                           double th2, uint32_t* at, double t2, uint32_t* sel, uint32_t* ns) {
-    uint32_t cnt = 0, cs = 0;
-    for (uint32_t w0 = 0; w0 < n; w0 += 32) {
-        uint32_t wa = 0, ws = 0;
-        const uint32_t e = n - w0 < 32 ? n - w0 : 32;
-        for (uint32_t b = 0; b < e; ++b) {
-            const double d = sampson_error(F, x1[w0 + b], x2[w0 + b]);
-            if (d < th2) wa |= 1u << b;
-            if (d < t2) ws |= 1u << b;
-        }
-        at[w0 >> 5] = wa;
-        cnt += __popc(wa);
-        if (sel) {
-            sel[w0 >> 5] = ws;
-            cs += __popc(ws);
-        }
-    }
-    if (ns) *ns = cs;
-    return cnt;
-}
-
-// matching.rs:993-1015 for one hypothesis; returns F_lo, its count, and its
-// inlier bitset in `inl`. `alt` and `sel` are scratch bitsets. The count of
-// stage r and the selection of stage r + 1 come from one Sampson pass (the
-// host evaluates the same errors twice).
-__device__ uint32_t local_opt(const RansacTask& t, const double* F0, uint32_t cnt0, double th2,
-                              double* f_lo, uint32_t* inl, uint32_t* alt, uint32_t* sel,
-                              uint32_t W) {
+// Lanes 2c and 2c+1 solve hypothesis 32 * round + c together (both
+// accumulate its 8-point system, each carries five eigenvector rows) and
+// split its Sampson count; results: hCnt[h] (-1 when fundamental_8pt gives
+// None) and the hypothesis F in loF[h].
+__device__ __noinline__ void hypotheses_pairs(const RansacTask& t, HypMem& hm, double th2) {
+    const double2* x1 = t.x1;
+    const double2* x2 = t.x2;
     const uint32_t n = t.n;
+    const int lane = threadIdx.x;
+#pragma unroll 1
+    for (int round = 0; round < 2; ++round) {
+        const int h = 32 * round + (lane >> 1);
+        PtsIdx8 s{x1, x2, hm.idx[h]};
+        double t1[9], t2[9], ata[45], F[9];
+        int cnt = -1;
+        if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_pair(ata, t1, t2, F)) {
+            uint32_t c = 0;
+            for (uint32_t i = lane & 1; i < n; i += 2)
+                c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
+            c += __shfl_xor(c, 1);
+            cnt = (int)c;
+            if ((lane & 1) == 0) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) f_lo[i] = F0[i];
-    uint32_t cnt = cnt0;
-    uint32_t* cur_inl = inl;
-    uint32_t* spare = alt;
-    uint32_t ns = 0;
-    masks2(F0, t.x1, t.x2, n, th2, cur_inl, th2 * kNM[0] * kNM[0], sel, &ns);
-    for (int r = 0; r < 6; ++r) {
-        PtsMask s{t.x1, t.x2, n, sel};
-        double g[9];
-        if (!fundamental_8pt(s, ns, g)) break;
-        const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
-        const uint32_t gc = masks2(g, t.x1, t.x2, n, th2, spare, t2n, r < 5 ? sel : nullptr, &ns);
-        if (gc > cnt) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) f_lo[i] = g[i];
-            cnt = gc;
-            uint32_t* tmp = cur_inl;
-            cur_inl = spare;
-            spare = tmp;
+                for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
+            }
         }
+        if ((lane & 1) == 0) hm.hCnt[h] = cnt;
     }
-    if (cur_inl != inl)
-        for (uint32_t w0 = 0; w0 < W; ++w0) inl[w0] = cur_inl[w0];
-    return cnt;
 }
 
-__global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict__ tasks,
+// Local optimisation (matching.rs:993-1015) of every candidate in `cand`
+// (hypothesis lanes), stage by stage: for each stage the candidates' point
+// sums run wave-cooperatively one candidate after another, then pairs of
+// lanes solve up to 32 candidates' 8-point systems at once, then each
+// candidate's Sampson pass gives its count at x1 and its next selection.
+// Results: loF[h], loCnt[h] and the inlier bitset lo + h * W.
+__device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, uint64_t cand,
+                                            double th2, uint32_t* lo, uint32_t* alt,
+                                            uint32_t* selb, uint32_t W) {
+    const double2* x1 = t.x1;
+    const double2* x2 = t.x2;
+    const uint32_t n = t.n;
+    const int lane = threadIdx.x;
+    // stage 0 selection (x3) and the x1 mask of the hypothesis itself
+    for (uint64_t m = cand; m; m &= m - 1) {
+        const int h = __ffsll((unsigned long long)m) - 1;
+        double F0[9];
+        load9(hm.loF[h], F0);
+        uint32_t ns = 0;
+        coop_masks2(F0, x1, x2, n, th2, lo + (size_t)h * W, th2 * kNM[0] * kNM[0],
+                    selb + (size_t)h * W, &ns);
+        if (lane == 0) {
+            hm.loCnt[h] = hm.hCnt[h];
+            hm.loNs[h] = ns;
+        }
+    }
+    __syncthreads();
+    uint64_t active = cand, swapped = 0;
+    for (int r = 0; r < 6 && active; ++r) {
+        const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
+        uint64_t todo = active;
+        while (todo) {
+            // up to 32 candidates, one pair of lanes each
+            uint64_t filled = 0;
+            int rank = 0;
+            double ata[45], tt1[9], tt2[9];
+#pragma unroll
+            for (int k = 0; k < 45; ++k) ata[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) tt1[k] = tt2[k] = 0.0;
+            bool mine = false;
+            while (todo && rank < NL / 2) {
+                const int h = __ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1;
+                const uint32_t ns = hm.loNs[h];
+                const uint32_t* sel = selb + (size_t)h * W;
+                double t1[9], t2[9];
+                // fundamental_8pt: fewer than 8 points or collinear -> None, LO stops
+                if (ns < 8 || coop_norm(x1, x2, n, sel, ns, t1, t2)) {
+                    active &= ~(1ull << h);
+                    continue;
+                }
+                const double a = coop_ata(x1, x2, n, sel, t1, t2);
+                const bool me = (lane >> 1) == rank;
+#pragma unroll
+                for (int k = 0; k < 45; ++k) {
+                    const double v = readlane_d(a, k);
+                    ata[k] = me ? v : ata[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    tt1[k] = me ? t1[k] : tt1[k];
+                    tt2[k] = me ? t2[k] : tt2[k];
+                }
+                mine = mine || me;
+                filled |= 1ull << h;
+                ++rank;
+            }
+            double g[9];
+            int ok = 0;
+            if (mine) ok = solve_8pt_pair(ata, tt1, tt2, g) ? 1 : 0;
+            rank = 0;
+            for (uint64_t m = filled; m; m &= m - 1, ++rank) {
+                const int h = __ffsll((unsigned long long)m) - 1;
+                if (!__builtin_amdgcn_readlane(ok, 2 * rank)) {
+                    active &= ~(1ull << h);
+                    continue;
+                }
+                double gh[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) gh[k] = readlane_d(g[k], 2 * rank);
+                const bool sw = (swapped >> h) & 1u;
+                uint32_t* spare = (sw ? lo : alt) + (size_t)h * W;
+                uint32_t ns = 0;
+                const uint32_t gc = coop_masks2(gh, x1, x2, n, th2, spare, t2n,
+                                                r < 5 ? selb + (size_t)h * W : nullptr, &ns);
+                const int cur = hm.loCnt[h];
+                __syncthreads();
+                if (lane == 0) hm.loNs[h] = ns;
+                if ((int)gc > cur) {
+                    if (lane == 0) {
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) hm.loF[h][k] = gh[k];
+                        hm.loCnt[h] = (int)gc;
+                    }
+                    swapped ^= 1ull << h;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // the current inlier bitset of a swapped candidate sits in its alt slot
+    for (uint64_t m = swapped; m; m &= m - 1) {
+        const int h = __ffsll((unsigned long long)m) - 1;
+        for (uint32_t w0 = lane; w0 < W; w0 += NL) lo[(size_t)h * W + w0] = alt[(size_t)h * W + w0];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(NL) __attribute__((amdgpu_waves_per_eu(SKH_RANSAC_WAVES_PER_EU)))
+void ransac_kernel(const RansacTask* __restrict__ tasks,
                                                     const RansacCfgDev cfg) {
     __shared__ Shared sh;
-    __shared__ PhaseMem pm;
-    __shared__ double2 lds_x1[kLdsPts], lds_x2[kLdsPts];
-    __shared__ uint32_t lds_masks[kMaskSets * kLdsWords];
+    PhaseMem& pm = s_pm;
+    __shared__ double2 lds_x1[kLdsPts > 0 ? kLdsPts : 1], lds_x2[kLdsPts > 0 ? kLdsPts : 1];
+    __shared__ uint32_t lds_masks[kLdsWords > 0 ? kMaskSets * kLdsWords : 1];
     RansacTask t = tasks[blockIdx.x];
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
     const int lane = threadIdx.x;
@@ -629,7 +948,7 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         t.x1 = lds_x1;
         t.x2 = lds_x2;
     }
-    uint32_t* masks = W <= kLdsWords ? lds_masks : t.scratch;
+    uint32_t* masks = W <= kLdsWords && kLdsWords > 0 ? lds_masks : t.scratch;
     uint32_t* lo_masks = masks;                                 // NL * W
     uint32_t* alt_masks = masks + (size_t)NL * W;               // NL * W
     uint32_t* sel_masks = masks + (size_t)2 * NL * W;           // NL * W
@@ -664,13 +983,14 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
 
     // Phase clock (lane 0, constant-rate wall clock): sampling, hypotheses,
     // local optimisation, replay, pool LM, final refits + gates, line test,
-    // model selection, batches, LO lanes, total.
-    uint64_t pc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // model selection, batches, LO candidates, total.
+    __shared__ uint64_t pc[11];
+    if (lane < 11) pc[lane] = 0;
     const uint64_t t_start = wall_clock64();
     uint64_t t_prev = t_start;
     auto lap = [&](int k) {
         const uint64_t now = wall_clock64();
-        pc[k] += now - t_prev;
+        if (lane == 0) pc[k] += now - t_prev;
         t_prev = now;
     };
     auto flush_prof = [&]() {
@@ -683,26 +1003,33 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
     // ---------------------------------------------------- hypothesis batches
     while (sh.it < sh.iters) {
         lap(3);
-        pc[8] += 1;
+        if (lane == 0) pc[8] += 1;
         if (lane == 0) {
+            // matching.rs:971-979: eight distinct draws per hypothesis
             for (int h = 0; h < NL; ++h) {
+                uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 int k = 0;
                 while (k < 8) {
-                    uint32_t c = lcg_draw(st, n);
+                    const uint32_t c = lcg_draw(st, n);
                     bool dup = false;
-                    for (int e = 0; e < k; ++e) dup |= pm.hyp.idx[h][e] == c;
-                    if (!dup) pm.hyp.idx[h][k++] = c;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) dup |= e < k && s[e] == c;
+                    if (!dup) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) s[e] = e == k ? c : s[e];
+                        ++k;
+                    }
                 }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) pm.hyp.idx[h][e] = s[e];
             }
         }
         __syncthreads();
         lap(0);
-        double F[9];
-        int cnt = -1;
-        if (hyp8(t, pm.hyp.idx[lane], F)) cnt = (int)count_of(F, t.x1, t.x2, n, th2);
+        hypotheses_pairs(t, pm.hyp, th2);
         __syncthreads();
         lap(1);
-        pm.hyp.hCnt[lane] = cnt;
+        const int cnt = pm.hyp.hCnt[lane];
         // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
         int v = cnt >= 8 ? cnt : 0;
         int incl = v;
@@ -715,19 +1042,11 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         if (lane == 0) excl = 0;
         const int bound = sh.bestCnt > excl ? sh.bestCnt : excl;
         const bool cand = cnt >= 8 && 4 * cnt >= bound;
-        pc[9] += __popcll(__ballot(cand));
-        if (cand) {
-            double flo[9];
-            pm.hyp.loCnt[lane] = (int)local_opt(t, F, (uint32_t)cnt, th2, flo,
-                                                lo_masks + (size_t)lane * W,
-                                                alt_masks + (size_t)lane * W,
-                                                sel_masks + (size_t)lane * W, W);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) pm.hyp.loF[lane][i] = flo[i];
-        } else {
-            pm.hyp.loCnt[lane] = -1;
-        }
+        const uint64_t cmask = __ballot(cand);
+        if (lane == 0) pc[9] += __popcll(cmask);
+        if (!cand) pm.hyp.loCnt[lane] = -1;
         __syncthreads();
+        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W);
         lap(2);
         // replay in order (lane 0)
         if (lane == 0) {
@@ -812,70 +1131,77 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
     }
 
     // ------------------------------------------- pool LM (matching.rs:1041-1057)
-    if (lane < sh.npool) {
-        double cur[9];
+    {
+        uint32_t* sel = sel_masks;
+        uint32_t* cnt_bits = alt_masks;
+        const int npool = sh.npool;
+        for (int p = 0; p < npool; ++p) {
+            double cur[9];
+            load9(sh.poolF[p], cur);
+            uint32_t ns = coop_mask_of(cur, t.x1, t.x2, n, th2 * kNM[0] * kNM[0], sel);
+            for (int r = 0; r < 6; ++r) {
+                double nxt[9];
+                coop_refine_sampson(cur, t.x1, t.x2, n, sel, ns, 30, nxt);
+                load9(nxt, cur);
+                if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) cur[i] = sh.poolF[lane][i];
-        uint32_t* sel = sel_masks + (size_t)lane * W;
-        uint32_t* cnt_bits = alt_masks + (size_t)lane * W;
-        uint32_t ns = mask_of(cur, t.x1, t.x2, n, th2 * kNM[0] * kNM[0], sel);
-        for (int r = 0; r < 6; ++r) {
-            PtsMask s{t.x1, t.x2, n, sel};
-            double nxt[9];
-            refine_sampson(cur, s, ns, 30, nxt);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                cur[i] = nxt[i];
-                sh.stF[lane][r][i] = nxt[i];
+                    for (int k = 0; k < 9; ++k) sh.stF[p][r][k] = nxt[k];
+                }
+                // this stage's inlier count and the next stage's selection, one pass
+                const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
+                const uint32_t c = coop_masks2(cur, t.x1, t.x2, n, th2, cnt_bits, t2n,
+                                               r < 5 ? sel : nullptr, &ns);
+                if (lane == 0) sh.stCnt[p][r] = (int)c;
             }
-            // this stage's inlier count and the next stage's selection, one pass
-            const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
-            sh.stCnt[lane][r] =
-                (int)masks2(cur, t.x1, t.x2, n, th2, cnt_bits, t2n, r < 5 ? sel : nullptr, &ns);
         }
     }
     __syncthreads();
     lap(4);
-    if (lane == 0) {
+    {
         double f[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) f[i] = sh.bestF[i];
+        load9(sh.bestF, f);
         int ic = sh.bestCnt;
         for (int p = 0; p < sh.npool; ++p)
             for (int r = 0; r < 6; ++r)
                 if (sh.stCnt[p][r] > ic) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) f[i] = sh.stF[p][r][i];
+                    load9(sh.stF[p][r], f);
                     ic = sh.stCnt[p][r];
                 }
         // matching.rs:1059-1069
         for (int r = 0; r < 2; ++r) {
-            const uint32_t ns = mask_of(f, t.x1, t.x2, n, th2, fin_sel);
-            PtsMask s{t.x1, t.x2, n, fin_sel};
+            const uint32_t ns = coop_mask_of(f, t.x1, t.x2, n, th2, fin_sel);
             double g[9];
-            refine_sampson(f, s, ns, 30, g);
-            const uint32_t c = count_of(g, t.x1, t.x2, n, th2);
+            coop_refine_sampson(f, t.x1, t.x2, n, fin_sel, ns, 30, g);
+            const uint32_t c = coop_count_of(g, t.x1, t.x2, n, th2);
             if (c < ns) break;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) f[i] = g[i];
+            load9(g, f);
             ic = (int)c;
         }
+        if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sh.f[i] = f[i];
-        sh.inlCnt = ic;
-        mask_of(f, t.x1, t.x2, n, th2, fin_mask);
+            for (int k = 0; k < 9; ++k) sh.f[k] = f[k];
+        }
+        coop_mask_of(f, t.x1, t.x2, n, th2, fin_mask);
         // matching.rs:1071-1080
-        int ok = 1;
+        bool ok = true;
         const uint64_t minc = cfg.min_inliers > 8 ? cfg.min_inliers : 8;
-        if ((uint64_t)ic < minc || (double)ic < cfg.min_inlier_ratio * (double)n) ok = 0;
+        if ((uint64_t)ic < minc || (double)ic < cfg.min_inlier_ratio * (double)n) ok = false;
         if (ok && cfg.max_chance_prob > 0.0) {
             double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY;
-            for (uint32_t i = 0; i < n; ++i) {
-                double2 q = t.x2[i];
+            for (uint32_t i = lane; i < n; i += NL) {
+                const double2 q = t.x2[i];
                 lox = fmin(lox, q.x);
                 loy = fmin(loy, q.y);
                 hix = fmax(hix, q.x);
                 hiy = fmax(hiy, q.y);
+            }
+            // min / max of finite values: any order gives the host's result
+#pragma unroll
+            for (int off = 1; off < NL; off <<= 1) {
+                lox = fmin(lox, __shfl_xor(lox, off));
+                loy = fmin(loy, __shfl_xor(loy, off));
+                hix = fmax(hix, __shfl_xor(hix, off));
+                hiy = fmax(hiy, __shfl_xor(hiy, off));
             }
             double dx = hix - lox, dy = hiy - loy;
             double area = dx * dy;
@@ -886,15 +1212,16 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
             }
             double chance = (uint64_t)ic <= 8 ? 1.0
                                               : binomial_tail(n > 8 ? n - 8 : 0, (uint64_t)ic - 8, p);
-            if (chance > cfg.max_chance_prob) ok = 0;
+            if (chance > cfg.max_chance_prob) ok = false;
         }
         if (ok) {
-            PtsMask s{t.x1, t.x2, n, fin_mask};
-            const uint32_t ns = (uint32_t)ic;
-            if (nearly_collinear(s, false, ns) || nearly_collinear(s, true, ns)) ok = 0;
+            double t1[9], t2[9];
+            if (coop_norm(t.x1, t.x2, n, fin_mask, (uint32_t)ic, t1, t2)) ok = false;
         }
-        sh.result = ok;
-        sh.es_n = (int)ic;
+        if (lane == 0) {
+            sh.result = ok ? 1 : 0;
+            sh.es_n = ic;
+        }
     }
     __syncthreads();
     lap(5);
@@ -972,10 +1299,25 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
         // F entries are finite (F is normalised); coordinates are finite.
         double sigma_hat = 0.0;
         if (has) {
-            // median of the finite Sampson errors over the inliers, rank ns_f / 2
-            int nf = 0;
-            for (uint32_t i = 0; i < n; ++i)
-                if (s.take(i) && isfinite(sampson_error(sh.f, t.x1[i], t.x2[i]))) ++nf;
+            // median of the finite Sampson errors over the inliers, rank nf / 2;
+            // the errors are computed once into LDS (or L2-resident scratch)
+            double* err = n <= (uint32_t)(NL * kStageW) ? &s_cp.stage[0][0]
+                                                        : reinterpret_cast<double*>(lo_masks);
+            double f[9];
+            load9(sh.f, f);
+            uint32_t nfu = 0;
+            for (uint32_t base = 0; base < n; base += NL) {
+                const uint32_t i = base + lane;
+                bool fin = false;
+                if (i < n && s.take(i)) {
+                    const double e = sampson_error(f, t.x1[i], t.x2[i]);
+                    err[i] = e;
+                    fin = isfinite(e);
+                }
+                nfu += (uint32_t)__popcll(__ballot(fin));
+            }
+            __syncthreads();
+            const int nf = (int)nfu;
             if (nf < 8) has = false;
             if (has) {
                 const int want = nf / 2;
@@ -983,12 +1325,12 @@ __global__ __launch_bounds__(NL) void ransac_kernel(const RansacTask* __restrict
                 int found_c = 0;
                 for (uint32_t i = lane; i < n; i += NL) {
                     if (!s.take(i)) continue;
-                    const double ei = sampson_error(sh.f, t.x1[i], t.x2[i]);
+                    const double ei = err[i];
                     if (!isfinite(ei)) continue;
                     int below = 0, tie_before = 0;
                     for (uint32_t j = 0; j < n; ++j) {
                         if (!s.take(j)) continue;
-                        const double ej = sampson_error(sh.f, t.x1[j], t.x2[j]);
+                        const double ej = err[j];
                         if (!isfinite(ej)) continue;
                         below += ej < ei;
                         tie_before += (ej == ei && j < i);

@@ -16,9 +16,24 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 namespace skh {
 namespace rdev {
+
+// ------------------------------------------------------ compile-time loops
+
+template <int Base, class F, int... I>
+__device__ __forceinline__ void sfor_impl(F& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, Base + I>{}), ...);
+}
+
+// f(std::integral_constant<int, i>) for i = B .. E-1, expanded at compile
+// time so that every array index in f is a constant (register-resident).
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (E > B) sfor_impl<B>(f, std::make_integer_sequence<int, E - B>{});
+}
 
 // ------------------------------------------------------------ point sources
 
@@ -48,6 +63,17 @@ struct Pts8 {
     __device__ bool take(uint32_t) const { return true; }
     __device__ double2 p(uint32_t i) const { return a[i]; }
     __device__ double2 q(uint32_t i) const { return b[i]; }
+};
+
+// Eight samples read through their indices (the indices live in LDS).
+struct PtsIdx8 {
+    const double2* x1;
+    const double2* x2;
+    const uint32_t* idx;
+    static constexpr uint32_t n = 8;
+    __device__ bool take(uint32_t) const { return true; }
+    __device__ double2 p(uint32_t i) const { return x1[idx[i]]; }
+    __device__ double2 q(uint32_t i) const { return x2[idx[i]]; }
 };
 
 // ------------------------------------------------------------------ 3x3 ops
@@ -136,14 +162,17 @@ __host__ __device__ constexpr int ut(int r, int c) {
     return r <= c ? r * 9 - r * (r - 1) / 2 + (c - r) : c * 9 - c * (c - 1) / 2 + (r - c);
 }
 
-__device__ __forceinline__ void jacobi_params(double app, double aqq, double apq, int sweep,
+// Rotation (c, s, t) annihilating a[p][q]; false when none is due (a zero
+// element, or after the fourth sweep one negligible against both diagonal
+// entries, which the sweep then just sets to zero).
+__device__ __forceinline__ bool jacobi_params(double app, double aqq, double apq, int sweep,
                                               double& c, double& s, double& t) {
     c = 1.0;
     s = 0.0;
     t = 0.0;
-    if (apq == 0.0) return;
+    if (apq == 0.0) return false;
     double g = 100.0 * fabs(apq);
-    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return;
+    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return false;
     double h = aqq - app;
     double tt;
     if (fabs(h) + g == fabs(h)) {
@@ -157,13 +186,46 @@ __device__ __forceinline__ void jacobi_params(double app, double aqq, double apq
     c = cc;
     s = tt * cc;
     t = tt;
+    return true;
 }
 
-// a: upper triangle (45); v: 9x9 row-major. Same rotation order and
-// arithmetic as the host's cyclic Jacobi.
-__device__ __forceinline__ void sym_eigen9(double* a, double* v) {
+// Rotates the upper triangle `a` (45) by (c, s, t) in the (p, q) plane.
+template <int p, int q>
+__device__ __forceinline__ void jacobi_rotate_a(double* a, double c, double s, double t) {
+    const double apq = a[ut(p, q)];
 #pragma unroll
-    for (int i = 0; i < 81; ++i) v[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 9; ++k) {
+        if (k == p || k == q) continue;
+        double akp = a[ut(k, p)], akq = a[ut(k, q)];
+        a[ut(k, p)] = c * akp - s * akq;
+        a[ut(k, q)] = s * akp + c * akq;
+    }
+    a[ut(p, p)] = a[ut(p, p)] - t * apq;
+    a[ut(q, q)] = a[ut(q, q)] + t * apq;
+}
+
+template <int R>
+__device__ __forceinline__ void jacobi_rotate_v(double (*v)[9], int p, int q, double c, double s) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double vkp = v[k][p], vkq = v[k][q];
+        v[k][p] = c * vkp - s * vkq;
+        v[k][q] = s * vkp + c * vkq;
+    }
+}
+
+// Cyclic Jacobi on the upper triangle `a` (45), accumulating R rows of the
+// eigenvector matrix starting at row r0 (v[i] = row r0 + i; rows past 8 are
+// padding). Same rotation order and arithmetic as the host's cyclic Jacobi;
+// rows of the eigenvector matrix evolve independently, so a pair of lanes
+// can split them (R = 5) while both rotate `a`, which keeps a and v in
+// registers.
+template <int R>
+__device__ __forceinline__ void sym_eigen9_rows(double* a, int r0, double (*v)[9]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v[i][j] = (r0 + i == j) ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 50; ++sweep) {
         double off = 0.0;
 #pragma unroll
@@ -171,32 +233,28 @@ __device__ __forceinline__ void sym_eigen9(double* a, double* v) {
 #pragma unroll
             for (int q = p + 1; q < 9; ++q) off = off + fabs(a[ut(p, q)]);
         if (off == 0.0) break;
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-#pragma unroll
-            for (int q = p + 1; q < 9; ++q) {
-                double c, s, t;
-                const double apq = a[ut(p, q)];
-                jacobi_params(a[ut(p, p)], a[ut(q, q)], apq, sweep, c, s, t);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == p || k == q) continue;
-                    double akp = a[ut(k, p)], akq = a[ut(k, q)];
-                    a[ut(k, p)] = c * akp - s * akq;
-                    a[ut(k, q)] = s * akp + c * akq;
-                }
-                a[ut(p, p)] = a[ut(p, p)] - t * apq;
-                a[ut(q, q)] = a[ut(q, q)] + t * apq;
-                a[ut(p, q)] = 0.0;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    double vkp = v[9 * k + p], vkq = v[9 * k + q];
-                    v[9 * k + p] = c * vkp - s * vkq;
-                    v[9 * k + q] = s * vkp + c * vkq;
-                }
-            }
-        }
+#define SKH_ROT(P, Q)                                                                 \
+    {                                                                                 \
+        double c, s, t;                                                               \
+        if (jacobi_params(a[ut(P, P)], a[ut(Q, Q)], a[ut(P, Q)], sweep, c, s, t)) {   \
+            jacobi_rotate_a<P, Q>(a, c, s, t);                                        \
+            jacobi_rotate_v<R>(v, P, Q, c, s);                                        \
+        }                                                                             \
+        a[ut(P, Q)] = 0.0;                                                            \
     }
+        SKH_ROT(0, 1) SKH_ROT(0, 2) SKH_ROT(0, 3) SKH_ROT(0, 4) SKH_ROT(0, 5) SKH_ROT(0, 6)
+        SKH_ROT(0, 7) SKH_ROT(0, 8) SKH_ROT(1, 2) SKH_ROT(1, 3) SKH_ROT(1, 4) SKH_ROT(1, 5)
+        SKH_ROT(1, 6) SKH_ROT(1, 7) SKH_ROT(1, 8) SKH_ROT(2, 3) SKH_ROT(2, 4) SKH_ROT(2, 5)
+        SKH_ROT(2, 6) SKH_ROT(2, 7) SKH_ROT(2, 8) SKH_ROT(3, 4) SKH_ROT(3, 5) SKH_ROT(3, 6)
+        SKH_ROT(3, 7) SKH_ROT(3, 8) SKH_ROT(4, 5) SKH_ROT(4, 6) SKH_ROT(4, 7) SKH_ROT(4, 8)
+        SKH_ROT(5, 6) SKH_ROT(5, 7) SKH_ROT(5, 8) SKH_ROT(6, 7) SKH_ROT(6, 8) SKH_ROT(7, 8)
+#undef SKH_ROT
+    }
+}
+
+// One lane, all nine rows (v: 9x9 row-major).
+__device__ __forceinline__ void sym_eigen9(double* a, double* v) {
+    sym_eigen9_rows<9>(a, 0, reinterpret_cast<double (*)[9]>(v));
 }
 
 // --------------------------------------------------------------- 3x3 rank 2
@@ -342,10 +400,45 @@ __device__ bool all_finite(const S& s) {
     return true;
 }
 
+// From the centroid sums (c: x1.x, x1.y, x2.x, x2.y) and the second-pass
+// sums (q: s1xx, s1xy, s1yy, s2xx, s2xy, s2yy, d1, d2) of len points: the
+// Hartley normalisers t1, t2 (matching.rs:498-505) and whether the points of
+// either image are nearly collinear (matching.rs:224-247).
+__device__ __forceinline__ bool collinear_finish(double c1x, double c1y, double c2x, double c2y,
+                                                 const double* q, uint32_t len, double* t1,
+                                                 double* t2) {
+    const double nn = (double)len, nc = (double)(len > 0 ? len : 1);
+    bool collinear = false;
+#pragma unroll
+    for (int im = 0; im < 2; ++im) {
+        double sxx = q[3 * im] / nc, sxy = q[3 * im + 1] / nc, syy = q[3 * im + 2] / nc;
+        double tr = sxx + syy;
+        double dd = sxx - syy;
+        double disc = sqrt(dd * dd + 4.0 * sxy * sxy);
+        double hi = 0.5 * (tr + disc);
+        double lo = 0.5 * (tr - disc);
+        if (!(lo >= 0.0)) lo = 0.0;
+        double slo = sqrt(lo), shi = sqrt(hi);
+        if (!(shi > 0.0 && slo > 0.02 * shi)) collinear = true;
+    }
+    const double m1 = q[6] / nn, m2 = q[7] / nn;
+    const double sc1 = m1 > 0.0 ? 1.4142135623730951 / m1 : 1.0;
+    const double sc2 = m2 > 0.0 ? 1.4142135623730951 / m2 : 1.0;
+    t1[0] = sc1; t1[1] = 0.0; t1[2] = -sc1 * c1x;
+    t1[3] = 0.0; t1[4] = sc1; t1[5] = -sc1 * c1y;
+    t1[6] = 0.0; t1[7] = 0.0; t1[8] = 1.0;
+    t2[0] = sc2; t2[1] = 0.0; t2[2] = -sc2 * c2x;
+    t2[3] = 0.0; t2[4] = sc2; t2[5] = -sc2 * c2y;
+    t2[6] = 0.0; t2[7] = 0.0; t2[8] = 1.0;
+    return collinear;
+}
+
 // Collinearity tests and Hartley normalisers of both images in two passes.
 // The host computes nearly_collinear(x1), nearly_collinear(x2),
 // normalizer(x1), normalizer(x2) one after another; each of their sums sees
-// the same operands in the same order here, so every value is identical.
+// the same operands in the same order here, so every value is identical
+// (axis_spread divides the centroid by max(len, 1), the normaliser by len:
+// equal for len >= 1).
 template <class S>
 __device__ bool collinear_and_normalizers(const S& s, uint32_t len, double* t1, double* t2) {
     double c1x = 0.0, c1y = 0.0, c2x = 0.0, c2y = 0.0;
@@ -357,59 +450,34 @@ __device__ bool collinear_and_normalizers(const S& s, uint32_t len, double* t1, 
         c2x = c2x + b.x;
         c2y = c2y + b.y;
     }
-    const double nn = (double)len, nc = (double)(len > 0 ? len : 1);
-    // axis_spread divides by max(len, 1), the normaliser by len: equal for len >= 1
+    const double nc = (double)(len > 0 ? len : 1);
     c1x = c1x / nc;
     c1y = c1y / nc;
     c2x = c2x / nc;
     c2y = c2y / nc;
-    double s1xx = 0.0, s1xy = 0.0, s1yy = 0.0, s2xx = 0.0, s2xy = 0.0, s2yy = 0.0;
-    double d1 = 0.0, d2 = 0.0;
+    double q[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t i = 0; i < s.n; ++i) {
         if (!s.take(i)) continue;
         const double2 a = s.p(i), b = s.q(i);
         const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
-        s1xx += ax * ax;
-        s1xy += ax * ay;
-        s1yy += ay * ay;
-        s2xx += bx * bx;
-        s2xy += bx * by;
-        s2yy += by * by;
-        d1 = d1 + norm2(ax, ay);
-        d2 = d2 + norm2(bx, by);
+        q[0] += ax * ax;
+        q[1] += ax * ay;
+        q[2] += ay * ay;
+        q[3] += bx * bx;
+        q[4] += bx * by;
+        q[5] += by * by;
+        q[6] = q[6] + norm2(ax, ay);
+        q[7] = q[7] + norm2(bx, by);
     }
-    double sp[2][3] = {{s1xx, s1xy, s1yy}, {s2xx, s2xy, s2yy}};
-#pragma unroll
-    for (int im = 0; im < 2; ++im) {
-        double sxx = sp[im][0] / nc, sxy = sp[im][1] / nc, syy = sp[im][2] / nc;
-        double tr = sxx + syy;
-        double dd = sxx - syy;
-        double disc = sqrt(dd * dd + 4.0 * sxy * sxy);
-        double hi = 0.5 * (tr + disc);
-        double lo = 0.5 * (tr - disc);
-        if (!(lo >= 0.0)) lo = 0.0;
-        double slo = sqrt(lo), shi = sqrt(hi);
-        if (!(shi > 0.0 && slo > 0.02 * shi)) return true;
-    }
-    const double m1 = d1 / nn, m2 = d2 / nn;
-    const double sc1 = m1 > 0.0 ? 1.4142135623730951 / m1 : 1.0;
-    const double sc2 = m2 > 0.0 ? 1.4142135623730951 / m2 : 1.0;
-    t1[0] = sc1; t1[1] = 0.0; t1[2] = -sc1 * c1x;
-    t1[3] = 0.0; t1[4] = sc1; t1[5] = -sc1 * c1y;
-    t1[6] = 0.0; t1[7] = 0.0; t1[8] = 1.0;
-    t2[0] = sc2; t2[1] = 0.0; t2[2] = -sc2 * c2x;
-    t2[3] = 0.0; t2[4] = sc2; t2[5] = -sc2 * c2y;
-    t2[6] = 0.0; t2[7] = 0.0; t2[8] = 1.0;
-    return false;
+    return collinear_finish(c1x, c1y, c2x, c2y, q, len, t1, t2);
 }
 
-// matching.rs:512-557 over the selected points (len of them).
+// matching.rs:512-538 over the selected points (len of them): the finite
+// and collinearity checks, the Hartley transforms and A^T A (upper triangle).
 template <class S>
-__device__ bool fundamental_8pt(const S& s, uint32_t len, double* F) {
+__device__ bool accumulate_8pt(const S& s, uint32_t len, double* t1, double* t2, double* ata) {
     if (len < 8 || !all_finite(s)) return false;
-    double t1[9], t2[9];
     if (collinear_and_normalizers(s, len, t1, t2)) return false;
-    double ata[45];
 #pragma unroll
     for (int i = 0; i < 45; ++i) ata[i] = 0.0;
     for (uint32_t i = 0; i < s.n; ++i) {
@@ -426,8 +494,17 @@ __device__ bool fundamental_8pt(const S& s, uint32_t len, double* F) {
 #pragma unroll
             for (int c = r; c < 9; ++c) ata[ut(r, c)] = ata[ut(r, c)] + row[r] * row[c];
     }
-    double v[81];
-    sym_eigen9(ata, v);
+    return true;
+}
+
+// matching.rs:539-556 computed by a pair of lanes (threadIdx.x ^ 1 is the
+// partner; both must be active with the same ata, t1, t2): the eigen-solve
+// (each lane accumulates five eigenvector rows), the degeneracy test, the
+// rank-2 projection and the denormalisation. ata is overwritten.
+__device__ bool solve_8pt_pair(double* ata, const double* t1, const double* t2, double* F) {
+    const int half = threadIdx.x & 1;
+    double v[5][9];
+    sym_eigen9_rows<5>(ata, 5 * half, v);
     double ev[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) ev[k] = ata[ut(k, k)];
@@ -449,14 +526,21 @@ __device__ bool fundamental_8pt(const S& s, uint32_t len, double* F) {
         if (x > e8) e8 = x;
     }
     if (e1 <= 1e-10 * e8) return false;
-    double fn[9];
+    double mine[5], other[5];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < 5; ++i) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[9 * i + j] : acc;
-        fn[i] = acc;
+        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[i][j] : acc;
+        mine[i] = acc;
     }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) other[i] = __shfl_xor(mine[i], 1);
+    double fn[9];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) fn[i] = half ? other[i] : mine[i];
+#pragma unroll
+    for (int i = 5; i < 9; ++i) fn[i] = half ? mine[i - 5] : other[i - 5];
     double r2[9];
     rank2_recompose(fn, r2);
     double t2t[9], tmp[9], f[9];
