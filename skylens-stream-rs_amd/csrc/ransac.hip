@@ -929,12 +929,13 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
 
 __global__ __launch_bounds__(NL) __attribute__((amdgpu_waves_per_eu(SKH_RANSAC_WAVES_PER_EU)))
 void ransac_kernel(const RansacTask* __restrict__ tasks,
+                                                    const uint32_t* __restrict__ order,
                                                     const RansacCfgDev cfg) {
     __shared__ Shared sh;
     PhaseMem& pm = s_pm;
     __shared__ double2 lds_x1[kLdsPts > 0 ? kLdsPts : 1], lds_x2[kLdsPts > 0 ? kLdsPts : 1];
     __shared__ uint32_t lds_masks[kLdsWords > 0 ? kMaskSets * kLdsWords : 1];
-    RansacTask t = tasks[blockIdx.x];
+    RansacTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
     const int lane = threadIdx.x;
     const uint32_t n = t.n;
@@ -1400,10 +1401,39 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
 
 }  // namespace
 
+// Pair p's rank when the pairs are ordered by descending correspondence
+// count (ties by index): order[rank] = p. Larger problems start first, so
+// the longest workgroups do not trail at the end of the launch.
+__global__ __launch_bounds__(256) void ransac_order_kernel(const uint32_t* __restrict__ count,
+                                                           uint32_t P, uint32_t* __restrict__ order) {
+    __shared__ uint32_t tile[1024];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t ki = i < P ? count[i] : 0u;
+    uint32_t rank = 0;
+    for (uint32_t base = 0; base < P; base += 1024) {
+        for (uint32_t j = threadIdx.x; j < 1024; j += 256)
+            tile[j] = base + j < P ? count[base + j] : 0u;
+        __syncthreads();
+        const uint32_t e = P - base < 1024 ? P - base : 1024;
+        for (uint32_t j = 0; j < e; ++j) {
+            const uint32_t kj = tile[j];
+            rank += (kj > ki || (kj == ki && base + j < i)) ? 1u : 0u;
+        }
+        __syncthreads();
+    }
+    if (i < P) order[rank] = i;
+}
+
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
-                   hipStream_t s) {
+                   hipStream_t s, const uint32_t* count, uint32_t* order) {
     if (!n_tasks) return;
-    hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg);
+    if (count && order)
+        hipLaunchKernelGGL(ransac_order_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, s,
+                           count, n_tasks, order);
+    else
+        order = nullptr;
+    hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks,
+                       (const uint32_t*)order, cfg);
 }
 
 uint32_t ransac_scratch_words(uint32_t n) {
