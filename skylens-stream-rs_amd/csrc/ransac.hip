@@ -144,223 +144,6 @@ __device__ uint64_t parallax_needed(uint64_t n) {
     return n + 1;
 }
 
-// two_view.rs:605-642 on Hartley-normalised points; matching.rs:251-266.
-template <class S>
-__device__ bool homography_dlt(const S& s, uint32_t len, double* H) {
-    if (len < 4) return false;
-    double t1[9], t2[9];
-    normalizer(s, false, t1, len);
-    normalizer(s, true, t2, len);
-    double ata[45];
-#pragma unroll
-    for (int i = 0; i < 45; ++i) ata[i] = 0.0;
-    for (uint32_t i = 0; i < s.n; ++i) {
-        if (!s.take(i)) continue;
-        double2 pp = s.p(i), qq = s.q(i);
-        double p3[3] = {pp.x, pp.y, 1.0}, v[3];
-        mv3(t1, p3, v);
-        const double x = v[0] / v[2], y = v[1] / v[2];
-        double q3[3] = {qq.x, qq.y, 1.0};
-        mv3(t2, q3, v);
-        const double u = v[0] / v[2], w = v[1] / v[2];
-        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, u * x, u * y, u};
-        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, w * x, w * y, w};
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-#pragma unroll
-            for (int c = r; c < 9; ++c)
-                ata[ut(r, c)] = ata[ut(r, c)] + (r1[r] * r1[c] + r2[r] * r2[c]);
-    }
-    double vv[81];
-    sym_eigen9(ata, vv);
-    int k = 0;
-#pragma unroll
-    for (int i = 1; i < 9; ++i)
-        if (ata[ut(i, i)] < ata[ut(k, k)]) k = i;
-    double hn[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) acc = (j == k) ? vv[9 * i + j] : acc;
-        hn[i] = acc;
-        if (!isfinite(acc)) return false;
-    }
-    double t2i[9], tmp[9], h[9];
-    if (!inv3(t2, t2i)) return false;
-    mm3(t2i, hn, tmp);
-    mm3(tmp, t1, h);
-    double nn = fro3(h);
-    if (!(isfinite(nn) && nn > 0.0)) return false;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
-    return true;
-}
-
-__device__ __forceinline__ bool htr(const double* m, double2 x, double2& out) {
-    double p[3] = {x.x, x.y, 1.0}, v[3];
-    mv3(m, p, v);
-    if (fabs(v[2]) < 1e-12) return false;
-    out.x = v[0] / v[2];
-    out.y = v[1] / v[2];
-    return true;
-}
-
-__device__ __forceinline__ double homography_error(const double* h, const double* hi, double2 p,
-                                                   double2 q) {
-    double2 a, b;
-    if (!htr(h, p, a) || !htr(hi, q, b)) return INFINITY;
-    return fmax(norm2(a.x - q.x, a.y - q.y), norm2(b.x - p.x, b.y - p.y));
-}
-
-template <class S>
-__device__ uint32_t count_h(const double* h, const double* hi, const S& s, double th) {
-    uint32_t c = 0;
-    for (uint32_t i = 0; i < s.n; ++i) {
-        if (!s.take(i)) continue;
-        c += homography_error(h, hi, s.p(i), s.q(i)) < th ? 1u : 0u;
-    }
-    return c;
-}
-
-struct SelState {
-    double F[NL][9];
-    int cnt[NL];
-    uint32_t idx[NL][8];
-    double best[9], besti[9];
-    int have, bcnt;
-    uint64_t iters, it;
-};
-
-// matching.rs:300-369 over the points of `s` (ns of them), all lanes.
-template <class S>
-__device__ __noinline__ bool fit_homography(const S& s, uint32_t ns, double th, double* H, double* Hi,
-                               uint32_t* cnt_out, SelState& sh, uint32_t* selbits,
-                               const double2* x1, const double2* x2) {
-    const int lane = threadIdx.x;
-    if (ns < 4) return false;
-    // matching.rs:314-320: draws index the selected points in order, so
-    // lanes translate draw ranks to point indices.
-    __shared__ uint32_t rank2idx_dummy;
-    (void)rank2idx_dummy;
-    if (lane == 0) {
-        sh.have = 0;
-        sh.bcnt = 0;
-        sh.iters = 500;
-        sh.it = 0;
-    }
-    __syncthreads();
-    uint64_t st = 0x2545F4914F6CDD1Dull;
-    while (sh.it < sh.iters) {
-        if (lane == 0) {
-            for (int h = 0; h < NL; ++h) {
-                int k = 0;
-                while (k < 4) {
-                    uint32_t c = lcg_draw(st, ns);
-                    bool dup = false;
-                    for (int e = 0; e < k; ++e) dup |= sh.idx[h][e] == c;
-                    if (!dup) sh.idx[h][k++] = c;
-                }
-            }
-        }
-        __syncthreads();
-        {
-            // the draws are ranks among the selected points
-            Pts8 p4;
-            for (int e = 0; e < 4; ++e) {
-                uint32_t r = sh.idx[lane][e], i = 0, seen = 0;
-                for (;; ++i) {
-                    if (s.take(i)) {
-                        if (seen == r) break;
-                        ++seen;
-                    }
-                }
-                p4.a[e] = s.p(i);
-                p4.b[e] = s.q(i);
-            }
-            Pts4 v4{&p4};
-            double h[9], hi[9];
-            int c = -1;
-            if (homography_dlt(v4, 4, h) && inv3(h, hi)) {
-                c = (int)count_h(h, hi, s, th);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) sh.F[lane][i] = h[i];
-            }
-            sh.cnt[lane] = c;
-        }
-        __syncthreads();
-        if (lane == 0) {
-            for (int h = 0; h < NL && sh.it < sh.iters; ++h) {
-                sh.it++;
-                const int c = sh.cnt[h];
-                if (c < 0) continue;
-                if (!sh.have || c > sh.bcnt) {
-                    sh.have = 1;
-                    sh.bcnt = c;
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) sh.best[i] = sh.F[h][i];
-                    sh.iters = ipow_adaptive((double)c / (double)ns, 4, 0.999, 500);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (!sh.have) return false;
-    bool ok = true;
-    if (lane == 0) {
-        double bh[9], bhi[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) bh[i] = sh.best[i];
-        inv3(bh, bhi);
-        uint32_t bc = (uint32_t)sh.bcnt;
-        double ch[9], chi[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            ch[i] = bh[i];
-            chi[i] = bhi[i];
-        }
-        const double HM[4] = {4.0, 2.0, 1.0, 1.0};
-        const uint32_t W = words(s.n);
-        for (int r = 0; r < 4; ++r) {
-            uint32_t nsel = 0;
-            for (uint32_t w0 = 0; w0 < W; ++w0) selbits[w0] = 0u;
-            for (uint32_t i = 0; i < s.n; ++i) {
-                if (!s.take(i)) continue;
-                if (homography_error(ch, chi, s.p(i), s.q(i)) < th * HM[r]) {
-                    selbits[i >> 5] |= 1u << (i & 31);
-                    ++nsel;
-                }
-            }
-            PtsMask sel{x1, x2, s.n, selbits};
-            double g[9], gi[9];
-            if (!homography_dlt(sel, nsel, g)) break;
-            if (!inv3(g, gi)) break;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                ch[i] = g[i];
-                chi[i] = gi[i];
-            }
-            uint32_t c = count_h(g, gi, s, th);
-            if (c > bc) {
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    bh[i] = g[i];
-                    bhi[i] = gi[i];
-                }
-                bc = c;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            H[i] = bh[i];
-            Hi[i] = bhi[i];
-        }
-        *cnt_out = bc;
-    }
-    __syncthreads();
-    return ok;
-}
-
 // ------------------------------------------------------ wave-cooperative sums
 //
 // The workgroup is one 64-lane wave. A sum over a selected point set keeps
@@ -736,6 +519,311 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
     }
 }
 
+// ------------------------------------------------------------ homography
+//
+// two_view.rs:605-642 on Hartley-normalised points; matching.rs:251-369.
+
+__device__ __forceinline__ bool htr(const double* m, double2 x, double2& out) {
+    double p[3] = {x.x, x.y, 1.0}, v[3];
+    mv3(m, p, v);
+    if (fabs(v[2]) < 1e-12) return false;
+    out.x = v[0] / v[2];
+    out.y = v[1] / v[2];
+    return true;
+}
+
+__device__ __forceinline__ double homography_error(const double* h, const double* hi, double2 p,
+                                                   double2 q) {
+    double2 a, b;
+    if (!htr(h, p, a) || !htr(hi, q, b)) return INFINITY;
+    return fmax(norm2(a.x - q.x, a.y - q.y), norm2(b.x - p.x, b.y - p.y));
+}
+
+// Four samples given as ranks among the inliers (`list`: rank -> index).
+struct PtsRank4 {
+    const double2* x1;
+    const double2* x2;
+    const uint32_t* list;
+    const uint32_t* rank;
+    static constexpr uint32_t n = 4;
+    __device__ bool take(uint32_t) const { return true; }
+    __device__ double2 p(uint32_t i) const { return x1[list[rank[i]]]; }
+    __device__ double2 q(uint32_t i) const { return x2[list[rank[i]]]; }
+};
+
+// The DLT normal matrix of len points of `s` (one lane).
+template <class S>
+__device__ void homography_accumulate(const S& s, uint32_t len, double* t1, double* t2,
+                                      double* ata) {
+    normalizer(s, false, t1, len);
+    normalizer(s, true, t2, len);
+#pragma unroll
+    for (int i = 0; i < 45; ++i) ata[i] = 0.0;
+    for (uint32_t i = 0; i < s.n; ++i) {
+        if (!s.take(i)) continue;
+        double2 pp = s.p(i), qq = s.q(i);
+        double p3[3] = {pp.x, pp.y, 1.0}, v[3];
+        mv3(t1, p3, v);
+        const double x = v[0] / v[2], y = v[1] / v[2];
+        double q3[3] = {qq.x, qq.y, 1.0};
+        mv3(t2, q3, v);
+        const double u = v[0] / v[2], w = v[1] / v[2];
+        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, u * x, u * y, u};
+        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, w * x, w * y, w};
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int c = r; c < 9; ++c)
+                ata[ut(r, c)] = ata[ut(r, c)] + (r1[r] * r1[c] + r2[r] * r2[c]);
+    }
+}
+
+// Eigen-solve and denormalisation of a DLT normal matrix by a solver group.
+__device__ bool homography_finish_group(double* ata, const double* t1, const double* t2,
+                                        double* H) {
+    double hn[9];
+    eigen_min_group(ata, hn);
+    bool fin = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fin = fin && isfinite(hn[i]);
+    if (!fin) return false;
+    double t2i[9], tmp[9], h[9];
+    if (!inv3(t2, t2i)) return false;
+    mm3(t2i, hn, tmp);
+    mm3(tmp, t1, h);
+    double nn = fro3(h);
+    if (!(isfinite(nn) && nn > 0.0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
+    return true;
+}
+
+// r1 / r2 entries of a DLT point row as indices into its staged vector
+// [-x, -y, -1, 0, u*x, u*y, u, w*x, w*y, w].
+__constant__ int kDltR1[9] = {0, 1, 2, 3, 3, 3, 4, 5, 6};
+__constant__ int kDltR2[9] = {3, 3, 3, 0, 1, 2, 7, 8, 9};
+
+// homography_dlt over the bitset `sel` (len points), all lanes; H is the
+// same in every lane.
+__device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32_t n,
+                                    const uint32_t* sel, uint32_t len, double* H) {
+    const int lane = threadIdx.x;
+    if (len < 4) return false;
+    double t1[9], t2[9];
+    coop_norm(x1, x2, n, sel, len, t1, t2);
+    int r, c;
+    ut_rc(lane < 45 ? lane : 0, r, c);
+    const int a1 = kDltR1[r], b1 = kDltR1[c], a2 = kDltR2[r], b2 = kDltR2[c];
+    double acc = 0.0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        if (!m) continue;
+        if ((m >> lane) & 1u) {
+            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+            const double2 pp = x1[base + lane], qq = x2[base + lane];
+            double p3[3] = {pp.x, pp.y, 1.0}, v[3];
+            mv3(t1, p3, v);
+            const double x = v[0] / v[2], y = v[1] / v[2];
+            double q3[3] = {qq.x, qq.y, 1.0};
+            mv3(t2, q3, v);
+            const double u = v[0] / v[2], w = v[1] / v[2];
+            double* row = s_cp.stage[slot];
+            row[0] = -x;
+            row[1] = -y;
+            row[2] = -1.0;
+            row[3] = 0.0;
+            row[4] = u * x;
+            row[5] = u * y;
+            row[6] = u;
+            row[7] = w * x;
+            row[8] = w * y;
+            row[9] = w;
+        }
+        __syncthreads();
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const double* row = s_cp.stage[j];
+            acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
+        }
+        __syncthreads();
+    }
+    double ata[45];
+#pragma unroll
+    for (int k = 0; k < 45; ++k) ata[k] = readlane_d(acc, k);
+    // every solver group solves the same system; lane 0's result is used
+    double h[9];
+    const int ok = homography_finish_group(ata, t1, t2, h) ? 1 : 0;
+    if (!__builtin_amdgcn_readlane(ok, 0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = readlane_d(h[i], 0);
+    return true;
+}
+
+// |{i in sel : homography_error(h, hi, i) < th}|; with `out`, also the
+// bitset of those points.
+__device__ uint32_t coop_count_h(const double* h, const double* hi, const double2* x1,
+                                 const double2* x2, uint32_t n, const uint32_t* sel, double th,
+                                 uint32_t* out) {
+    const int lane = threadIdx.x;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(sel, base, n);
+        const bool in = ((m >> lane) & 1u) &&
+                        homography_error(h, hi, x1[base + lane], x2[base + lane]) < th;
+        const uint64_t b = __ballot(in);
+        if (out && lane < 2 && base + 32u * lane < n)
+            out[(base >> 5) + lane] = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
+        cnt += (uint32_t)__popcll(b);
+    }
+    if (out) __syncthreads();
+    return cnt;
+}
+
+// rank -> index of the points of an n-bit bitset, in index order.
+__device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
+    const int lane = threadIdx.x;
+    uint32_t done = 0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(bits, base, n);
+        if ((m >> lane) & 1u) list[done + (uint32_t)__popcll(m & lanes_below())] = base + lane;
+        done += (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+}
+
+constexpr int kHypPerBatch = 3 * kGroups;  // hypotheses per batch: one per solver group, 3 rounds
+
+struct SelState {
+    double F[kHypPerBatch][9];
+    int cnt[kHypPerBatch];
+    uint32_t idx[kHypPerBatch][4];
+    double best[9];
+    int have, bcnt;
+    uint64_t iters, it;
+};
+
+__constant__ double kHM[4] = {4.0, 2.0, 1.0, 1.0};
+
+// matching.rs:300-369 over the inlier set `sel` (ns points, `list` maps
+// their ranks to indices), all lanes. H, Hi, *cnt_out are the same in every
+// lane; selbits is scratch (words(n) words).
+__device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2, uint32_t n,
+                                            const uint32_t* sel, const uint32_t* list,
+                                            uint32_t ns, double th, double* H, double* Hi,
+                                            uint32_t* cnt_out, SelState& sh, uint32_t* selbits) {
+    const int lane = threadIdx.x;
+    if (ns < 4) return false;
+    if (lane == 0) {
+        sh.have = 0;
+        sh.bcnt = 0;
+        sh.iters = 500;
+        sh.it = 0;
+    }
+    __syncthreads();
+    uint64_t st = 0x2545F4914F6CDD1Dull;
+    const int g = solver_group(), part = solver_part(), gbase = 3 * g;
+    while (sh.it < sh.iters) {
+        if (lane == 0) {
+            // matching.rs:314-320: four distinct ranks among the inliers
+            for (int h = 0; h < kHypPerBatch; ++h) {
+                uint32_t s[4] = {0, 0, 0, 0};
+                int k = 0;
+                while (k < 4) {
+                    const uint32_t c = lcg_draw(st, ns);
+                    bool dup = false;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) dup |= e < k && s[e] == c;
+                    if (!dup) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) s[e] = e == k ? c : s[e];
+                        ++k;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) sh.idx[h][e] = s[e];
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int round = 0; round < 3; ++round) {
+            const int h = kGroups * round + g;
+            if (g < kGroups) {
+                PtsRank4 s4{x1, x2, list, sh.idx[h]};
+                double t1[9], t2[9], ata[45], hh[9], hi[9];
+                homography_accumulate(s4, 4, t1, t2, ata);
+                int c = -1;
+                if (homography_finish_group(ata, t1, t2, hh) && inv3(hh, hi)) {
+                    uint32_t cc = 0;
+                    for (uint32_t r = part; r < ns; r += 3) {
+                        const uint32_t i = list[r];
+                        cc += homography_error(hh, hi, x1[i], x2[i]) < th ? 1u : 0u;
+                    }
+                    c = (int)(__shfl(cc, gbase) + __shfl(cc, gbase + 1) + __shfl(cc, gbase + 2));
+                    if (part == 0) {
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) sh.F[h][i] = hh[i];
+                    }
+                }
+                if (part == 0) sh.cnt[h] = c;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
+                sh.it++;
+                const int c = sh.cnt[h];
+                if (c < 0) continue;
+                if (!sh.have || c > sh.bcnt) {
+                    sh.have = 1;
+                    sh.bcnt = c;
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) sh.best[i] = sh.F[h][i];
+                    sh.iters = ipow_adaptive((double)c / (double)ns, 4, 0.999, 500);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!sh.have) return false;
+    double bh[9], bhi[9], ch[9], chi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) bh[i] = sh.best[i];
+    inv3(bh, bhi);
+    uint32_t bc = (uint32_t)sh.bcnt;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        ch[i] = bh[i];
+        chi[i] = bhi[i];
+    }
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t nsel = coop_count_h(ch, chi, x1, x2, n, sel, th * kHM[r], selbits);
+        double gg[9], gi[9];
+        if (!coop_homography_dlt(x1, x2, n, selbits, nsel, gg)) break;
+        if (!inv3(gg, gi)) break;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            ch[i] = gg[i];
+            chi[i] = gi[i];
+        }
+        const uint32_t c = coop_count_h(gg, gi, x1, x2, n, sel, th, nullptr);
+        if (c > bc) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                bh[i] = gg[i];
+                bhi[i] = gi[i];
+            }
+            bc = c;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        H[i] = bh[i];
+        Hi[i] = bhi[i];
+    }
+    *cnt_out = bc;
+    return true;
+}
+
 // Hypothesis-batch state; its LDS is reused by the homography fit and the
 // line test after the hypothesis loop.
 struct HypMem {
@@ -788,33 +876,34 @@ __device__ __forceinline__ void load9(const double* src, double* dst) {
     for (int i = 0; i < 9; ++i) dst[i] = src[i];
 }
 
-// Lanes 2c and 2c+1 solve hypothesis 32 * round + c together (both
-// accumulate its 8-point system, each carries five eigenvector rows) and
-// split its Sampson count; results: hCnt[h] (-1 when fundamental_8pt gives
+// Solver group g solves hypothesis kGroups * round + g (all three lanes
+// accumulate its 8-point system, each carries three eigenvector rows) and
+// splits its Sampson count; results: hCnt[h] (-1 when fundamental_8pt gives
 // None) and the hypothesis F in loF[h].
-__device__ __noinline__ void hypotheses_pairs(const RansacTask& t, HypMem& hm, double th2) {
+__device__ __noinline__ void hypotheses_groups(const RansacTask& t, HypMem& hm, double th2) {
     const double2* x1 = t.x1;
     const double2* x2 = t.x2;
     const uint32_t n = t.n;
-    const int lane = threadIdx.x;
+    const int g = solver_group(), part = solver_part(), gbase = 3 * g;
 #pragma unroll 1
-    for (int round = 0; round < 2; ++round) {
-        const int h = 32 * round + (lane >> 1);
-        PtsIdx8 s{x1, x2, hm.idx[h]};
-        double t1[9], t2[9], ata[45], F[9];
-        int cnt = -1;
-        if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_pair(ata, t1, t2, F)) {
-            uint32_t c = 0;
-            for (uint32_t i = lane & 1; i < n; i += 2)
-                c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
-            c += __shfl_xor(c, 1);
-            cnt = (int)c;
-            if ((lane & 1) == 0) {
+    for (int round = 0; round < 3; ++round) {
+        const int h = kGroups * round + g;
+        if (g < kGroups) {
+            PtsIdx8 s{x1, x2, hm.idx[h]};
+            double t1[9], t2[9], ata[45], F[9];
+            int cnt = -1;
+            if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_group(ata, t1, t2, F)) {
+                uint32_t c = 0;
+                for (uint32_t i = part; i < n; i += 3)
+                    c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
+                cnt = (int)(__shfl(c, gbase) + __shfl(c, gbase + 1) + __shfl(c, gbase + 2));
+                if (part == 0) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
+                    for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
+                }
             }
+            if (part == 0) hm.hCnt[h] = cnt;
         }
-        if ((lane & 1) == 0) hm.hCnt[h] = cnt;
     }
 }
 
@@ -850,7 +939,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
         const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
         uint64_t todo = active;
         while (todo) {
-            // up to 32 candidates, one pair of lanes each
+            // up to kGroups candidates, one solver group each
             uint64_t filled = 0;
             int rank = 0;
             double ata[45], tt1[9], tt2[9];
@@ -859,7 +948,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
 #pragma unroll
             for (int k = 0; k < 9; ++k) tt1[k] = tt2[k] = 0.0;
             bool mine = false;
-            while (todo && rank < NL / 2) {
+            while (todo && rank < kGroups) {
                 const int h = __ffsll((unsigned long long)todo) - 1;
                 todo &= todo - 1;
                 const uint32_t ns = hm.loNs[h];
@@ -871,7 +960,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                     continue;
                 }
                 const double a = coop_ata(x1, x2, n, sel, t1, t2);
-                const bool me = (lane >> 1) == rank;
+                const bool me = solver_group() == rank;
 #pragma unroll
                 for (int k = 0; k < 45; ++k) {
                     const double v = readlane_d(a, k);
@@ -888,17 +977,17 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
             }
             double g[9];
             int ok = 0;
-            if (mine) ok = solve_8pt_pair(ata, tt1, tt2, g) ? 1 : 0;
+            if (mine) ok = solve_8pt_group(ata, tt1, tt2, g) ? 1 : 0;
             rank = 0;
             for (uint64_t m = filled; m; m &= m - 1, ++rank) {
                 const int h = __ffsll((unsigned long long)m) - 1;
-                if (!__builtin_amdgcn_readlane(ok, 2 * rank)) {
+                if (!__builtin_amdgcn_readlane(ok, 3 * rank)) {
                     active &= ~(1ull << h);
                     continue;
                 }
                 double gh[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) gh[k] = readlane_d(g[k], 2 * rank);
+                for (int k = 0; k < 9; ++k) gh[k] = readlane_d(g[k], 3 * rank);
                 const bool sw = (swapped >> h) & 1u;
                 uint32_t* spare = (sw ? lo : alt) + (size_t)h * W;
                 uint32_t ns = 0;
@@ -1007,7 +1096,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         if (lane == 0) pc[8] += 1;
         if (lane == 0) {
             // matching.rs:971-979: eight distinct draws per hypothesis
-            for (int h = 0; h < NL; ++h) {
+            for (int h = 0; h < kHypPerBatch; ++h) {
                 uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 int k = 0;
                 while (k < 8) {
@@ -1027,7 +1116,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         }
         __syncthreads();
         lap(0);
-        hypotheses_pairs(t, pm.hyp, th2);
+        if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
+        hypotheses_groups(t, pm.hyp, th2);
         __syncthreads();
         lap(1);
         const int cnt = pm.hyp.hCnt[lane];
@@ -1051,7 +1141,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         lap(2);
         // replay in order (lane 0)
         if (lane == 0) {
-            for (int h = 0; h < NL && sh.it < sh.iters; ++h) {
+            for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
                 sh.it++;
                 const int c0 = pm.hyp.hCnt[h];
                 if (c0 < 0) continue;
@@ -1234,6 +1324,9 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
 
     // ---------------------------- off_line_count (matching.rs:878-906), both images
     const uint32_t ns = (uint32_t)sh.es_n;
+    // rank -> index of the final inliers (the samplers below draw ranks)
+    uint32_t* inl_list = alt_masks;
+    coop_list(fin_mask, n, inl_list);
     bool line_ok = true;
     for (int side = 0; side < 2; ++side) {
         if (ns < 3) {
@@ -1245,17 +1338,10 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             for (int k = 0; k < 600; ++k) pm.draws[k] = lcg_draw(ls, ns);
         }
         __syncthreads();
-        // rank -> point index among the inliers
         int best = 0;
         for (int it = lane; it < 300; it += NL) {
             const uint32_t ri = pm.draws[2 * it], rj = pm.draws[2 * it + 1];
-            uint32_t pi = 0, pj = 0, seen = 0;
-            for (uint32_t i = 0; i < n; ++i) {
-                if (!((fin_mask[i >> 5] >> (i & 31)) & 1u)) continue;
-                if (seen == ri) pi = i;
-                if (seen == rj) pj = i;
-                ++seen;
-            }
+            const uint32_t pi = inl_list[ri], pj = inl_list[rj];
             const double2 a = side ? t.x2[pi] : t.x1[pi];
             const double2 b = side ? t.x2[pj] : t.x1[pj];
             const double dx = b.x - a.x, dy = b.y - a.y;
@@ -1263,8 +1349,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             if (ri == rj || isnan(len) || len <= 0.0) continue;
             const double nx = -dy / len, ny = dx / len;
             int c = 0;
-            for (uint32_t i = 0; i < n; ++i) {
-                if (!((fin_mask[i >> 5] >> (i & 31)) & 1u)) continue;
+            for (uint32_t r = 0; r < ns; ++r) {
+                const uint32_t i = inl_list[r];
                 const double2 q = side ? t.x2[i] : t.x1[i];
                 const double ex = q.x - a.x, ey = q.y - a.y;
                 if (fabs(ex * nx + ey * ny) < cfg.threshold_px) ++c;
@@ -1359,23 +1445,36 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             const double s2 = sigma * sigma;
             double H[9], Hi[9];
             uint32_t hc = 0;
-            const bool hok =
-                fit_homography(s, ns, 3.0 * sigma, H, Hi, &hc, pm.hom, fin_sel, t.x1, t.x2);
-            if (hok && lane == 0) {
-                double rho_f = 0.0, rho_h = 0.0;
+            const bool hok = fit_homography(t.x1, t.x2, n, fin_mask, inl_list, ns, 3.0 * sigma, H,
+                                            Hi, &hc, pm.hom, fin_sel);
+            if (hok) {
+                // GRIC sums over the inliers in index order (lanes 0 and 1)
+                double f[9];
+                load9(sh.f, f);
+                double acc = 0.0;
                 uint64_t parallax = 0;
-                for (uint32_t i = 0; i < n; ++i) {
-                    if (!s.take(i)) continue;
-                    const double2 p = t.x1[i], q = t.x2[i];
-                    const double ef = sampson_error(sh.f, p, q) / s2;
-                    double2 a, b;
-                    const double e1 = htr(H, p, a) ? (a.x - q.x) * (a.x - q.x) + (a.y - q.y) * (a.y - q.y) : INFINITY;
-                    const double e2 = htr(Hi, q, b) ? (b.x - p.x) * (b.x - p.x) + (b.y - p.y) * (b.y - p.y) : INFINITY;
-                    const double eh = 0.25 * (e1 + e2) / s2;
-                    rho_f += fmin(ef, 2.0 * (4.0 - 3.0));
-                    rho_h += fmin(eh, 2.0 * (4.0 - 2.0));
-                    if (eh > 13.8 && ef < 10.8) parallax++;
+                for (uint32_t base = 0; base < n; base += NL) {
+                    const uint64_t m = bits64(fin_mask, base, n);
+                    if (!m) continue;
+                    bool par = false;
+                    if ((m >> lane) & 1u) {
+                        const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
+                        const double2 p = t.x1[base + lane], q = t.x2[base + lane];
+                        const double ef = sampson_error(f, p, q) / s2;
+                        double2 a, b;
+                        const double e1 = htr(H, p, a) ? (a.x - q.x) * (a.x - q.x) + (a.y - q.y) * (a.y - q.y) : INFINITY;
+                        const double e2 = htr(Hi, q, b) ? (b.x - p.x) * (b.x - p.x) + (b.y - p.y) * (b.y - p.y) : INFINITY;
+                        const double eh = 0.25 * (e1 + e2) / s2;
+                        s_cp.stage[slot][0] = fmin(ef, 2.0 * (4.0 - 3.0));
+                        s_cp.stage[slot][1] = fmin(eh, 2.0 * (4.0 - 2.0));
+                        par = eh > 13.8 && ef < 10.8;
+                    }
+                    parallax += (uint64_t)__popcll(__ballot(par));
+                    __syncthreads();
+                    acc = stage_sum(lane & 1, (uint32_t)__popcll(m), acc);
+                    __syncthreads();
                 }
+                const double rho_f = readlane_d(acc, 0), rho_h = readlane_d(acc, 1);
                 const double nf = (double)ns;
                 const double l1 = log(4.0), l2 = log(4.0 * nf);
                 selo.gric_f = rho_f + l1 * 3.0 * nf + l2 * 7.0;

@@ -497,26 +497,61 @@ __device__ bool accumulate_8pt(const S& s, uint32_t len, double* t1, double* t2,
     return true;
 }
 
-// matching.rs:539-556 computed by a pair of lanes (threadIdx.x ^ 1 is the
-// partner; both must be active with the same ata, t1, t2): the eigen-solve
-// (each lane accumulates five eigenvector rows), the degeneracy test, the
-// rank-2 projection and the denormalisation. ata is overwritten.
-__device__ bool solve_8pt_pair(double* ata, const double* t1, const double* t2, double* F) {
-    const int half = threadIdx.x & 1;
-    double v[5][9];
-    sym_eigen9_rows<5>(ata, 5 * half, v);
-    double ev[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) ev[k] = ata[ut(k, k)];
+// Lanes 3g, 3g+1, 3g+2 form solver group g (g < kGroups); a group splits the
+// nine eigenvector rows of a 9x9 eigen-solve three per lane while every lane
+// of the group rotates the same matrix, which keeps both in registers.
+constexpr int kGroups = 21;
+
+__device__ __forceinline__ int solver_group() { return (int)threadIdx.x / 3; }
+__device__ __forceinline__ int solver_part() { return (int)threadIdx.x - 3 * ((int)threadIdx.x / 3); }
+
+// Eigen-solve of the upper triangle `a` (45, overwritten with the final
+// rotated matrix) by a solver group; on return every lane of the group holds
+// the eigenvector of the smallest diagonal entry (first minimum) in vk.
+// All three lanes of the group must be active with the same `a`.
+__device__ __forceinline__ int eigen_min_group(double* a, double* vk) {
+    const int part = solver_part();
+    const int base = 3 * solver_group();
+    double v[3][9];
+    sym_eigen9_rows<3>(a, 3 * part, v);
+    // first minimum of the diagonal (the value is tracked so that no array
+    // is indexed at run time)
     int k = 0;
+    double dmin = a[ut(0, 0)];
 #pragma unroll
-    for (int i = 1; i < 9; ++i)
-        if (ev[i] < ev[k]) k = i;
+    for (int i = 1; i < 9; ++i) {
+        const double x = a[ut(i, i)];
+        if (x < dmin) {
+            dmin = x;
+            k = i;
+        }
+    }
+    double mine[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[i][j] : acc;
+        mine[i] = acc;
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vk[3 * t + i] = __shfl(mine[i], base + t);
+    return k;
+}
+
+// matching.rs:539-556 computed by a solver group (see eigen_min_group):
+// the eigen-solve, the degeneracy test, the rank-2 projection and the
+// denormalisation. ata is overwritten.
+__device__ bool solve_8pt_group(double* ata, const double* t1, const double* t2, double* F) {
+    double fn[9];
+    eigen_min_group(ata, fn);
     // second smallest and largest of the nine eigenvalues
     double e1 = INFINITY, e0 = INFINITY, e8 = -INFINITY;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        double x = ev[i];
+        double x = ata[ut(i, i)];
         if (x < e0) {
             e1 = e0;
             e0 = x;
@@ -526,21 +561,6 @@ __device__ bool solve_8pt_pair(double* ata, const double* t1, const double* t2, 
         if (x > e8) e8 = x;
     }
     if (e1 <= 1e-10 * e8) return false;
-    double mine[5], other[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[i][j] : acc;
-        mine[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < 5; ++i) other[i] = __shfl_xor(mine[i], 1);
-    double fn[9];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) fn[i] = half ? other[i] : mine[i];
-#pragma unroll
-    for (int i = 5; i < 9; ++i) fn[i] = half ? mine[i - 5] : other[i - 5];
     double r2[9];
     rank2_recompose(fn, r2);
     double t2t[9], tmp[9], f[9];
