@@ -268,38 +268,48 @@ __device__ uint32_t coop_count_of(const double* F, const double2* x1, const doub
     return cnt;
 }
 
+// rank -> index of the points of an n-bit bitset, in index order.
+__device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
+    const int lane = threadIdx.x;
+    uint32_t done = 0;
+    for (uint32_t base = 0; base < n; base += NL) {
+        const uint64_t m = bits64(bits, base, n);
+        if ((m >> lane) & 1u) list[done + (uint32_t)__popcll(m & lanes_below())] = base + lane;
+        done += (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+}
+
 // collinear_and_normalizers over the bitset `sel` (len points): fills the
 // Hartley normalisers t1, t2 and returns whether either image is nearly
 // collinear.
-__device__ bool coop_norm(const double2* x1, const double2* x2, uint32_t n,
-                          const uint32_t* sel, uint32_t len, double* t1, double* t2) {
+__device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* list,
+                          uint32_t len, double* t1, double* t2) {
     const int lane = threadIdx.x;
     double acc = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 a = x1[base + lane], b = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 a = x1[pi], b = x2[pi];
             s_cp.stage[slot][0] = a.x;
             s_cp.stage[slot][1] = a.y;
             s_cp.stage[slot][2] = b.x;
             s_cp.stage[slot][3] = b.y;
         }
         __syncthreads();
-        acc = stage_sum(lane & 3, (uint32_t)__popcll(m), acc);
+        acc = stage_sum(lane & 3, cnt, acc);
         __syncthreads();
     }
     const double nc = (double)(len > 0 ? len : 1);
     const double c1x = readlane_d(acc, 0) / nc, c1y = readlane_d(acc, 1) / nc;
     const double c2x = readlane_d(acc, 2) / nc, c2y = readlane_d(acc, 3) / nc;
     acc = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 a = x1[base + lane], b = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 a = x1[pi], b = x2[pi];
             const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
             s_cp.stage[slot][0] = ax * ax;
             s_cp.stage[slot][1] = ax * ay;
@@ -311,7 +321,7 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, uint32_t n,
             s_cp.stage[slot][7] = norm2(bx, by);
         }
         __syncthreads();
-        acc = stage_sum(lane & 7, (uint32_t)__popcll(m), acc);
+        acc = stage_sum(lane & 7, cnt, acc);
         __syncthreads();
     }
     double q[8];
@@ -322,18 +332,17 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, uint32_t n,
 
 // A^T A of the normalised 8-point system over `sel` (matching.rs:522-538);
 // lane k < 45 returns upper-triangle entry k.
-__device__ double coop_ata(const double2* x1, const double2* x2, uint32_t n,
-                           const uint32_t* sel, const double* t1, const double* t2) {
+__device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t* list,
+                           uint32_t len, const double* t1, const double* t2) {
     const int lane = threadIdx.x;
     int r, c;
     ut_rc(lane < 45 ? lane : 0, r, c);
     double acc = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 p = x1[base + lane], q = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 p = x1[pi], q = x2[pi];
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -349,7 +358,7 @@ __device__ double coop_ata(const double2* x1, const double2* x2, uint32_t n,
             row[8] = 1.0;
         }
         __syncthreads();
-        acc = stage_dot(r, c, (uint32_t)__popcll(m), acc);
+        acc = stage_dot(r, c, cnt, acc);
         __syncthreads();
     }
     return acc;
@@ -361,16 +370,15 @@ __device__ double coop_ata(const double2* x1, const double2* x2, uint32_t n,
 // (matching.rs:620-641 summed as the host does).
 __device__ double coop_lm_cost(const double* g, double s1, double s2, const double* t1,
                                const double* t2, const double2* x1, const double2* x2,
-                               uint32_t n, const uint32_t* sel) {
+                               const uint32_t* list, uint32_t len) {
     const int lane = threadIdx.x;
     const double q1 = s1 * s1, q2 = s2 * s2;
     double cost = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 p = x1[base + lane], q = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 p = x1[pi], q = x2[pi];
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -387,7 +395,7 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
             s_cp.stage[slot][0] = e * e;
         }
         __syncthreads();
-        cost = stage_sum(0, (uint32_t)__popcll(m), cost);
+        cost = stage_sum(0, cnt, cost);
         __syncthreads();
     }
     return cost;
@@ -397,7 +405,7 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
 // Sampson residuals over `sel` (matching.rs:585-618, :690-706).
 __device__ double coop_normal_eq(const double* g, double s1, double s2,
                                  const double* t1, const double* t2, const double2* x1,
-                                 const double2* x2, uint32_t n, const uint32_t* sel) {
+                                 const double2* x2, const uint32_t* list, uint32_t len) {
     const int lane = threadIdx.x;
     const double q1 = s1 * s1, q2 = s2 * s2;
     int u = 0, w = 0;
@@ -408,12 +416,11 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
         w = 9;
     }
     double acc = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 p = x1[base + lane], q = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 p = x1[pi], q = x2[pi];
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -441,22 +448,25 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
             }
         }
         __syncthreads();
-        acc = stage_dot(u, w, (uint32_t)__popcll(m), acc);
+        acc = stage_dot(u, w, cnt, acc);
         __syncthreads();
     }
     return acc;
 }
 
 // refine_sampson (matching.rs:661-760) over the bitset `sel` (len points),
-// all lanes; out receives the refined F (or F unchanged).
+// all lanes; out receives the refined F (or F unchanged). list: scratch for
+// the selected indices (len words).
 __device__ __noinline__ void coop_refine_sampson(const double* F, const double2* x1,
-                                    const double2* x2, uint32_t n, const uint32_t* sel,
-                                    uint32_t len, int iters, double* out) {
+                                                 const double2* x2, uint32_t n,
+                                                 const uint32_t* sel, uint32_t len, int iters,
+                                                 double* out, uint32_t* list) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) out[i] = F[i];
     if (len < 8) return;
+    coop_list(sel, n, list);
     double t1[9], t2[9], t1i[9], t2i[9];
-    coop_norm(x1, x2, n, sel, len, t1, t2);
+    coop_norm(x1, x2, list, len, t1, t2);
     if (!inv3(t1, t1i) || !inv3(t2, t2i)) return;
     const double s1 = t1[0], s2 = t2[0];
     double t2it[9], tmp[9], g0[9], g[9];
@@ -464,10 +474,10 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
     mm3(t2it, F, tmp);
     mm3(tmp, t1i, g0);
     if (!rank2_unit(g0, g)) return;
-    double cost = coop_lm_cost(g, s1, s2, t1, t2, x1, x2, n, sel);
+    double cost = coop_lm_cost(g, s1, s2, t1, t2, x1, x2, list, len);
     double lambda = 1e-3;
     for (int it = 0; it < iters; ++it) {
-        const double ne = coop_normal_eq(g, s1, s2, t1, t2, x1, x2, n, sel);
+        const double ne = coop_normal_eq(g, s1, s2, t1, t2, x1, x2, list, len);
         double jtj[45], jtr[9];
 #pragma unroll
         for (int k = 0; k < 45; ++k) jtj[k] = readlane_d(ne, k);
@@ -494,7 +504,7 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
                 lambda *= 10.0;
                 continue;
             }
-            double c = coop_lm_cost(cand, s1, s2, t1, t2, x1, x2, n, sel);
+            double c = coop_lm_cost(cand, s1, s2, t1, t2, x1, x2, list, len);
             if (c < cost) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) g[k] = cand[k];
@@ -604,23 +614,24 @@ __constant__ int kDltR1[9] = {0, 1, 2, 3, 3, 3, 4, 5, 6};
 __constant__ int kDltR2[9] = {3, 3, 3, 0, 1, 2, 7, 8, 9};
 
 // homography_dlt over the bitset `sel` (len points), all lanes; H is the
-// same in every lane.
+// same in every lane. list: scratch for the selected indices (len words).
 __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32_t n,
-                                    const uint32_t* sel, uint32_t len, double* H) {
+                                    const uint32_t* sel, uint32_t len, double* H,
+                                    uint32_t* list) {
     const int lane = threadIdx.x;
     if (len < 4) return false;
+    coop_list(sel, n, list);
     double t1[9], t2[9];
-    coop_norm(x1, x2, n, sel, len, t1, t2);
+    coop_norm(x1, x2, list, len, t1, t2);
     int r, c;
     ut_rc(lane < 45 ? lane : 0, r, c);
     const int a1 = kDltR1[r], b1 = kDltR1[c], a2 = kDltR2[r], b2 = kDltR2[c];
     double acc = 0.0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(sel, base, n);
-        if (!m) continue;
-        if ((m >> lane) & 1u) {
-            const uint32_t slot = (uint32_t)__popcll(m & lanes_below());
-            const double2 pp = x1[base + lane], qq = x2[base + lane];
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 pp = x1[pi], qq = x2[pi];
             double p3[3] = {pp.x, pp.y, 1.0}, v[3];
             mv3(t1, p3, v);
             const double x = v[0] / v[2], y = v[1] / v[2];
@@ -640,7 +651,6 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
             row[9] = w;
         }
         __syncthreads();
-        const uint32_t cnt = (uint32_t)__popcll(m);
         for (uint32_t j = 0; j < cnt; ++j) {
             const double* row = s_cp.stage[j];
             acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
@@ -679,18 +689,6 @@ __device__ uint32_t coop_count_h(const double* h, const double* hi, const double
     return cnt;
 }
 
-// rank -> index of the points of an n-bit bitset, in index order.
-__device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
-    const int lane = threadIdx.x;
-    uint32_t done = 0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(bits, base, n);
-        if ((m >> lane) & 1u) list[done + (uint32_t)__popcll(m & lanes_below())] = base + lane;
-        done += (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-}
-
 constexpr int kHypPerBatch = 3 * kGroups;  // hypotheses per batch: one per solver group, 3 rounds
 
 struct SelState {
@@ -706,11 +704,12 @@ __constant__ double kHM[4] = {4.0, 2.0, 1.0, 1.0};
 
 // matching.rs:300-369 over the inlier set `sel` (ns points, `list` maps
 // their ranks to indices), all lanes. H, Hi, *cnt_out are the same in every
-// lane; selbits is scratch (words(n) words).
+// lane; selbits (words(n) words) and list_buf (n words) are scratch.
 __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2, uint32_t n,
                                             const uint32_t* sel, const uint32_t* list,
                                             uint32_t ns, double th, double* H, double* Hi,
-                                            uint32_t* cnt_out, SelState& sh, uint32_t* selbits) {
+                                            uint32_t* cnt_out, SelState& sh, uint32_t* selbits,
+                                            uint32_t* list_buf) {
     const int lane = threadIdx.x;
     if (ns < 4) return false;
     if (lane == 0) {
@@ -798,7 +797,7 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
     for (int r = 0; r < 4; ++r) {
         const uint32_t nsel = coop_count_h(ch, chi, x1, x2, n, sel, th * kHM[r], selbits);
         double gg[9], gi[9];
-        if (!coop_homography_dlt(x1, x2, n, selbits, nsel, gg)) break;
+        if (!coop_homography_dlt(x1, x2, n, selbits, nsel, gg, list_buf)) break;
         if (!inv3(gg, gi)) break;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -840,17 +839,22 @@ union PhaseMem {
 };
 __shared__ PhaseMem s_pm;
 
-// Correspondences and per-hypothesis bitsets live in LDS for pairs up to
-// this size; larger pairs read them from L2-resident device memory.
+// Occupancy: two waves (pairs) per SIMD hide the FP64 dependency latency of
+// the serial solver chains better than LDS-resident point data does, so by
+// default correspondences and bitsets stay in L2-resident device memory
+// (LDS 17 KB per workgroup) and every function fits 256 VGPRs. A build with
+// SKH_RANSAC_LDS_PTS=N stages pairs of up to N correspondences in LDS.
 #ifndef SKH_RANSAC_LDS_PTS
-#define SKH_RANSAC_LDS_PTS 384
+#define SKH_RANSAC_LDS_PTS 0
 #endif
 #ifndef SKH_RANSAC_WAVES_PER_EU
-#define SKH_RANSAC_WAVES_PER_EU 1
+#define SKH_RANSAC_WAVES_PER_EU 2
 #endif
 constexpr uint32_t kLdsPts = SKH_RANSAC_LDS_PTS;
 constexpr uint32_t kLdsWords = (kLdsPts + 31) / 32;
-constexpr uint32_t kMaskSets = 3 * NL + 8;  // lo, alt, sel per hypothesis + pool(6) + final(2)
+// bitsets of W words: lo, alt, sel per hypothesis, pool (6), final (2), then
+// an index list of up to 32 W >= n words
+constexpr uint32_t kMaskSets = 3 * NL + 8 + 32;
 
 struct Shared {
     double poolF[6][9];
@@ -912,10 +916,11 @@ __device__ __noinline__ void hypotheses_groups(const RansacTask& t, HypMem& hm, 
 // sums run wave-cooperatively one candidate after another, then pairs of
 // lanes solve up to 32 candidates' 8-point systems at once, then each
 // candidate's Sampson pass gives its count at x1 and its next selection.
-// Results: loF[h], loCnt[h] and the inlier bitset lo + h * W.
+// Results: loF[h], loCnt[h] and the inlier bitset lo + h * W; list is
+// scratch (n words).
 __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, uint64_t cand,
                                             double th2, uint32_t* lo, uint32_t* alt,
-                                            uint32_t* selb, uint32_t W) {
+                                            uint32_t* selb, uint32_t W, uint32_t* list) {
     const double2* x1 = t.x1;
     const double2* x2 = t.x2;
     const uint32_t n = t.n;
@@ -952,14 +957,18 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                 const int h = __ffsll((unsigned long long)todo) - 1;
                 todo &= todo - 1;
                 const uint32_t ns = hm.loNs[h];
-                const uint32_t* sel = selb + (size_t)h * W;
                 double t1[9], t2[9];
                 // fundamental_8pt: fewer than 8 points or collinear -> None, LO stops
-                if (ns < 8 || coop_norm(x1, x2, n, sel, ns, t1, t2)) {
+                if (ns < 8) {
                     active &= ~(1ull << h);
                     continue;
                 }
-                const double a = coop_ata(x1, x2, n, sel, t1, t2);
+                coop_list(selb + (size_t)h * W, n, list);
+                if (coop_norm(x1, x2, list, ns, t1, t2)) {
+                    active &= ~(1ull << h);
+                    continue;
+                }
+                const double a = coop_ata(x1, x2, list, ns, t1, t2);
                 const bool me = solver_group() == rank;
 #pragma unroll
                 for (int k = 0; k < 45; ++k) {
@@ -1045,6 +1054,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
     uint32_t* pool_masks = masks + (size_t)3 * NL * W;          // 6 * W
     uint32_t* fin_mask = masks + (size_t)(3 * NL + 6) * W;      // W
     uint32_t* fin_sel = fin_mask + W;                           // W
+    uint32_t* list_buf = fin_sel + W;                           // 32 * W >= n
     __syncthreads();
 
     // matching.rs:946-949
@@ -1137,7 +1147,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         if (lane == 0) pc[9] += __popcll(cmask);
         if (!cand) pm.hyp.loCnt[lane] = -1;
         __syncthreads();
-        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W);
+        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W, list_buf);
         lap(2);
         // replay in order (lane 0)
         if (lane == 0) {
@@ -1232,7 +1242,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             uint32_t ns = coop_mask_of(cur, t.x1, t.x2, n, th2 * kNM[0] * kNM[0], sel);
             for (int r = 0; r < 6; ++r) {
                 double nxt[9];
-                coop_refine_sampson(cur, t.x1, t.x2, n, sel, ns, 30, nxt);
+                coop_refine_sampson(cur, t.x1, t.x2, n, sel, ns, 30, nxt, list_buf);
                 load9(nxt, cur);
                 if (lane == 0) {
 #pragma unroll
@@ -1262,7 +1272,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         for (int r = 0; r < 2; ++r) {
             const uint32_t ns = coop_mask_of(f, t.x1, t.x2, n, th2, fin_sel);
             double g[9];
-            coop_refine_sampson(f, t.x1, t.x2, n, fin_sel, ns, 30, g);
+            coop_refine_sampson(f, t.x1, t.x2, n, fin_sel, ns, 30, g, list_buf);
             const uint32_t c = coop_count_of(g, t.x1, t.x2, n, th2);
             if (c < ns) break;
             load9(g, f);
@@ -1307,7 +1317,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         }
         if (ok) {
             double t1[9], t2[9];
-            if (coop_norm(t.x1, t.x2, n, fin_mask, (uint32_t)ic, t1, t2)) ok = false;
+            coop_list(fin_mask, n, list_buf);
+            if (coop_norm(t.x1, t.x2, list_buf, (uint32_t)ic, t1, t2)) ok = false;
         }
         if (lane == 0) {
             sh.result = ok ? 1 : 0;
@@ -1446,7 +1457,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             double H[9], Hi[9];
             uint32_t hc = 0;
             const bool hok = fit_homography(t.x1, t.x2, n, fin_mask, inl_list, ns, 3.0 * sigma, H,
-                                            Hi, &hc, pm.hom, fin_sel);
+                                            Hi, &hc, pm.hom, fin_sel, list_buf);
             if (hok) {
                 // GRIC sums over the inliers in index order (lanes 0 and 1)
                 double f[9];
