@@ -173,6 +173,72 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// ---------------------------------------------------------------- sampler
+//
+// The seeded LCG streams (matching.rs:956-962, :314-320, :884-889) are
+// generated 64 draws at a time: lane l jumps l + 1 steps ahead
+// (st' = A^(l+1) st + c_(l+1) mod 2^64) and reduces its state to a draw;
+// the rejection rule then runs on the scalar unit over the draws in order.
+
+struct LcgLane {
+    uint64_t mul, add;  // jump of threadIdx.x + 1 steps
+};
+
+__device__ __forceinline__ LcgLane lcg_lane_jump() {
+    uint64_t m = 1, a = 0;
+    for (int j = 0; j <= (int)threadIdx.x; ++j) {
+        m = m * 6364136223846793005ull;
+        a = a * 6364136223846793005ull + 1442695040888963407ull;
+    }
+    return {m, a};
+}
+
+struct LcgStream {
+    uint64_t next;  // state after the last generated draw (uniform)
+    uint32_t raw;   // this lane's draw of the current block of 64
+    uint32_t pos;   // first unconsumed draw of the block; 64 = none left
+};
+
+__device__ __forceinline__ void lcg_refill(LcgStream& s, const LcgLane& j, uint32_t m) {
+    const uint64_t st = j.mul * s.next + j.add;
+    s.raw = (uint32_t)(st >> 33) % m;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)st, NL - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(st >> 32), NL - 1);
+    s.next = ((uint64_t)hi << 32) | lo;
+    s.pos = 0;
+}
+
+// nh samples of PER distinct draws in [0, m) each, in stream order (a draw
+// equal to one already taken by the sample is redrawn); lane 0 writes
+// out[h * stride + e]. All lanes.
+template <int PER>
+__device__ void lcg_samples(LcgStream& s, const LcgLane& j, uint32_t m, int nh, uint32_t* out,
+                            int stride) {
+    for (int h = 0; h < nh; ++h) {
+        uint32_t v[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[e] = 0;
+        int k = 0;
+        while (k < PER) {
+            if (s.pos == NL) lcg_refill(s, j, m);
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)s.raw, (int)s.pos);
+            ++s.pos;
+            bool dup = false;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) dup |= e < k && v[e] == c;
+            if (!dup) {
+#pragma unroll
+                for (int e = 0; e < PER; ++e) v[e] = e == k ? c : v[e];
+                ++k;
+            }
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int e = 0; e < PER; ++e) out[h * stride + e] = v[e];
+        }
+    }
+}
+
 // Bits [base, base + 64) of an n-bit bitset stored in 32-bit words.
 __device__ __forceinline__ uint64_t bits64(const uint32_t* bits, uint32_t base, uint32_t n) {
     const uint32_t w = base >> 5;
@@ -719,29 +785,12 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
         sh.it = 0;
     }
     __syncthreads();
-    uint64_t st = 0x2545F4914F6CDD1Dull;
+    const LcgLane jump = lcg_lane_jump();
+    LcgStream hs{0x2545F4914F6CDD1Dull, 0u, (uint32_t)NL};
     const int g = solver_group(), part = solver_part(), gbase = 3 * g;
     while (sh.it < sh.iters) {
-        if (lane == 0) {
-            // matching.rs:314-320: four distinct ranks among the inliers
-            for (int h = 0; h < kHypPerBatch; ++h) {
-                uint32_t s[4] = {0, 0, 0, 0};
-                int k = 0;
-                while (k < 4) {
-                    const uint32_t c = lcg_draw(st, ns);
-                    bool dup = false;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) dup |= e < k && s[e] == c;
-                    if (!dup) {
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) s[e] = e == k ? c : s[e];
-                        ++k;
-                    }
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) sh.idx[h][e] = s[e];
-            }
-        }
+        // matching.rs:314-320: four distinct ranks among the inliers
+        lcg_samples<4>(hs, jump, ns, kHypPerBatch, &sh.idx[0][0], 4);
         __syncthreads();
 #pragma unroll 1
         for (int round = 0; round < 3; ++round) {
@@ -1069,7 +1118,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         return;
     }
     const double th2 = cfg.threshold_px * cfg.threshold_px;
-    uint64_t st = cfg.seed ^ 0x9E3779B97F4A7C15ull;
+    const LcgLane jump = lcg_lane_jump();
+    LcgStream smp{cfg.seed ^ 0x9E3779B97F4A7C15ull, 0u, (uint32_t)NL};
     if (lane == 0) {
         sh.npool = 0;
         sh.bestCnt = 0;
@@ -1104,26 +1154,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
     while (sh.it < sh.iters) {
         lap(3);
         if (lane == 0) pc[8] += 1;
-        if (lane == 0) {
-            // matching.rs:971-979: eight distinct draws per hypothesis
-            for (int h = 0; h < kHypPerBatch; ++h) {
-                uint32_t s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                int k = 0;
-                while (k < 8) {
-                    const uint32_t c = lcg_draw(st, n);
-                    bool dup = false;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) dup |= e < k && s[e] == c;
-                    if (!dup) {
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) s[e] = e == k ? c : s[e];
-                        ++k;
-                    }
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) pm.hyp.idx[h][e] = s[e];
-            }
-        }
+        // matching.rs:971-979: eight distinct draws per hypothesis
+        lcg_samples<8>(smp, jump, n, kHypPerBatch, &pm.hyp.idx[0][0], 8);
         __syncthreads();
         lap(0);
         if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
@@ -1344,9 +1376,13 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             line_ok = false;  // off_line_count = 0 < 8
             break;
         }
-        if (lane == 0) {
-            uint64_t ls = 0x6A09E667F3BCC909ull;
-            for (int k = 0; k < 600; ++k) pm.draws[k] = lcg_draw(ls, ns);
+        {
+            // matching.rs:884-889: 300 pairs of draws
+            LcgStream ls{0x6A09E667F3BCC909ull, 0u, (uint32_t)NL};
+            for (int b = 0; b < 600; b += NL) {
+                lcg_refill(ls, jump, ns);
+                if (b + lane < 600) pm.draws[b + lane] = ls.raw;
+            }
         }
         __syncthreads();
         int best = 0;

@@ -150,11 +150,6 @@ __device__ __forceinline__ bool inv3(const double* m, double* out) {
     return true;
 }
 
-__device__ __forceinline__ uint32_t lcg_draw(uint64_t& st, uint32_t m) {
-    st = st * 6364136223846793005ull + 1442695040888963407ull;
-    return (uint32_t)(st >> 33) % m;
-}
-
 // ---------------------------------------------------------- Jacobi eigen 9x9
 
 // Symmetric matrix stored as its upper triangle, index of (r, c), r <= c.
