@@ -9,16 +9,15 @@
  * 577), Cholesky (:714), try_inverse (:263, :340, :359, :675). nalgebra is not
  * vendored and cannot be built here, so the oracle uses its own solvers with
  * the same mathematical contract:
- *   - symmetric 9x9 eigen: cyclic Jacobi (Golub & Van Loan, Matrix
- *     Computations, 4th ed., Alg. 8.5.3) with the classic tiny-element rule;
+ *   - symmetric 9x9 eigen: Householder tridiagonalisation and implicit QL
+ *     (the scheme of nalgebra's SymmetricEigen: reduction to tridiagonal
+ *     form, then shifted implicit iterations);
  *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958);
  *   - Cholesky: right-looking column factorisation, forward/back solves;
  *   - 3x3 inverse: cofactor expansion.
- * A rotation that is not due (zero or negligible element) is skipped and the
- * element set to zero; the GPU performs the same operations in the same
- * order and produces the same bits. The F bits therefore pin the HIP path to
- * this oracle, not to the Rust binary (parity with the reference is at the
- * level of its tests).
+ * The GPU performs the same operations in the same order and produces the
+ * same bits. The F bits therefore pin the HIP path to this oracle, not to
+ * the Rust binary (parity with the reference is at the level of its tests).
  *
  * Matrices are row-major double[9] / double[81]. All sums run in the
  * reference's order (first element first).
@@ -110,79 +109,133 @@ static inline uint64_t rnd(uint64_t* st, uint64_t m) {
 
 /* ------------------------------------------------------------ eigen / SVD */
 
-/* One Jacobi rotation of the symmetric 9x9 `a` (and eigenvector matrix `v`)
- * annihilating a[p][q]. Only called for a due rotation: the sweep skips the
- * others and just zeroes a[p][q]. */
-static inline void jacobi_rotate9(double* a, double* v, int p, int q, double c, double s, double t) {
-    const double apq = a[9 * p + q];
-    for (int k = 0; k < 9; ++k) {
-        if (k == p || k == q) continue;
-        double akp = a[9 * k + p], akq = a[9 * k + q];
-        double np = c * akp - s * akq;
-        double nq = s * akp + c * akq;
-        a[9 * k + p] = np; a[9 * p + k] = np;
-        a[9 * k + q] = nq; a[9 * q + k] = nq;
+/* sqrt(a^2 + b^2) without undue overflow: the larger magnitude times
+ * sqrt(1 + ratio^2). */
+static inline double pythag(double a, double b) {
+    const double aa = fabs(a), ab = fabs(b);
+    if (aa > ab) {
+        const double r = ab / aa;
+        return aa * sqrt(1.0 + r * r);
     }
-    a[9 * p + p] = a[9 * p + p] - t * apq;
-    a[9 * q + q] = a[9 * q + q] + t * apq;
-    a[9 * p + q] = 0.0;
-    a[9 * q + p] = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        double vkp = v[9 * k + p], vkq = v[9 * k + q];
-        v[9 * k + p] = c * vkp - s * vkq;
-        v[9 * k + q] = s * vkp + c * vkq;
-    }
+    if (ab == 0.0) return 0.0;
+    const double r = aa / ab;
+    return ab * sqrt(1.0 + r * r);
 }
 
-/* Rotation parameters for a[p][q] in sweep `sweep` (classic rule: after the
- * fourth sweep an element negligible against both diagonal entries is set to
- * zero without rotating). Returns 1 if a rotation is due. */
-static inline int jacobi_params(double app, double aqq, double apq, int sweep, double* c,
-                                double* s, double* t) {
-    *c = 1.0; *s = 0.0; *t = 0.0;
-    if (apq == 0.0) return 0;
-    double g = 100.0 * fabs(apq);
-    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return 0;
-    double h = aqq - app;
-    double tt;
-    if (fabs(h) + g == fabs(h)) {
-        tt = apq / h;
-    } else {
-        double theta = 0.5 * h / apq;
-        tt = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
-        if (theta < 0.0) tt = -tt;
+/* Householder tridiagonalisation of the symmetric 9x9 `a` (row-major, full,
+ * overwritten; Golub & Van Loan, Matrix Computations 4th ed., Alg. 8.3.1),
+ * accumulating q = H_0 H_1 ... H_6 row by row. Step k reflects
+ * x = a[k+1..8][k] onto beta e_1 with beta = -sign(x_0) |x| and
+ * v = (1, x_1 / (x_0 - beta), ...), tau = (beta - x_0) / beta; the trailing
+ * block B becomes B - v w^T - w v^T with p = tau B v,
+ * w = p - (tau / 2)(v^T p) v. d: diagonal, e[i]: the (i, i+1) coupling,
+ * e[8] = 0. */
+static void tridiag9(double* a, double* q, double* d, double* e) {
+    for (int i = 0; i < 81; ++i) q[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 7; ++k) {
+        double xn2 = 0.0;
+        for (int i = k + 2; i < 9; ++i) xn2 = xn2 + a[9 * i + k] * a[9 * i + k];
+        const double x0 = a[9 * (k + 1) + k];
+        if (xn2 == 0.0) continue;
+        const double nrm = sqrt(x0 * x0 + xn2);
+        const double beta = x0 >= 0.0 ? -nrm : nrm;
+        const double tau = (beta - x0) / beta;
+        const double rs = 1.0 / (x0 - beta);
+        double v[9], p[9], w[9];
+        v[k + 1] = 1.0;
+        for (int i = k + 2; i < 9; ++i) v[i] = a[9 * i + k] * rs;
+        for (int i = k + 1; i < 9; ++i) {
+            double sum = 0.0;
+            for (int j = k + 1; j < 9; ++j) sum = sum + a[9 * i + j] * v[j];
+            p[i] = tau * sum;
+        }
+        double vp = 0.0;
+        for (int i = k + 1; i < 9; ++i) vp = vp + v[i] * p[i];
+        const double kk = 0.5 * tau * vp;
+        for (int i = k + 1; i < 9; ++i) w[i] = p[i] - kk * v[i];
+        for (int i = k + 1; i < 9; ++i)
+            for (int j = i; j < 9; ++j) {
+                const double nv = a[9 * i + j] - (v[i] * w[j] + w[i] * v[j]);
+                a[9 * i + j] = nv;
+                a[9 * j + i] = nv;
+            }
+        a[9 * (k + 1) + k] = beta;
+        a[9 * k + (k + 1)] = beta;
+        for (int i = k + 2; i < 9; ++i) {
+            a[9 * i + k] = 0.0;
+            a[9 * k + i] = 0.0;
+        }
+        for (int r = 0; r < 9; ++r) {
+            double sum = 0.0;
+            for (int j = k + 1; j < 9; ++j) sum = sum + q[9 * r + j] * v[j];
+            const double t = tau * sum;
+            for (int j = k + 1; j < 9; ++j) q[9 * r + j] = q[9 * r + j] - t * v[j];
+        }
     }
-    double cc = 1.0 / sqrt(1.0 + tt * tt);
-    *c = cc; *s = tt * cc; *t = tt;
-    return 1;
+    for (int i = 0; i < 9; ++i) d[i] = a[9 * i + i];
+    for (int i = 0; i < 8; ++i) e[i] = a[9 * (i + 1) + i];
+    e[8] = 0.0;
 }
 
-#define JACOBI_MAX_SWEEPS 50
+#define QL_MAX_ITERS 30
 
-void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor) {
-    double a[81], v[81];
-    memcpy(a, A, sizeof a);
-    for (int i = 0; i < 81; ++i) v[i] = (i % 10 == 0) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < JACOBI_MAX_SWEEPS; ++sweep) {
-        double off = 0.0;
-        for (int p = 0; p < 8; ++p)
-            for (int q = p + 1; q < 9; ++q) off = off + fabs(a[9 * p + q]);
-        if (off == 0.0) break;
-        for (int p = 0; p < 8; ++p)
-            for (int q = p + 1; q < 9; ++q) {
-                double c, s, t;
-                if (jacobi_params(a[9 * p + p], a[9 * q + q], a[9 * p + q], sweep, &c, &s, &t)) {
-                    jacobi_rotate9(a, v, p, q, c, s, t);
-                } else {
-                    /* no rotation: a zero stays, a negligible element is dropped */
-                    a[9 * p + q] = 0.0;
-                    a[9 * q + p] = 0.0;
+/* Implicit QL with shifts on the symmetric tridiagonal (d, e) (Wilkinson &
+ * Reinsch, Handbook for Automatic Computation II, tql2), rotating the rows
+ * of q. Eigenvalue l is split off once e[l] is negligible against its
+ * neighbours; at most QL_MAX_ITERS sweeps per eigenvalue. */
+static void tql9(double* d, double* e, double* q) {
+    for (int l = 0; l < 9; ++l) {
+        for (int iter = 0; iter < QL_MAX_ITERS; ++iter) {
+            int m = l;
+            for (; m < 8; ++m) {
+                const double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) + dd == dd) break;
+            }
+            if (m == l) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = pythag(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int underflow = 0;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * e[i], b = c * e[i];
+                r = pythag(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = d[i + 1] - p;
+                    e[m] = 0.0;
+                    underflow = 1;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < 9; ++k) {
+                    const double fz = q[9 * k + i + 1];
+                    q[9 * k + i + 1] = s * q[9 * k + i] + c * fz;
+                    q[9 * k + i] = c * q[9 * k + i] - s * fz;
                 }
             }
+            if (underflow) continue;
+            d[l] = d[l] - p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
     }
+}
+
+void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor) {
+    double a[81], q[81], d[9], e[9];
+    memcpy(a, A, sizeof a);
+    tridiag9(a, q, d, e);
+    tql9(d, e, q);
     for (int k = 0; k < 9; ++k) {
-        evals[k] = a[9 * k + k];
-        for (int i = 0; i < 9; ++i) evecs_colmajor[9 * k + i] = v[9 * i + k];
+        evals[k] = d[k];
+        for (int i = 0; i < 9; ++i) evecs_colmajor[9 * k + i] = q[9 * i + k];
     }
 }
 

@@ -651,14 +651,15 @@ __device__ void homography_accumulate(const S& s, uint32_t len, double* t1, doub
 #pragma unroll
             for (int c = r; c < 9; ++c)
                 ata[ut(r, c)] = ata[ut(r, c)] + (r1[r] * r1[c] + r2[r] * r2[c]);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // Eigen-solve and denormalisation of a DLT normal matrix by a solver group.
 __device__ bool homography_finish_group(double* ata, const double* t1, const double* t2,
                                         double* H) {
-    double hn[9];
-    eigen_min_group(ata, hn);
+    double hn[9], ev[9];
+    eigen_min_group(ata, ev, hn);
     bool fin = true;
 #pragma unroll
     for (int i = 0; i < 9; ++i) fin = fin && isfinite(hn[i]);
@@ -766,6 +767,34 @@ struct SelState {
     uint64_t iters, it;
 };
 
+// Solver group g fits homography hypothesis kGroups * round + g from its
+// four samples and counts its inliers over the ns listed points (split
+// across the group's lanes). Out of line: its registers are allocated on
+// their own.
+__device__ __noinline__ void hom_round(const double2* x1, const double2* x2, const uint32_t* list,
+                                       uint32_t ns, double th, SelState& sh, int round) {
+    const int g = solver_group(), part = solver_part(), gbase = 3 * g;
+    const int h = kGroups * round + g;
+    if (g >= kGroups) return;
+    PtsRank4 s4{x1, x2, list, sh.idx[h]};
+    double t1[9], t2[9], ata[45], hh[9], hi[9];
+    homography_accumulate(s4, 4, t1, t2, ata);
+    int c = -1;
+    if (homography_finish_group(ata, t1, t2, hh) && inv3(hh, hi)) {
+        uint32_t cc = 0;
+        for (uint32_t r = part; r < ns; r += 3) {
+            const uint32_t i = list[r];
+            cc += homography_error(hh, hi, x1[i], x2[i]) < th ? 1u : 0u;
+        }
+        c = (int)(__shfl(cc, gbase) + __shfl(cc, gbase + 1) + __shfl(cc, gbase + 2));
+        if (part == 0) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sh.F[h][i] = hh[i];
+        }
+    }
+    if (part == 0) sh.cnt[h] = c;
+}
+
 __constant__ double kHM[4] = {4.0, 2.0, 1.0, 1.0};
 
 // matching.rs:300-369 over the inlier set `sel` (ns points, `list` maps
@@ -787,34 +816,11 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
     __syncthreads();
     const LcgLane jump = lcg_lane_jump();
     LcgStream hs{0x2545F4914F6CDD1Dull, 0u, (uint32_t)NL};
-    const int g = solver_group(), part = solver_part(), gbase = 3 * g;
     while (sh.it < sh.iters) {
         // matching.rs:314-320: four distinct ranks among the inliers
         lcg_samples<4>(hs, jump, ns, kHypPerBatch, &sh.idx[0][0], 4);
         __syncthreads();
-#pragma unroll 1
-        for (int round = 0; round < 3; ++round) {
-            const int h = kGroups * round + g;
-            if (g < kGroups) {
-                PtsRank4 s4{x1, x2, list, sh.idx[h]};
-                double t1[9], t2[9], ata[45], hh[9], hi[9];
-                homography_accumulate(s4, 4, t1, t2, ata);
-                int c = -1;
-                if (homography_finish_group(ata, t1, t2, hh) && inv3(hh, hi)) {
-                    uint32_t cc = 0;
-                    for (uint32_t r = part; r < ns; r += 3) {
-                        const uint32_t i = list[r];
-                        cc += homography_error(hh, hi, x1[i], x2[i]) < th ? 1u : 0u;
-                    }
-                    c = (int)(__shfl(cc, gbase) + __shfl(cc, gbase + 1) + __shfl(cc, gbase + 2));
-                    if (part == 0) {
-#pragma unroll
-                        for (int i = 0; i < 9; ++i) sh.F[h][i] = hh[i];
-                    }
-                }
-                if (part == 0) sh.cnt[h] = c;
-            }
-        }
+        for (int round = 0; round < 3; ++round) hom_round(x1, x2, list, ns, th, sh, round);
         __syncthreads();
         if (lane == 0) {
             for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
@@ -932,44 +938,59 @@ __device__ __forceinline__ void load9(const double* src, double* dst) {
 // Solver group g solves hypothesis kGroups * round + g (all three lanes
 // accumulate its 8-point system, each carries three eigenvector rows) and
 // splits its Sampson count; results: hCnt[h] (-1 when fundamental_8pt gives
-// None) and the hypothesis F in loF[h].
-__device__ __noinline__ void hypotheses_groups(const RansacTask& t, HypMem& hm, double th2) {
-    const double2* x1 = t.x1;
-    const double2* x2 = t.x2;
-    const uint32_t n = t.n;
+// None) and the hypothesis F in loF[h]. Kept out of line so that its
+// registers are allocated on their own.
+__device__ __noinline__ void hyp_round(const double2* x1, const double2* x2, uint32_t n,
+                                       double th2, HypMem& hm, int round) {
     const int g = solver_group(), part = solver_part(), gbase = 3 * g;
-#pragma unroll 1
-    for (int round = 0; round < 3; ++round) {
-        const int h = kGroups * round + g;
-        if (g < kGroups) {
-            PtsIdx8 s{x1, x2, hm.idx[h]};
-            double t1[9], t2[9], ata[45], F[9];
-            int cnt = -1;
-            if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_group(ata, t1, t2, F)) {
-                uint32_t c = 0;
-                for (uint32_t i = part; i < n; i += 3)
-                    c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
-                cnt = (int)(__shfl(c, gbase) + __shfl(c, gbase + 1) + __shfl(c, gbase + 2));
-                if (part == 0) {
+    const int h = kGroups * round + g;
+    if (g >= kGroups) return;
+    PtsIdx8 s{x1, x2, hm.idx[h]};
+    double t1[9], t2[9], ata[45], F[9];
+    int cnt = -1;
+    if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_group(ata, t1, t2, F)) {
+        uint32_t c = 0;
+        for (uint32_t i = part; i < n; i += 3) c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
+        cnt = (int)(__shfl(c, gbase) + __shfl(c, gbase + 1) + __shfl(c, gbase + 2));
+        if (part == 0) {
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
-                }
-            }
-            if (part == 0) hm.hCnt[h] = cnt;
+            for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
         }
     }
+    if (part == 0) hm.hCnt[h] = cnt;
 }
+
+// The 8-point tail of one LO candidate from its hand-off record (A^T A in
+// slots 0..44, the Hartley scales and offsets in 45..50), solved by the
+// calling solver group; slot 51 receives 1 or 0 (Some / None), 52..60 F.
+__device__ __noinline__ void lo_group_solve(double* hd) {
+    double ata[45], F[9];
+#pragma unroll
+    for (int k = 0; k < 45; ++k) ata[k] = hd[k];
+    const double t1[9] = {hd[45], 0.0, hd[46], 0.0, hd[45], hd[47], 0.0, 0.0, 1.0};
+    const double t2[9] = {hd[48], 0.0, hd[49], 0.0, hd[48], hd[50], 0.0, 0.0, 1.0};
+    const bool ok = solve_8pt_group(ata, t1, t2, F);
+    if (solver_part() == 0) {
+        hd[51] = ok ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hd[52 + k] = F[k];
+    }
+}
+
+constexpr uint32_t kHandStride = 64;                         // doubles per candidate record
+constexpr uint32_t kHandWords = 2 * kGroups * kHandStride;   // 32-bit words
 
 // Local optimisation (matching.rs:993-1015) of every candidate in `cand`
 // (hypothesis lanes), stage by stage: for each stage the candidates' point
 // sums run wave-cooperatively one candidate after another, then pairs of
 // lanes solve up to 32 candidates' 8-point systems at once, then each
 // candidate's Sampson pass gives its count at x1 and its next selection.
-// Results: loF[h], loCnt[h] and the inlier bitset lo + h * W; list is
-// scratch (n words).
+// Results: loF[h], loCnt[h] and the inlier bitset lo + h * W; list (n
+// words) and hand (kGroups candidate records) are device scratch.
 __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, uint64_t cand,
                                             double th2, uint32_t* lo, uint32_t* alt,
-                                            uint32_t* selb, uint32_t W, uint32_t* list) {
+                                            uint32_t* selb, uint32_t W, uint32_t* list,
+                                            double* hand) {
     const double2* x1 = t.x1;
     const double2* x2 = t.x2;
     const uint32_t n = t.n;
@@ -996,12 +1017,6 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
             // up to kGroups candidates, one solver group each
             uint64_t filled = 0;
             int rank = 0;
-            double ata[45], tt1[9], tt2[9];
-#pragma unroll
-            for (int k = 0; k < 45; ++k) ata[k] = 0.0;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) tt1[k] = tt2[k] = 0.0;
-            bool mine = false;
             while (todo && rank < kGroups) {
                 const int h = __ffsll((unsigned long long)todo) - 1;
                 todo &= todo - 1;
@@ -1018,34 +1033,33 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                     continue;
                 }
                 const double a = coop_ata(x1, x2, list, ns, t1, t2);
-                const bool me = solver_group() == rank;
-#pragma unroll
-                for (int k = 0; k < 45; ++k) {
-                    const double v = readlane_d(a, k);
-                    ata[k] = me ? v : ata[k];
+                double* hd = hand + (size_t)rank * kHandStride;
+                if (lane < 45) hd[lane] = a;
+                if (lane == 0) {
+                    hd[45] = t1[0];
+                    hd[46] = t1[2];
+                    hd[47] = t1[5];
+                    hd[48] = t2[0];
+                    hd[49] = t2[2];
+                    hd[50] = t2[5];
                 }
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    tt1[k] = me ? t1[k] : tt1[k];
-                    tt2[k] = me ? t2[k] : tt2[k];
-                }
-                mine = mine || me;
                 filled |= 1ull << h;
                 ++rank;
             }
-            double g[9];
-            int ok = 0;
-            if (mine) ok = solve_8pt_group(ata, tt1, tt2, g) ? 1 : 0;
+            __syncthreads();
+            if (solver_group() < rank) lo_group_solve(hand + (size_t)solver_group() * kHandStride);
+            __syncthreads();
             rank = 0;
             for (uint64_t m = filled; m; m &= m - 1, ++rank) {
                 const int h = __ffsll((unsigned long long)m) - 1;
-                if (!__builtin_amdgcn_readlane(ok, 3 * rank)) {
+                const double* hd = hand + (size_t)rank * kHandStride;
+                if (hd[51] == 0.0) {
                     active &= ~(1ull << h);
                     continue;
                 }
                 double gh[9];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) gh[k] = readlane_d(g[k], 3 * rank);
+                for (int k = 0; k < 9; ++k) gh[k] = hd[52 + k];
                 const bool sw = (swapped >> h) & 1u;
                 uint32_t* spare = (sw ? lo : alt) + (size_t)h * W;
                 uint32_t ns = 0;
@@ -1096,7 +1110,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         t.x1 = lds_x1;
         t.x2 = lds_x2;
     }
-    uint32_t* masks = W <= kLdsWords && kLdsWords > 0 ? lds_masks : t.scratch;
+    double* hand = reinterpret_cast<double*>(t.scratch);  // kHandWords, 256-byte aligned
+    uint32_t* masks = W <= kLdsWords && kLdsWords > 0 ? lds_masks : t.scratch + kHandWords;
     uint32_t* lo_masks = masks;                                 // NL * W
     uint32_t* alt_masks = masks + (size_t)NL * W;               // NL * W
     uint32_t* sel_masks = masks + (size_t)2 * NL * W;           // NL * W
@@ -1159,7 +1174,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         __syncthreads();
         lap(0);
         if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
-        hypotheses_groups(t, pm.hyp, th2);
+        for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, pm.hyp, round);
         __syncthreads();
         lap(1);
         const int cnt = pm.hyp.hCnt[lane];
@@ -1179,7 +1194,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         if (lane == 0) pc[9] += __popcll(cmask);
         if (!cand) pm.hyp.loCnt[lane] = -1;
         __syncthreads();
-        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W, list_buf);
+        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W, list_buf, hand);
         lap(2);
         // replay in order (lane 0)
         if (lane == 0) {
@@ -1583,8 +1598,10 @@ void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev
 }
 
 uint32_t ransac_scratch_words(uint32_t n) {
-    // pairs whose bitsets fit in LDS need no global scratch
-    return words(n) <= kLdsWords ? 1u : kMaskSets * words(n);
+    // LO hand-off records, then the bitsets unless they fit in LDS; a
+    // multiple of 64 words keeps every pair's scratch 256-byte aligned
+    const uint32_t w = kHandWords + (words(n) <= kLdsWords ? 0u : kMaskSets * words(n));
+    return (w + 63u) & ~63u;
 }
 
 }  // namespace skh

@@ -4,9 +4,10 @@
 // crates/core/src/two_view.rs:605-642. Every routine performs the same
 // floating-point operations in the same order as the host restatement, with
 // no FMA contraction, so a lane's F, counts and masks are bit-identical to
-// it. The eigen/SVD solvers are the project's own (cyclic Jacobi for the
-// symmetric 9x9 problem, one-sided Jacobi for the 3x3 rank-2 projection):
-// nalgebra's routines (the reference's) are not part of this build.
+// it. The eigen/SVD solvers are the project's own (Householder
+// tridiagonalisation and implicit QL for the symmetric 9x9 problem,
+// one-sided Jacobi for the 3x3 rank-2 projection): nalgebra's routines (the
+// reference's) are not part of this build.
 //
 // Point sets are read through a selector: all points, a bitset over the
 // n correspondences, or eight gathered samples, always in index order.
@@ -150,106 +151,175 @@ __device__ __forceinline__ bool inv3(const double* m, double* out) {
     return true;
 }
 
-// ---------------------------------------------------------- Jacobi eigen 9x9
+// ------------------------------------------------------- symmetric eigen 9x9
+//
+// Householder tridiagonalisation, then implicit QL with shifts, the same
+// operations in the same order as the host restatement. The rows of the
+// eigenvector matrix transform independently (every update multiplies it
+// from the right), so a solver group of three lanes splits them three per
+// lane while each lane reduces the same matrix: all state stays in
+// registers and every array index is a compile-time constant. Scheduling
+// barriers between the unrolled steps keep the scheduler from hoisting work
+// across them, which would multiply the live registers.
 
 // Symmetric matrix stored as its upper triangle, index of (r, c), r <= c.
 __host__ __device__ constexpr int ut(int r, int c) {
     return r <= c ? r * 9 - r * (r - 1) / 2 + (c - r) : c * 9 - c * (c - 1) / 2 + (r - c);
 }
 
-// Rotation (c, s, t) annihilating a[p][q]; false when none is due (a zero
-// element, or after the fourth sweep one negligible against both diagonal
-// entries, which the sweep then just sets to zero).
-__device__ __forceinline__ bool jacobi_params(double app, double aqq, double apq, int sweep,
-                                              double& c, double& s, double& t) {
-    c = 1.0;
-    s = 0.0;
-    t = 0.0;
-    if (apq == 0.0) return false;
-    double g = 100.0 * fabs(apq);
-    if (sweep > 3 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) return false;
-    double h = aqq - app;
-    double tt;
-    if (fabs(h) + g == fabs(h)) {
-        tt = apq / h;
-    } else {
-        double theta = 0.5 * h / apq;
-        tt = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
-        if (theta < 0.0) tt = -tt;
+// sqrt(a^2 + b^2) without undue overflow.
+__device__ __forceinline__ double pythag(double a, double b) {
+    const double aa = fabs(a), ab = fabs(b);
+    if (aa > ab) {
+        const double r = ab / aa;
+        return aa * sqrt(1.0 + r * r);
     }
-    double cc = 1.0 / sqrt(1.0 + tt * tt);
-    c = cc;
-    s = tt * cc;
-    t = tt;
-    return true;
+    if (ab == 0.0) return 0.0;
+    const double r = aa / ab;
+    return ab * sqrt(1.0 + r * r);
 }
 
-// Rotates the upper triangle `a` (45) by (c, s, t) in the (p, q) plane.
-template <int p, int q>
-__device__ __forceinline__ void jacobi_rotate_a(double* a, double c, double s, double t) {
-    const double apq = a[ut(p, q)];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        if (k == p || k == q) continue;
-        double akp = a[ut(k, p)], akq = a[ut(k, q)];
-        a[ut(k, p)] = c * akp - s * akq;
-        a[ut(k, q)] = s * akp + c * akq;
-    }
-    a[ut(p, p)] = a[ut(p, p)] - t * apq;
-    a[ut(q, q)] = a[ut(q, q)] + t * apq;
-}
-
+// Tridiagonalises `a` (upper triangle, 45, overwritten); q: rows r0..r0+R-1
+// of H_0 ... H_6; d: diagonal, e[i]: (i, i+1) coupling, e[8] = 0.
 template <int R>
-__device__ __forceinline__ void jacobi_rotate_v(double (*v)[9], int p, int q, double c, double s) {
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        double vkp = v[k][p], vkq = v[k][q];
-        v[k][p] = c * vkp - s * vkq;
-        v[k][q] = s * vkp + c * vkq;
-    }
-}
-
-// Cyclic Jacobi on the upper triangle `a` (45), accumulating R rows of the
-// eigenvector matrix starting at row r0 (v[i] = row r0 + i; rows past 8 are
-// padding). Same rotation order and arithmetic as the host's cyclic Jacobi;
-// rows of the eigenvector matrix evolve independently, so a pair of lanes
-// can split them (R = 5) while both rotate `a`, which keeps a and v in
-// registers.
-template <int R>
-__device__ __forceinline__ void sym_eigen9_rows(double* a, int r0, double (*v)[9]) {
+__device__ __forceinline__ void tridiag9_rows(double* a, int r0, double (*q)[9], double* d,
+                                              double* e) {
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 9; ++j) v[i][j] = (r0 + i == j) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 50; ++sweep) {
-        double off = 0.0;
+        for (int j = 0; j < 9; ++j) q[i][j] = (r0 + i == j) ? 1.0 : 0.0;
+    sfor<0, 7>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        double xn2 = 0.0;
+        sfor<k + 2, 9>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            xn2 = xn2 + a[ut(i, k)] * a[ut(i, k)];
+        });
+        const double x0 = a[ut(k + 1, k)];
+        if (xn2 != 0.0) {
+            const double nrm = sqrt(x0 * x0 + xn2);
+            const double beta = x0 >= 0.0 ? -nrm : nrm;
+            const double tau = (beta - x0) / beta;
+            const double rs = 1.0 / (x0 - beta);
+            double v[9], p[9], w[9];
+            v[k + 1] = 1.0;
+            sfor<k + 2, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                v[i] = a[ut(i, k)] * rs;
+            });
+            sfor<k + 1, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                double sum = 0.0;
+                sfor<k + 1, 9>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    sum = sum + a[ut(i, j)] * v[j];
+                });
+                p[i] = tau * sum;
+            });
+            double vp = 0.0;
+            sfor<k + 1, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                vp = vp + v[i] * p[i];
+            });
+            const double kk = 0.5 * tau * vp;
+            sfor<k + 1, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                w[i] = p[i] - kk * v[i];
+            });
+            sfor<k + 1, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                sfor<i, 9>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    a[ut(i, j)] = a[ut(i, j)] - (v[i] * w[j] + w[i] * v[j]);
+                });
+            });
+            a[ut(k, k + 1)] = beta;
+            sfor<k + 2, 9>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                a[ut(k, i)] = 0.0;
+            });
 #pragma unroll
-        for (int p = 0; p < 8; ++p)
+            for (int r = 0; r < R; ++r) {
+                double sum = 0.0;
+                sfor<k + 1, 9>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    sum = sum + q[r][j] * v[j];
+                });
+                const double t = tau * sum;
+                sfor<k + 1, 9>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    q[r][j] = q[r][j] - t * v[j];
+                });
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
 #pragma unroll
-            for (int q = p + 1; q < 9; ++q) off = off + fabs(a[ut(p, q)]);
-        if (off == 0.0) break;
-#define SKH_ROT(P, Q)                                                                 \
-    {                                                                                 \
-        double c, s, t;                                                               \
-        if (jacobi_params(a[ut(P, P)], a[ut(Q, Q)], a[ut(P, Q)], sweep, c, s, t)) {   \
-            jacobi_rotate_a<P, Q>(a, c, s, t);                                        \
-            jacobi_rotate_v<R>(v, P, Q, c, s);                                        \
-        }                                                                             \
-        a[ut(P, Q)] = 0.0;                                                            \
-    }
-        SKH_ROT(0, 1) SKH_ROT(0, 2) SKH_ROT(0, 3) SKH_ROT(0, 4) SKH_ROT(0, 5) SKH_ROT(0, 6)
-        SKH_ROT(0, 7) SKH_ROT(0, 8) SKH_ROT(1, 2) SKH_ROT(1, 3) SKH_ROT(1, 4) SKH_ROT(1, 5)
-        SKH_ROT(1, 6) SKH_ROT(1, 7) SKH_ROT(1, 8) SKH_ROT(2, 3) SKH_ROT(2, 4) SKH_ROT(2, 5)
-        SKH_ROT(2, 6) SKH_ROT(2, 7) SKH_ROT(2, 8) SKH_ROT(3, 4) SKH_ROT(3, 5) SKH_ROT(3, 6)
-        SKH_ROT(3, 7) SKH_ROT(3, 8) SKH_ROT(4, 5) SKH_ROT(4, 6) SKH_ROT(4, 7) SKH_ROT(4, 8)
-        SKH_ROT(5, 6) SKH_ROT(5, 7) SKH_ROT(5, 8) SKH_ROT(6, 7) SKH_ROT(6, 8) SKH_ROT(7, 8)
-#undef SKH_ROT
-    }
+    for (int i = 0; i < 9; ++i) d[i] = a[ut(i, i)];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) e[i] = a[ut(i, i + 1)];
+    e[8] = 0.0;
 }
 
-// One lane, all nine rows (v: 9x9 row-major).
-__device__ __forceinline__ void sym_eigen9(double* a, double* v) {
-    sym_eigen9_rows<9>(a, 0, reinterpret_cast<double (*)[9]>(v));
+// Implicit QL with shifts on (d, e) (tql2 scheme), rotating the R rows of
+// q; at most 30 sweeps per eigenvalue.
+template <int R>
+__device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) {
+    sfor<0, 9>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        for (int iter = 0; iter < 30; ++iter) {
+            // first m >= l with a negligible e[m] (8 if none)
+            int m = 8;
+            sfor<l, 8>([&](auto mc) {
+                constexpr int mm = decltype(mc)::value;
+                const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (m == 8 && fabs(e[mm]) + dd == dd) m = mm;
+            });
+            if (m == l) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = pythag(g, 1.0);
+            double dm = d[8];
+#pragma unroll
+            for (int j = l; j < 8; ++j) dm = (j == m) ? d[j] : dm;
+            g = dm - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool underflow = false;
+            sfor<0, 8 - l>([&](auto ic) {
+                constexpr int i = 7 - decltype(ic)::value;
+                if (i < m && !underflow) {
+                    const double f = s * e[i], b = c * e[i];
+                    r = pythag(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] = d[i + 1] - p;
+#pragma unroll
+                        for (int j = l; j < 9; ++j) e[j] = (j == m) ? 0.0 : e[j];
+                        underflow = true;
+                    } else {
+                        s = f / r;
+                        c = g / r;
+                        g = d[i + 1] - p;
+                        r = (d[i] - g) * s + 2.0 * c * b;
+                        p = s * r;
+                        d[i + 1] = g + p;
+                        g = c * r - b;
+#pragma unroll
+                        for (int k = 0; k < R; ++k) {
+                            const double fz = q[k][i + 1];
+                            q[k][i + 1] = s * q[k][i] + c * fz;
+                            q[k][i] = c * q[k][i] - s * fz;
+                        }
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            if (underflow) continue;
+            d[l] = d[l] - p;
+            e[l] = g;
+#pragma unroll
+            for (int j = l; j < 9; ++j) e[j] = (j == m) ? 0.0 : e[j];
+        }
+    });
 }
 
 // --------------------------------------------------------------- 3x3 rank 2
@@ -463,6 +533,7 @@ __device__ bool collinear_and_normalizers(const S& s, uint32_t len, double* t1, 
         q[5] += by * by;
         q[6] = q[6] + norm2(ax, ay);
         q[7] = q[7] + norm2(bx, by);
+        __builtin_amdgcn_sched_barrier(0);
     }
     return collinear_finish(c1x, c1y, c2x, c2y, q, len, t1, t2);
 }
@@ -488,6 +559,7 @@ __device__ bool accumulate_8pt(const S& s, uint32_t len, double* t1, double* t2,
         for (int r = 0; r < 9; ++r)
 #pragma unroll
             for (int c = r; c < 9; ++c) ata[ut(r, c)] = ata[ut(r, c)] + row[r] * row[c];
+        __builtin_amdgcn_sched_barrier(0);
     }
     return true;
 }
@@ -500,24 +572,23 @@ constexpr int kGroups = 21;
 __device__ __forceinline__ int solver_group() { return (int)threadIdx.x / 3; }
 __device__ __forceinline__ int solver_part() { return (int)threadIdx.x - 3 * ((int)threadIdx.x / 3); }
 
-// Eigen-solve of the upper triangle `a` (45, overwritten with the final
-// rotated matrix) by a solver group; on return every lane of the group holds
-// the eigenvector of the smallest diagonal entry (first minimum) in vk.
-// All three lanes of the group must be active with the same `a`.
-__device__ __forceinline__ int eigen_min_group(double* a, double* vk) {
+// Eigen-solve of the upper triangle `a` (45, overwritten) by a solver group:
+// d receives the nine eigenvalues and every lane of the group the
+// eigenvector of the smallest one (first minimum) in vk. All three lanes of
+// the group must be active with the same `a`.
+__device__ __forceinline__ int eigen_min_group(double* a, double* d, double* vk) {
     const int part = solver_part();
     const int base = 3 * solver_group();
-    double v[3][9];
-    sym_eigen9_rows<3>(a, 3 * part, v);
-    // first minimum of the diagonal (the value is tracked so that no array
-    // is indexed at run time)
+    double q[3][9], e[9];
+    tridiag9_rows<3>(a, 3 * part, q, d, e);
+    tql9_rows<3>(d, e, q);
+    // first minimum (the value is tracked so that no array is indexed at run time)
     int k = 0;
-    double dmin = a[ut(0, 0)];
+    double dmin = d[0];
 #pragma unroll
     for (int i = 1; i < 9; ++i) {
-        const double x = a[ut(i, i)];
-        if (x < dmin) {
-            dmin = x;
+        if (d[i] < dmin) {
+            dmin = d[i];
             k = i;
         }
     }
@@ -526,7 +597,7 @@ __device__ __forceinline__ int eigen_min_group(double* a, double* vk) {
     for (int i = 0; i < 3; ++i) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) acc = (j == k) ? v[i][j] : acc;
+        for (int j = 0; j < 9; ++j) acc = (j == k) ? q[i][j] : acc;
         mine[i] = acc;
     }
 #pragma unroll
@@ -540,13 +611,13 @@ __device__ __forceinline__ int eigen_min_group(double* a, double* vk) {
 // the eigen-solve, the degeneracy test, the rank-2 projection and the
 // denormalisation. ata is overwritten.
 __device__ bool solve_8pt_group(double* ata, const double* t1, const double* t2, double* F) {
-    double fn[9];
-    eigen_min_group(ata, fn);
+    double fn[9], ev[9];
+    eigen_min_group(ata, ev, fn);
     // second smallest and largest of the nine eigenvalues
     double e1 = INFINITY, e0 = INFINITY, e8 = -INFINITY;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        double x = ata[ut(i, i)];
+        double x = ev[i];
         if (x < e0) {
             e1 = e0;
             e0 = x;

@@ -54,7 +54,7 @@ def test_parallax_needed_values():
         assert ob.binomial_tail(n, k - 1, 0.001) >= 1e-6
 
 
-def test_jacobi_eigen_against_lapack():
+def test_sym_eigen_against_lapack():
     rng = np.random.default_rng(2)
     for trial in range(40):
         X = rng.standard_normal((9 if trial % 2 else 12, 9)) * rng.uniform(0.1, 10, 9)
