@@ -1560,10 +1560,48 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
     flush_prof();
 }
 
+// Cost estimate of each pair for the launch order: the first hypothesis
+// batch of the pair (the same samples and solves as the real run), its best
+// raw inlier ratio w, and est = adaptive_iterations(w) * (64 + n) / 64, the
+// hypotheses the run will draw times their cost in 64-point chunks. Pairs
+// with a low inlier ratio run the full iteration budget whatever their size,
+// so this orders the launch far better than the size alone.
+__global__ __launch_bounds__(NL) __attribute__((amdgpu_waves_per_eu(SKH_RANSAC_WAVES_PER_EU)))
+void ransac_probe_kernel(const RansacTask* __restrict__ tasks, const RansacCfgDev cfg,
+                         uint32_t* __restrict__ est) {
+    RansacTask t = tasks[blockIdx.x];
+    if (t.n_dev) t.n = min(*t.n_dev, t.n);
+    const int lane = threadIdx.x;
+    const uint32_t n = t.n;
+    if (n < 8) {
+        if (lane == 0) est[blockIdx.x] = 0;
+        return;
+    }
+    const double th2 = cfg.threshold_px * cfg.threshold_px;
+    const LcgLane jump = lcg_lane_jump();
+    LcgStream smp{cfg.seed ^ 0x9E3779B97F4A7C15ull, 0u, (uint32_t)NL};
+    lcg_samples<8>(smp, jump, n, kHypPerBatch, &s_pm.hyp.idx[0][0], 8);
+    __syncthreads();
+    for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, s_pm.hyp, round);
+    __syncthreads();
+    int best = lane < kHypPerBatch ? s_pm.hyp.hCnt[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < NL; off <<= 1) {
+        const int o = __shfl_xor(best, off);
+        best = best > o ? best : o;
+    }
+    if (lane == 0) {
+        const double w = best > 0 ? (double)best / (double)n : 0.0;
+        const uint64_t it = ipow_adaptive(w, 8, cfg.confidence, cfg.max_iters);
+        const double c = (double)it * (64.0 + (double)n) / 64.0;
+        est[blockIdx.x] = c < 4.0e9 ? (uint32_t)c : 4000000000u;
+    }
+}
+
 }  // namespace
 
-// Pair p's rank when the pairs are ordered by descending correspondence
-// count (ties by index): order[rank] = p. Larger problems start first, so
+// Pair p's rank when the pairs are ordered by descending cost estimate
+// (ties by index): order[rank] = p. The costliest problems start first, so
 // the longest workgroups do not trail at the end of the launch.
 __global__ __launch_bounds__(256) void ransac_order_kernel(const uint32_t* __restrict__ count,
                                                            uint32_t P, uint32_t* __restrict__ order) {
@@ -1586,13 +1624,15 @@ __global__ __launch_bounds__(256) void ransac_order_kernel(const uint32_t* __res
 }
 
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
-                   hipStream_t s, const uint32_t* count, uint32_t* order) {
+                   hipStream_t s, uint32_t* est, uint32_t* order) {
     if (!n_tasks) return;
-    if (count && order)
+    if (est && order) {
+        hipLaunchKernelGGL(ransac_probe_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg, est);
         hipLaunchKernelGGL(ransac_order_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, s,
-                           count, n_tasks, order);
-    else
+                           (const uint32_t*)est, n_tasks, order);
+    } else {
         order = nullptr;
+    }
     hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks,
                        (const uint32_t*)order, cfg);
 }

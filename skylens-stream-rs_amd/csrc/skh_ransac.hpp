@@ -31,10 +31,10 @@ struct RansacTask {
 };
 
 uint32_t ransac_scratch_words(uint32_t n);
-// One workgroup per task. With `count` (device, n_tasks correspondence
-// counts) and `order` (device, n_tasks words of scratch), the workgroups take
-// the tasks in descending count order.
+// One workgroup per task. With `est` and `order` (device, n_tasks words of
+// scratch each), a probe of every task's first hypothesis batch estimates its
+// cost and the workgroups take the tasks costliest first.
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
-                   hipStream_t s, const uint32_t* count = nullptr, uint32_t* order = nullptr);
+                   hipStream_t s, uint32_t* est = nullptr, uint32_t* order = nullptr);
 
 }  // namespace skh
