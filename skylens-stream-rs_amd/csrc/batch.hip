@@ -64,7 +64,7 @@ struct Plan {
     DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
         d_kp_b;
     DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters, d_prof, d_order,
-        d_est;
+        d_est, d_rqueue;
 
     hipGraph_t g_match = nullptr, g_ransac = nullptr;
     hipGraphExec_t x_match = nullptr, x_ransac = nullptr;
@@ -80,7 +80,7 @@ struct Plan {
                 &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len,
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
-                &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est};
+                &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue};
     }
 
     ~Plan() {
@@ -241,7 +241,8 @@ int Plan::build() {
             (rc = alloc(d_scratch, 4ull * sbase[P], "scratch")) ||
             (rc = alloc(d_iters, 8ull * P, "iters")) ||
             (rc = alloc(d_prof, 8ull * 11 * P, "prof")) ||
-            (rc = alloc(d_order, 4ull * P, "order")) || (rc = alloc(d_est, 4ull * P, "est")))
+            (rc = alloc(d_order, 4ull * P, "order")) || (rc = alloc(d_est, 4ull * P, "est")) ||
+            (rc = alloc(d_rqueue, 8, "rqueue")))
             return rc;
         for (uint32_t p = 0; p < P; ++p) {
             RansacTask& t = rt[p];
@@ -345,8 +346,8 @@ void Plan::enqueue_ransac(hipStream_t s) {
                        d_kp_a.as<uint64_t>(), d_kp_b.as<uint64_t>(),
                        reinterpret_cast<const float2*>(desc.d_kp), desc.px_offset,
                        d_x1.as<double2>(), d_x2.as<double2>());
-    launch_ransac(d_rtasks.as<RansacTask>(), P, desc.ransac, s, d_est.as<uint32_t>(),
-                  d_order.as<uint32_t>());
+    launch_ransac(d_rtasks.as<RansacTask>(), P, desc.ransac, s, d_rqueue.as<uint32_t>(),
+                  d_est.as<uint32_t>(), d_order.as<uint32_t>());
 }
 
 }  // namespace skh

@@ -23,12 +23,27 @@
 #include "ransac_dev.hpp"
 #include "skh_ransac.hpp"
 
+#include <algorithm>
+
 namespace skh {
 namespace {
 
 using namespace rdev;
 
-constexpr int NL = 64;  // lanes per pair
+constexpr int NL = 64;     // lanes per wave
+constexpr int kWaves = 4;  // waves per workgroup: four small pairs, or one large pair
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
+
+// Orders this wave's LDS and device-memory accesses before the following
+// ones: the lanes of a wave exchange data through LDS and device scratch,
+// and the waves of a workgroup may work on different pairs.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __constant__ double kNM[6] = {3.0, 2.0, 1.5, 1.0, 1.0, 1.0};
 
 __host__ __device__ __forceinline__ uint32_t words(uint32_t n) { return (n + 31) / 32; }
@@ -160,10 +175,12 @@ constexpr int kStageW = 10;  // doubles staged per point
 struct Coop {
     double stage[NL][kStageW];
 };
-__shared__ Coop s_cp;
+__shared__ Coop s_cp[kWaves];
+
+__device__ __forceinline__ Coop& coop() { return s_cp[wave_id()]; }
 
 __device__ __forceinline__ uint64_t lanes_below() {
-    return (1ull << threadIdx.x) - 1ull;
+    return (1ull << lane_id()) - 1ull;
 }
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
@@ -181,12 +198,12 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // the rejection rule then runs on the scalar unit over the draws in order.
 
 struct LcgLane {
-    uint64_t mul, add;  // jump of threadIdx.x + 1 steps
+    uint64_t mul, add;  // jump of lane + 1 steps
 };
 
 __device__ __forceinline__ LcgLane lcg_lane_jump() {
     uint64_t m = 1, a = 0;
-    for (int j = 0; j <= (int)threadIdx.x; ++j) {
+    for (int j = 0; j <= lane_id(); ++j) {
         m = m * 6364136223846793005ull;
         a = a * 6364136223846793005ull + 1442695040888963407ull;
     }
@@ -232,7 +249,7 @@ __device__ void lcg_samples(LcgStream& s, const LcgLane& j, uint32_t m, int nh, 
                 ++k;
             }
         }
-        if (threadIdx.x == 0) {
+        if (lane_id() == 0) {
 #pragma unroll
             for (int e = 0; e < PER; ++e) out[h * stride + e] = v[e];
         }
@@ -254,11 +271,11 @@ __device__ __forceinline__ double stage_sum(int u, uint32_t cnt, double acc) {
     for (; j + 8 <= cnt; j += 8) {
         double v[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = s_cp.stage[j + e][u];
+        for (int e = 0; e < 8; ++e) v[e] = coop().stage[j + e][u];
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc = acc + v[e];
     }
-    for (; j < cnt; ++j) acc = acc + s_cp.stage[j][u];
+    for (; j < cnt; ++j) acc = acc + coop().stage[j][u];
     return acc;
 }
 
@@ -269,11 +286,11 @@ __device__ __forceinline__ double stage_dot(int u, int w, uint32_t cnt,
     for (; j + 8 <= cnt; j += 8) {
         double v[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = s_cp.stage[j + e][u] * s_cp.stage[j + e][w];
+        for (int e = 0; e < 8; ++e) v[e] = coop().stage[j + e][u] * coop().stage[j + e][w];
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc = acc + v[e];
     }
-    for (; j < cnt; ++j) acc = acc + s_cp.stage[j][u] * s_cp.stage[j][w];
+    for (; j < cnt; ++j) acc = acc + coop().stage[j][u] * coop().stage[j][w];
     return acc;
 }
 
@@ -293,7 +310,7 @@ __device__ __forceinline__ void ut_rc(int k, int& r, int& c) {
 __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double2* x2, uint32_t n,
                                 double th2, uint32_t* at, double t2, uint32_t* sel,
                                 uint32_t* ns) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     uint32_t cnt = 0, cs = 0;
     for (uint32_t base = 0; base < n; base += NL) {
         const uint32_t i = base + lane;
@@ -312,7 +329,7 @@ __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double
         cnt += (uint32_t)__popcll(ma);
         cs += (uint32_t)__popcll(mb);
     }
-    __syncthreads();
+    wave_sync();
     if (ns) *ns = cs;
     return cnt;
 }
@@ -327,7 +344,7 @@ __device__ uint32_t coop_count_of(const double* F, const double2* x1, const doub
                                   uint32_t n, double t2) {
     uint32_t cnt = 0;
     for (uint32_t base = 0; base < n; base += NL) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t i = base + lane_id();
         const bool a = i < n && sampson_error(F, x1[i], x2[i]) < t2;
         cnt += (uint32_t)__popcll(__ballot(a));
     }
@@ -336,14 +353,14 @@ __device__ uint32_t coop_count_of(const double* F, const double2* x1, const doub
 
 // rank -> index of the points of an n-bit bitset, in index order.
 __device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     uint32_t done = 0;
     for (uint32_t base = 0; base < n; base += NL) {
         const uint64_t m = bits64(bits, base, n);
         if ((m >> lane) & 1u) list[done + (uint32_t)__popcll(m & lanes_below())] = base + lane;
         done += (uint32_t)__popcll(m);
     }
-    __syncthreads();
+    wave_sync();
 }
 
 // collinear_and_normalizers over the bitset `sel` (len points): fills the
@@ -351,21 +368,21 @@ __device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
 // collinear.
 __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* list,
                           uint32_t len, double* t1, double* t2) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     double acc = 0.0;
     for (uint32_t base = 0; base < len; base += NL) {
         const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
         if ((uint32_t)lane < cnt) {
             const uint32_t slot = lane, pi = list[base + lane];
             const double2 a = x1[pi], b = x2[pi];
-            s_cp.stage[slot][0] = a.x;
-            s_cp.stage[slot][1] = a.y;
-            s_cp.stage[slot][2] = b.x;
-            s_cp.stage[slot][3] = b.y;
+            coop().stage[slot][0] = a.x;
+            coop().stage[slot][1] = a.y;
+            coop().stage[slot][2] = b.x;
+            coop().stage[slot][3] = b.y;
         }
-        __syncthreads();
+        wave_sync();
         acc = stage_sum(lane & 3, cnt, acc);
-        __syncthreads();
+        wave_sync();
     }
     const double nc = (double)(len > 0 ? len : 1);
     const double c1x = readlane_d(acc, 0) / nc, c1y = readlane_d(acc, 1) / nc;
@@ -377,18 +394,18 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
             const uint32_t slot = lane, pi = list[base + lane];
             const double2 a = x1[pi], b = x2[pi];
             const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
-            s_cp.stage[slot][0] = ax * ax;
-            s_cp.stage[slot][1] = ax * ay;
-            s_cp.stage[slot][2] = ay * ay;
-            s_cp.stage[slot][3] = bx * bx;
-            s_cp.stage[slot][4] = bx * by;
-            s_cp.stage[slot][5] = by * by;
-            s_cp.stage[slot][6] = norm2(ax, ay);
-            s_cp.stage[slot][7] = norm2(bx, by);
+            coop().stage[slot][0] = ax * ax;
+            coop().stage[slot][1] = ax * ay;
+            coop().stage[slot][2] = ay * ay;
+            coop().stage[slot][3] = bx * bx;
+            coop().stage[slot][4] = bx * by;
+            coop().stage[slot][5] = by * by;
+            coop().stage[slot][6] = norm2(ax, ay);
+            coop().stage[slot][7] = norm2(bx, by);
         }
-        __syncthreads();
+        wave_sync();
         acc = stage_sum(lane & 7, cnt, acc);
-        __syncthreads();
+        wave_sync();
     }
     double q[8];
 #pragma unroll
@@ -400,7 +417,7 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
 // lane k < 45 returns upper-triangle entry k.
 __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t* list,
                            uint32_t len, const double* t1, const double* t2) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     int r, c;
     ut_rc(lane < 45 ? lane : 0, r, c);
     double acc = 0.0;
@@ -412,7 +429,7 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
-            double* row = s_cp.stage[slot];
+            double* row = coop().stage[slot];
             row[0] = b[0] * a[0];
             row[1] = b[0] * a[1];
             row[2] = b[0];
@@ -423,9 +440,9 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
             row[7] = a[1];
             row[8] = 1.0;
         }
-        __syncthreads();
+        wave_sync();
         acc = stage_dot(r, c, cnt, acc);
-        __syncthreads();
+        wave_sync();
     }
     return acc;
 }
@@ -437,7 +454,7 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
 __device__ double coop_lm_cost(const double* g, double s1, double s2, const double* t1,
                                const double* t2, const double2* x1, const double2* x2,
                                const uint32_t* list, uint32_t len) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const double q1 = s1 * s1, q2 = s2 * s2;
     double cost = 0.0;
     for (uint32_t base = 0; base < len; base += NL) {
@@ -458,11 +475,11 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
                 e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
             else
                 e = 0.0;
-            s_cp.stage[slot][0] = e * e;
+            coop().stage[slot][0] = e * e;
         }
-        __syncthreads();
+        wave_sync();
         cost = stage_sum(0, cnt, cost);
-        __syncthreads();
+        wave_sync();
     }
     return cost;
 }
@@ -472,7 +489,7 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
 __device__ double coop_normal_eq(const double* g, double s1, double s2,
                                  const double* t1, const double* t2, const double2* x1,
                                  const double2* x2, const uint32_t* list, uint32_t len) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const double q1 = s1 * s1, q2 = s2 * s2;
     int u = 0, w = 0;
     if (lane < 45)
@@ -496,7 +513,7 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
             double e = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
             double den =
                 q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-            double* row = s_cp.stage[slot];
+            double* row = coop().stage[slot];
             if (den <= 0.0) {
 #pragma unroll
                 for (int k = 0; k < 10; ++k) row[k] = 0.0;
@@ -513,9 +530,9 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
                 row[9] = e * inv;
             }
         }
-        __syncthreads();
+        wave_sync();
         acc = stage_dot(u, w, cnt, acc);
-        __syncthreads();
+        wave_sync();
     }
     return acc;
 }
@@ -685,7 +702,7 @@ __constant__ int kDltR2[9] = {3, 3, 3, 0, 1, 2, 7, 8, 9};
 __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32_t n,
                                     const uint32_t* sel, uint32_t len, double* H,
                                     uint32_t* list) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if (len < 4) return false;
     coop_list(sel, n, list);
     double t1[9], t2[9];
@@ -705,7 +722,7 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
             double q3[3] = {qq.x, qq.y, 1.0};
             mv3(t2, q3, v);
             const double u = v[0] / v[2], w = v[1] / v[2];
-            double* row = s_cp.stage[slot];
+            double* row = coop().stage[slot];
             row[0] = -x;
             row[1] = -y;
             row[2] = -1.0;
@@ -717,12 +734,12 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
             row[8] = w * y;
             row[9] = w;
         }
-        __syncthreads();
+        wave_sync();
         for (uint32_t j = 0; j < cnt; ++j) {
-            const double* row = s_cp.stage[j];
+            const double* row = coop().stage[j];
             acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
         }
-        __syncthreads();
+        wave_sync();
     }
     double ata[45];
 #pragma unroll
@@ -741,7 +758,7 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
 __device__ uint32_t coop_count_h(const double* h, const double* hi, const double2* x1,
                                  const double2* x2, uint32_t n, const uint32_t* sel, double th,
                                  uint32_t* out) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     uint32_t cnt = 0;
     for (uint32_t base = 0; base < n; base += NL) {
         const uint64_t m = bits64(sel, base, n);
@@ -752,7 +769,7 @@ __device__ uint32_t coop_count_h(const double* h, const double* hi, const double
             out[(base >> 5) + lane] = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
         cnt += (uint32_t)__popcll(b);
     }
-    if (out) __syncthreads();
+    if (out) wave_sync();
     return cnt;
 }
 
@@ -805,7 +822,7 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
                                             uint32_t ns, double th, double* H, double* Hi,
                                             uint32_t* cnt_out, SelState& sh, uint32_t* selbits,
                                             uint32_t* list_buf) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if (ns < 4) return false;
     if (lane == 0) {
         sh.have = 0;
@@ -813,15 +830,15 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
         sh.iters = 500;
         sh.it = 0;
     }
-    __syncthreads();
+    wave_sync();
     const LcgLane jump = lcg_lane_jump();
     LcgStream hs{0x2545F4914F6CDD1Dull, 0u, (uint32_t)NL};
     while (sh.it < sh.iters) {
         // matching.rs:314-320: four distinct ranks among the inliers
         lcg_samples<4>(hs, jump, ns, kHypPerBatch, &sh.idx[0][0], 4);
-        __syncthreads();
+        wave_sync();
         for (int round = 0; round < 3; ++round) hom_round(x1, x2, list, ns, th, sh, round);
-        __syncthreads();
+        wave_sync();
         if (lane == 0) {
             for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
                 sh.it++;
@@ -836,7 +853,7 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
     if (!sh.have) return false;
     double bh[9], bhi[9], ch[9], chi[9];
@@ -892,24 +909,17 @@ union PhaseMem {
     SelState hom;
     uint32_t draws[600];
 };
-__shared__ PhaseMem s_pm;
+__shared__ PhaseMem s_pm[kWaves];
 
-// Occupancy: two waves (pairs) per SIMD hide the FP64 dependency latency of
-// the serial solver chains better than LDS-resident point data does, so by
-// default correspondences and bitsets stay in L2-resident device memory
-// (LDS 17 KB per workgroup) and every function fits 256 VGPRs. A build with
-// SKH_RANSAC_LDS_PTS=N stages pairs of up to N correspondences in LDS.
-#ifndef SKH_RANSAC_LDS_PTS
-#define SKH_RANSAC_LDS_PTS 0
-#endif
+// Occupancy: two waves per SIMD hide the FP64 dependency latency of the
+// serial solver chains; correspondences and bitsets stay in L2-resident
+// device memory and every function fits 256 VGPRs.
 #ifndef SKH_RANSAC_WAVES_PER_EU
 #define SKH_RANSAC_WAVES_PER_EU 2
 #endif
-constexpr uint32_t kLdsPts = SKH_RANSAC_LDS_PTS;
-constexpr uint32_t kLdsWords = (kLdsPts + 31) / 32;
 // bitsets of W words: lo, alt, sel per hypothesis, pool (6), final (2), then
-// an index list of up to 32 W >= n words
-constexpr uint32_t kMaskSets = 3 * NL + 8 + 32;
+// one index list of up to 32 W >= n words per wave
+constexpr uint32_t kMaskSets = 3 * NL + 8 + 32 * kWaves;
 
 struct Shared {
     double poolF[6][9];
@@ -924,6 +934,7 @@ struct Shared {
     double f[9];
     int inlCnt;
     int result;
+    uint64_t cmask;
     int lineBest[NL];
     double es_med;
     int es_n;
@@ -994,7 +1005,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
     const double2* x1 = t.x1;
     const double2* x2 = t.x2;
     const uint32_t n = t.n;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     // stage 0 selection (x3) and the x1 mask of the hypothesis itself
     for (uint64_t m = cand; m; m &= m - 1) {
         const int h = __ffsll((unsigned long long)m) - 1;
@@ -1008,7 +1019,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
             hm.loNs[h] = ns;
         }
     }
-    __syncthreads();
+    wave_sync();
     uint64_t active = cand, swapped = 0;
     for (int r = 0; r < 6 && active; ++r) {
         const double t2n = r < 5 ? th2 * kNM[r + 1] * kNM[r + 1] : 0.0;
@@ -1046,9 +1057,9 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                 filled |= 1ull << h;
                 ++rank;
             }
-            __syncthreads();
+            wave_sync();
             if (solver_group() < rank) lo_group_solve(hand + (size_t)solver_group() * kHandStride);
-            __syncthreads();
+            wave_sync();
             rank = 0;
             for (uint64_t m = filled; m; m &= m - 1, ++rank) {
                 const int h = __ffsll((unsigned long long)m) - 1;
@@ -1066,7 +1077,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                 const uint32_t gc = coop_masks2(gh, x1, x2, n, th2, spare, t2n,
                                                 r < 5 ? selb + (size_t)h * W : nullptr, &ns);
                 const int cur = hm.loCnt[h];
-                __syncthreads();
+                wave_sync();
                 if (lane == 0) hm.loNs[h] = ns;
                 if ((int)gc > cur) {
                     if (lane == 0) {
@@ -1076,7 +1087,7 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                     }
                     swapped ^= 1ull << h;
                 }
-                __syncthreads();
+                wave_sync();
             }
         }
     }
@@ -1085,41 +1096,44 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
         const int h = __ffsll((unsigned long long)m) - 1;
         for (uint32_t w0 = lane; w0 < W; w0 += NL) lo[(size_t)h * W + w0] = alt[(size_t)h * W + w0];
     }
-    __syncthreads();
+    wave_sync();
 }
 
-__global__ __launch_bounds__(NL) __attribute__((amdgpu_waves_per_eu(SKH_RANSAC_WAVES_PER_EU)))
-void ransac_kernel(const RansacTask* __restrict__ tasks,
-                                                    const uint32_t* __restrict__ order,
-                                                    const RansacCfgDev cfg) {
-    __shared__ Shared sh;
-    PhaseMem& pm = s_pm;
-    __shared__ double2 lds_x1[kLdsPts > 0 ? kLdsPts : 1], lds_x2[kLdsPts > 0 ? kLdsPts : 1];
-    __shared__ uint32_t lds_masks[kLdsWords > 0 ? kMaskSets * kLdsWords : 1];
-    RansacTask t = tasks[order ? order[blockIdx.x] : blockIdx.x];
+__shared__ Shared s_sh[kWaves];
+__shared__ uint64_t s_pc[kWaves][11];
+
+// ransac_fundamental of one pair. Small mode (big = false): the calling wave
+// alone, with the LDS slot of its wave. Large mode (big = true): all waves of
+// the workgroup on one pair, with LDS slot 0: wave 0 draws the samples,
+// replays the batches and runs the final stages; the hypothesis rounds, the
+// local optimisations of a batch and the pool refits are spread over the
+// waves, each with its own staging LDS, index list and hand-off records.
+__device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) {
+    const int lane = lane_id(), wave = wave_id();
+    const int slot = big ? 0 : wave;
+    const bool lead = !big || wave == 0;
+    Shared& sh = s_sh[slot];
+    PhaseMem& pm = s_pm[slot];
+    uint64_t* pc = s_pc[slot];
+    auto pair_sync = [&]() {
+        if (big)
+            __syncthreads();
+        else
+            wave_sync();
+    };
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
-    const int lane = threadIdx.x;
     const uint32_t n = t.n;
     const uint32_t W = words(n);
-    // Small pairs run from LDS (generic pointers, so every routine is shared).
-    if (n <= kLdsPts) {
-        for (uint32_t i = lane; i < n; i += NL) {
-            lds_x1[i] = t.x1[i];
-            lds_x2[i] = t.x2[i];
-        }
-        t.x1 = lds_x1;
-        t.x2 = lds_x2;
-    }
-    double* hand = reinterpret_cast<double*>(t.scratch);  // kHandWords, 256-byte aligned
-    uint32_t* masks = W <= kLdsWords && kLdsWords > 0 ? lds_masks : t.scratch + kHandWords;
+    // device scratch: kWaves hand-off record sets, the bitsets, kWaves index lists
+    double* hand = reinterpret_cast<double*>(t.scratch + (size_t)wave * kHandWords);
+    uint32_t* masks = t.scratch + (size_t)kWaves * kHandWords;
     uint32_t* lo_masks = masks;                                 // NL * W
     uint32_t* alt_masks = masks + (size_t)NL * W;               // NL * W
     uint32_t* sel_masks = masks + (size_t)2 * NL * W;           // NL * W
     uint32_t* pool_masks = masks + (size_t)3 * NL * W;          // 6 * W
     uint32_t* fin_mask = masks + (size_t)(3 * NL + 6) * W;      // W
     uint32_t* fin_sel = fin_mask + W;                           // W
-    uint32_t* list_buf = fin_sel + W;                           // 32 * W >= n
-    __syncthreads();
+    uint32_t* list_buf = fin_sel + W + (size_t)wave * 32 * W;   // 32 * W >= n per wave
 
     // matching.rs:946-949
     bool valid = n >= 8;
@@ -1127,15 +1141,15 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         double2 a = t.x1[i], b = t.x2[i];
         if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(b.x) || !isfinite(b.y)) valid = false;
     }
-    valid = __syncthreads_and(valid);
+    valid = __all(valid);  // every wave scans every point: the same verdict
     if (!valid) {
-        if (lane == 0) *t.found = 0;
+        if (lead && lane == 0) *t.found = 0;
         return;
     }
     const double th2 = cfg.threshold_px * cfg.threshold_px;
     const LcgLane jump = lcg_lane_jump();
     LcgStream smp{cfg.seed ^ 0x9E3779B97F4A7C15ull, 0u, (uint32_t)NL};
-    if (lane == 0) {
+    if (lead && lane == 0) {
         sh.npool = 0;
         sh.bestCnt = 0;
         sh.haveBest = 0;
@@ -1144,22 +1158,21 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         sh.result = 0;
         for (int k = 0; k < 6; ++k) sh.poolSlot[k] = k;
     }
-    __syncthreads();
+    if (lead && lane < 11) pc[lane] = 0;
+    pair_sync();
 
     // Phase clock (lane 0, constant-rate wall clock): sampling, hypotheses,
     // local optimisation, replay, pool LM, final refits + gates, line test,
     // model selection, batches, LO candidates, total.
-    __shared__ uint64_t pc[11];
-    if (lane < 11) pc[lane] = 0;
     const uint64_t t_start = wall_clock64();
     uint64_t t_prev = t_start;
     auto lap = [&](int k) {
         const uint64_t now = wall_clock64();
-        if (lane == 0) pc[k] += now - t_prev;
+        if (lead && lane == 0) pc[k] += now - t_prev;
         t_prev = now;
     };
     auto flush_prof = [&]() {
-        if (lane == 0 && t.prof) {
+        if (lead && lane == 0 && t.prof) {
             pc[10] = wall_clock64() - t_start;
             for (int k = 0; k < 11; ++k) t.prof[k] = pc[k];
         }
@@ -1168,36 +1181,57 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
     // ---------------------------------------------------- hypothesis batches
     while (sh.it < sh.iters) {
         lap(3);
-        if (lane == 0) pc[8] += 1;
-        // matching.rs:971-979: eight distinct draws per hypothesis
-        lcg_samples<8>(smp, jump, n, kHypPerBatch, &pm.hyp.idx[0][0], 8);
-        __syncthreads();
-        lap(0);
-        if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
-        for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, pm.hyp, round);
-        __syncthreads();
-        lap(1);
-        const int cnt = pm.hyp.hCnt[lane];
-        // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
-        int v = cnt >= 8 ? cnt : 0;
-        int incl = v;
-#pragma unroll
-        for (int off = 1; off < NL; off <<= 1) {
-            int o = __shfl_up(incl, off);
-            if (lane >= off) incl = incl > o ? incl : o;
+        if (lead) {
+            if (lane == 0) pc[8] += 1;
+            // matching.rs:971-979: eight distinct draws per hypothesis
+            lcg_samples<8>(smp, jump, n, kHypPerBatch, &pm.hyp.idx[0][0], 8);
+            if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
         }
-        int excl = __shfl_up(incl, 1);
-        if (lane == 0) excl = 0;
-        const int bound = sh.bestCnt > excl ? sh.bestCnt : excl;
-        const bool cand = cnt >= 8 && 4 * cnt >= bound;
-        const uint64_t cmask = __ballot(cand);
-        if (lane == 0) pc[9] += __popcll(cmask);
-        if (!cand) pm.hyp.loCnt[lane] = -1;
-        __syncthreads();
-        local_opt_batch(t, pm.hyp, cmask, th2, lo_masks, alt_masks, sel_masks, W, list_buf, hand);
+        pair_sync();
+        lap(0);
+        if (!big) {
+            for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, pm.hyp, round);
+        } else if (wave < 3) {
+            hyp_round(t.x1, t.x2, n, th2, pm.hyp, wave);
+        }
+        pair_sync();
+        lap(1);
+        if (lead) {
+            const int cnt = pm.hyp.hCnt[lane];
+            // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
+            int v = cnt >= 8 ? cnt : 0;
+            int incl = v;
+#pragma unroll
+            for (int off = 1; off < NL; off <<= 1) {
+                int o = __shfl_up(incl, off);
+                if (lane >= off) incl = incl > o ? incl : o;
+            }
+            int excl = __shfl_up(incl, 1);
+            if (lane == 0) excl = 0;
+            const int bound = sh.bestCnt > excl ? sh.bestCnt : excl;
+            const bool cand = cnt >= 8 && 4 * cnt >= bound;
+            const uint64_t cm = __ballot(cand);
+            if (lane == 0) {
+                pc[9] += __popcll(cm);
+                sh.cmask = cm;
+            }
+            if (!cand) pm.hyp.loCnt[lane] = -1;
+        }
+        pair_sync();
+        // a large pair's waves take every kWaves-th candidate
+        uint64_t mine = sh.cmask;
+        if (big) {
+            uint64_t sub = 0;
+            int r = 0;
+            for (uint64_t m = mine; m; m &= m - 1, ++r)
+                if (r % kWaves == wave) sub |= m & (~m + 1);
+            mine = sub;
+        }
+        local_opt_batch(t, pm.hyp, mine, th2, lo_masks, alt_masks, sel_masks, W, list_buf, hand);
+        pair_sync();
         lap(2);
         // replay in order (lane 0)
-        if (lane == 0) {
+        if (lead && lane == 0) {
             for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
                 sh.it++;
                 const int c0 = pm.hyp.hCnt[h];
@@ -1263,27 +1297,28 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
                 sh.haveBest = 1;
             }
         }
-        __syncthreads();
+        pair_sync();
     }
     lap(3);
-    if (lane == 0 && t.iters_out) *t.iters_out = sh.it;
+    if (lead && lane == 0 && t.iters_out) *t.iters_out = sh.it;
     if (sh.result < 0) {
-        if (lane == 0) *t.found = -1;
+        if (lead && lane == 0) *t.found = -1;
         flush_prof();
         return;
     }
     if (!sh.haveBest) {
-        if (lane == 0) *t.found = 0;
+        if (lead && lane == 0) *t.found = 0;
         flush_prof();
         return;
     }
 
     // ------------------------------------------- pool LM (matching.rs:1041-1057)
     {
-        uint32_t* sel = sel_masks;
-        uint32_t* cnt_bits = alt_masks;
+        // a large pair's waves take every kWaves-th pool entry
         const int npool = sh.npool;
-        for (int p = 0; p < npool; ++p) {
+        for (int p = big ? wave : 0; p < npool; p += big ? kWaves : 1) {
+            uint32_t* sel = sel_masks + (size_t)p * W;
+            uint32_t* cnt_bits = alt_masks + (size_t)p * W;
             double cur[9];
             load9(sh.poolF[p], cur);
             uint32_t ns = coop_mask_of(cur, t.x1, t.x2, n, th2 * kNM[0] * kNM[0], sel);
@@ -1303,7 +1338,8 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             }
         }
     }
-    __syncthreads();
+    pair_sync();
+    if (!lead) return;  // the remaining stages run on the lead wave alone
     lap(4);
     {
         double f[9];
@@ -1372,7 +1408,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             sh.es_n = ic;
         }
     }
-    __syncthreads();
+    wave_sync();
     lap(5);
     if (!sh.result) {
         if (lane == 0) *t.found = 0;
@@ -1399,7 +1435,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
                 if (b + lane < 600) pm.draws[b + lane] = ls.raw;
             }
         }
-        __syncthreads();
+        wave_sync();
         int best = 0;
         for (int it = lane; it < 300; it += NL) {
             const uint32_t ri = pm.draws[2 * it], rj = pm.draws[2 * it + 1];
@@ -1420,13 +1456,13 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
             best = best > c ? best : c;
         }
         sh.lineBest[lane] = best;
-        __syncthreads();
+        wave_sync();
         if (lane == 0) {
             int b = 0;
             for (int l = 0; l < NL; ++l) b = b > sh.lineBest[l] ? b : sh.lineBest[l];
             sh.result = (int)(ns - (uint32_t)b) >= 8;
         }
-        __syncthreads();
+        wave_sync();
         if (!sh.result) {
             line_ok = false;
             break;
@@ -1450,7 +1486,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
         if (has) {
             // median of the finite Sampson errors over the inliers, rank nf / 2;
             // the errors are computed once into LDS (or L2-resident scratch)
-            double* err = n <= (uint32_t)(NL * kStageW) ? &s_cp.stage[0][0]
+            double* err = n <= (uint32_t)(NL * kStageW) ? &coop().stage[0][0]
                                                         : reinterpret_cast<double*>(lo_masks);
             double f[9];
             load9(sh.f, f);
@@ -1465,7 +1501,7 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
                 }
                 nfu += (uint32_t)__popcll(__ballot(fin));
             }
-            __syncthreads();
+            wave_sync();
             const int nf = (int)nfu;
             if (nf < 8) has = false;
             if (has) {
@@ -1491,14 +1527,14 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
                 }
                 sh.redA[lane] = found_v;
                 sh.lineBest[lane] = found_c;
-                __syncthreads();
+                wave_sync();
                 if (lane == 0) {
                     double m = 0.0;
                     for (int l = 0; l < NL; ++l)
                         if (sh.lineBest[l]) m = sh.redA[l];
                     sh.es_med = m;
                 }
-                __syncthreads();
+                wave_sync();
                 sigma_hat = sqrt(sh.es_med / 0.455);
             }
         }
@@ -1527,14 +1563,14 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
                         const double e1 = htr(H, p, a) ? (a.x - q.x) * (a.x - q.x) + (a.y - q.y) * (a.y - q.y) : INFINITY;
                         const double e2 = htr(Hi, q, b) ? (b.x - p.x) * (b.x - p.x) + (b.y - p.y) * (b.y - p.y) : INFINITY;
                         const double eh = 0.25 * (e1 + e2) / s2;
-                        s_cp.stage[slot][0] = fmin(ef, 2.0 * (4.0 - 3.0));
-                        s_cp.stage[slot][1] = fmin(eh, 2.0 * (4.0 - 2.0));
+                        coop().stage[slot][0] = fmin(ef, 2.0 * (4.0 - 3.0));
+                        coop().stage[slot][1] = fmin(eh, 2.0 * (4.0 - 2.0));
                         par = eh > 13.8 && ef < 10.8;
                     }
                     parallax += (uint64_t)__popcll(__ballot(par));
-                    __syncthreads();
+                    wave_sync();
                     acc = stage_sum(lane & 1, (uint32_t)__popcll(m), acc);
-                    __syncthreads();
+                    wave_sync();
                 }
                 const double rho_f = readlane_d(acc, 0), rho_h = readlane_d(acc, 1);
                 const double nf = (double)ns;
@@ -1560,6 +1596,39 @@ void ransac_kernel(const RansacTask* __restrict__ tasks,
     flush_prof();
 }
 
+// Persistent workgroups over a work queue of pairs in `order` (costliest
+// first): the first n_big pairs are large (a whole workgroup each, taken by
+// workgroup from queue[0]), the rest small (a wave each, taken by wave from
+// queue[1]). Every loop ends when its counter passes its range.
+__global__ __launch_bounds__(kWaves * NL) __attribute__((amdgpu_waves_per_eu(SKH_RANSAC_WAVES_PER_EU)))
+void ransac_kernel(const RansacTask* __restrict__ tasks, const uint32_t* __restrict__ order,
+                   uint32_t n_tasks, uint32_t n_big, const RansacCfgDev cfg,
+                   uint32_t* __restrict__ queue) {
+    __shared__ uint32_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= n_big) break;
+        run_pair(tasks[order ? order[it] : it], cfg, true);
+        __syncthreads();
+    }
+    for (;;) {
+        uint32_t it = 0;
+        if (lane_id() == 0) it = atomicAdd(&queue[1], 1u);
+        it = (uint32_t)__shfl((int)it, 0);
+        if (it >= n_tasks - n_big) break;
+        const uint32_t k = n_big + it;
+        run_pair(tasks[order ? order[k] : k], cfg, false);
+    }
+}
+
+__global__ void ransac_queue_reset_kernel(uint32_t* queue) {
+    queue[0] = 0;
+    queue[1] = 0;
+}
+
 // Cost estimate of each pair for the launch order: the first hypothesis
 // batch of the pair (the same samples and solves as the real run), its best
 // raw inlier ratio w, and est = adaptive_iterations(w) * (64 + n) / 64, the
@@ -1571,7 +1640,7 @@ void ransac_probe_kernel(const RansacTask* __restrict__ tasks, const RansacCfgDe
                          uint32_t* __restrict__ est) {
     RansacTask t = tasks[blockIdx.x];
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const uint32_t n = t.n;
     if (n < 8) {
         if (lane == 0) est[blockIdx.x] = 0;
@@ -1580,11 +1649,11 @@ void ransac_probe_kernel(const RansacTask* __restrict__ tasks, const RansacCfgDe
     const double th2 = cfg.threshold_px * cfg.threshold_px;
     const LcgLane jump = lcg_lane_jump();
     LcgStream smp{cfg.seed ^ 0x9E3779B97F4A7C15ull, 0u, (uint32_t)NL};
-    lcg_samples<8>(smp, jump, n, kHypPerBatch, &s_pm.hyp.idx[0][0], 8);
-    __syncthreads();
-    for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, s_pm.hyp, round);
-    __syncthreads();
-    int best = lane < kHypPerBatch ? s_pm.hyp.hCnt[lane] : 0;
+    lcg_samples<8>(smp, jump, n, kHypPerBatch, &s_pm[0].hyp.idx[0][0], 8);
+    wave_sync();
+    for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, s_pm[0].hyp, round);
+    wave_sync();
+    int best = lane < kHypPerBatch ? s_pm[0].hyp.hCnt[lane] : 0;
 #pragma unroll
     for (int off = 1; off < NL; off <<= 1) {
         const int o = __shfl_xor(best, off);
@@ -1624,7 +1693,7 @@ __global__ __launch_bounds__(256) void ransac_order_kernel(const uint32_t* __res
 }
 
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
-                   hipStream_t s, uint32_t* est, uint32_t* order) {
+                   hipStream_t s, uint32_t* queue, uint32_t* est, uint32_t* order) {
     if (!n_tasks) return;
     if (est && order) {
         hipLaunchKernelGGL(ransac_probe_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg, est);
@@ -1633,14 +1702,28 @@ void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev
     } else {
         order = nullptr;
     }
-    hipLaunchKernelGGL(ransac_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks,
-                       (const uint32_t*)order, cfg);
+    hipLaunchKernelGGL(ransac_queue_reset_kernel, dim3(1), dim3(1), 0, s, queue);
+    // the costliest 1/32 of the pairs (a single pair: itself) get a whole workgroup
+    const uint32_t n_big = order ? std::min(n_tasks, std::max(1u, n_tasks / 32)) : 1u;
+    // resident workgroups: two per CU (64 KB LDS and 256 VGPRs per lane each)
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    const uint32_t need = n_big + (n_tasks - n_big + kWaves - 1) / kWaves;
+    const uint32_t grid = std::min(need, 2u * (uint32_t)cus);
+    hipLaunchKernelGGL(ransac_kernel, dim3(grid), dim3(kWaves * NL), 0, s, tasks,
+                       (const uint32_t*)order, n_tasks, n_big, cfg, queue);
 }
 
 uint32_t ransac_scratch_words(uint32_t n) {
-    // LO hand-off records, then the bitsets unless they fit in LDS; a
-    // multiple of 64 words keeps every pair's scratch 256-byte aligned
-    const uint32_t w = kHandWords + (words(n) <= kLdsWords ? 0u : kMaskSets * words(n));
+    // kWaves sets of LO hand-off records, then the bitsets and kWaves index
+    // lists; a multiple of 64 words keeps every pair's scratch 256-byte aligned
+    const uint32_t w = kWaves * kHandWords + kMaskSets * words(n);
     return (w + 63u) & ~63u;
 }
 

@@ -50,7 +50,8 @@ extern "C" int skh_ransac_fundamental(skh_ctx* ctx, const double* x1, const doub
         (rc = ctx->ensure(ctx->r_x2, 16 * n, "r_x2")) ||
         (rc = ctx->ensure(ctx->r_out, sizeof(ROut) + n, "r_out")) ||
         (rc = ctx->ensure(ctx->r_scratch, 4ull * ransac_scratch_words(nn), "r_scratch")) ||
-        (rc = ctx->ensure(ctx->r_task, sizeof(RansacTask), "r_task")))
+        (rc = ctx->ensure(ctx->r_task, sizeof(RansacTask), "r_task")) ||
+        (rc = ctx->ensure(ctx->r_queue, 8, "r_queue")))
         return rc;
     ROut* o = ctx->r_out.as<ROut>();
     RansacTask t{};
@@ -71,7 +72,7 @@ extern "C" int skh_ransac_fundamental(skh_ctx* ctx, const double* x1, const doub
                                       ctx->stream),
                        "copy task")))
         return rc;
-    launch_ransac(ctx->r_task.as<RansacTask>(), 1, *cfg, ctx->stream);
+    launch_ransac(ctx->r_task.as<RansacTask>(), 1, *cfg, ctx->stream, ctx->r_queue.as<uint32_t>());
     if ((rc = ctx->hip(hipGetLastError(), "launch ransac"))) return rc;
     ROut h;
     if ((rc = ctx->hip(hipMemcpyAsync(&h, o, sizeof h, hipMemcpyDeviceToHost, ctx->stream),

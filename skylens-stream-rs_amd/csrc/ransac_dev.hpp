@@ -564,13 +564,16 @@ __device__ bool accumulate_8pt(const S& s, uint32_t len, double* t1, double* t2,
     return true;
 }
 
-// Lanes 3g, 3g+1, 3g+2 form solver group g (g < kGroups); a group splits the
+// Lanes 3g, 3g+1, 3g+2 of a wave form solver group g (g < kGroups); a group splits the
 // nine eigenvector rows of a 9x9 eigen-solve three per lane while every lane
 // of the group rotates the same matrix, which keeps both in registers.
 constexpr int kGroups = 21;
 
-__device__ __forceinline__ int solver_group() { return (int)threadIdx.x / 3; }
-__device__ __forceinline__ int solver_part() { return (int)threadIdx.x - 3 * ((int)threadIdx.x / 3); }
+__device__ __forceinline__ int solver_group() { return (int)(threadIdx.x & 63u) / 3; }
+__device__ __forceinline__ int solver_part() {
+    const int l = (int)(threadIdx.x & 63u);
+    return l - 3 * (l / 3);
+}
 
 // Eigen-solve of the upper triangle `a` (45, overwritten) by a solver group:
 // d receives the nine eigenvalues and every lane of the group the

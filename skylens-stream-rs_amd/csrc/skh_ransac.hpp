@@ -31,10 +31,12 @@ struct RansacTask {
 };
 
 uint32_t ransac_scratch_words(uint32_t n);
-// One workgroup per task. With `est` and `order` (device, n_tasks words of
-// scratch each), a probe of every task's first hypothesis batch estimates its
-// cost and the workgroups take the tasks costliest first.
+// Runs every task. queue: 2 words of device scratch (the work counters).
+// With `est` and `order` (device, n_tasks words of scratch each), a probe of
+// every task's first hypothesis batch estimates its cost and the tasks are
+// taken costliest first, the costliest 1/32 by whole workgroups.
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
-                   hipStream_t s, uint32_t* est = nullptr, uint32_t* order = nullptr);
+                   hipStream_t s, uint32_t* queue, uint32_t* est = nullptr,
+                   uint32_t* order = nullptr);
 
 }  // namespace skh
