@@ -77,6 +77,18 @@ def cpu_baseline(sc, pairs_idx, threads):
     return time.time() - t0
 
 
+def same_results(a, b, P):
+    """Match lists, Some/None, and for the Some pairs F bits and inlier masks are equal."""
+    import numpy as np
+
+    return (np.array_equal(a.match_len, b.match_len)
+            and all(np.array_equal(a.matches(p), b.matches(p)) for p in range(P))
+            and np.array_equal(a.found, b.found)
+            and all(np.array_equal(a.F[p].view(np.uint64), b.F[p].view(np.uint64))
+                    and np.array_equal(a.inlier_mask(p), b.inlier_mask(p))
+                    for p in range(P) if a.found[p] == 1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +99,14 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the secondary BF16-vs-FP32 timing of the uneven 4000-pair batch")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="build the plan, run it once, then launch the distance kernel 3 times "
+                         "(for rocprofv3 counter runs); prints no JSON")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run a plain device-to-device copy of BYTES three times (the reference "
+                         "kernel for HBM counter calibration); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is measured on "
@@ -97,6 +117,16 @@ def main():
     import torch
 
     import skylens_hip
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {4 * n} bytes 3 times")
+        return
 
     t0 = time.time()
     sc, wl_desc = build_workload(args.workload)
@@ -138,6 +168,13 @@ def main():
 
     plan = make_plan(args.precision)
     log(f"plan {args.precision} built")
+    if args.kernel_only:
+        plan.run()
+        plan.sync()
+        log(f"distance kernel {plan.time_distance(3):.3f} ms")
+        plan.close()
+        ctx.close()
+        return
     el, kern_ms, match_ms, ransac_ms = timed(plan, args.steps, args.warmup)
     res = plan.results()
     ms_step = el / args.steps * 1e3
@@ -153,10 +190,7 @@ def main():
         plan32 = make_plan("fp32")
         el32, k32, m32, r32 = timed(plan32, args.fp32_steps, 1)
         r32res = plan32.results()
-        same = (np.array_equal(r32res.match_len, res.match_len)
-                and all(np.array_equal(r32res.matches(p), res.matches(p)) for p in range(P))
-                and np.array_equal(r32res.found, res.found)
-                and np.array_equal(r32res.F.view(np.uint64), res.F.view(np.uint64)))
+        same = same_results(r32res, res, P)
         v32 = P * args.fp32_steps / el32
         extra = {"fp32_gpu_pairs_per_s": round(v32, 2), "speedup_vs_fp32_gpu": round(value / v32, 3),
                  "fp32_ms_per_step": round(el32 / args.fp32_steps * 1e3, 2),
@@ -165,6 +199,31 @@ def main():
         log(f"fp32: {el32 / args.fp32_steps * 1e3:.2f} ms/step ({m32:.2f} match, {r32:.2f} ransac, "
             f"exact kernel {k32:.2f}), {v32:.0f} pairs/s, identical={same}")
         plan32.close()
+
+    if not args.no_uneven and args.workload != "uneven":
+        # The 4000-pair uneven batch (view sizes log-uniform in [200, 8000]), on which the
+        # BF16-vs-FP32 speedup target is stated: both plans, identical results required.
+        su, _ = build_workload("uneven")
+        du = torch.from_numpy(su.desc).cuda()
+        ku = torch.from_numpy(np.ascontiguousarray(su.kp)).cuda()
+        res_u = {}
+        for prec in (args.precision, "fp32"):
+            pu = skylens_hip.Plan(ctx, su.offsets, du.data_ptr(), ku.data_ptr(), su.pairs, 0.8,
+                                  True, prec, True, 0.0)
+            pu.set_timing(True)
+            el_u, _, m_u, r_u = timed(pu, 2, 1)
+            res_u[prec] = (len(su.pairs) * 2 / el_u, m_u, r_u, pu.results())
+            pu.close()
+        same_u = same_results(res_u[args.precision][3], res_u["fp32"][3], len(su.pairs))
+        extra["uneven_batch"] = {
+            "pairs": len(su.pairs), f"{args.precision}_pairs_per_s": round(res_u[args.precision][0], 2),
+            "fp32_pairs_per_s": round(res_u["fp32"][0], 2),
+            "speedup_vs_fp32_gpu": round(res_u[args.precision][0] / res_u["fp32"][0], 3),
+            f"{args.precision}_match_ms": round(res_u[args.precision][1], 3),
+            "fp32_match_ms": round(res_u["fp32"][1], 3),
+            "ransac_ms": round(res_u[args.precision][2], 3), "identical_to_fp32_gpu": bool(same_u)}
+        log(f"uneven: {extra['uneven_batch']}")
+        del du, ku
 
     achieved = gemm_flop / (kern_ms * 1e-3) / 1e12
     peak = PEAK.get(args.precision, FP32_VALU_PEAK_TOPS)
