@@ -898,7 +898,6 @@ __device__ __noinline__ bool fit_homography(const double2* x1, const double2* x2
 // Hypothesis-batch state; its LDS is reused by the homography fit and the
 // line test after the hypothesis loop.
 struct HypMem {
-    uint32_t idx[NL][8];
     int hCnt[NL];
     double loF[NL][9];
     int loCnt[NL];
@@ -935,6 +934,8 @@ struct Shared {
     int inlCnt;
     int result;
     uint64_t cmask;
+    int pre_drawn;
+    uint32_t pre_next;
     int lineBest[NL];
     double es_med;
     int es_n;
@@ -946,46 +947,66 @@ __device__ __forceinline__ void load9(const double* src, double* dst) {
     for (int i = 0; i < 9; ++i) dst[i] = src[i];
 }
 
-// Solver group g solves hypothesis kGroups * round + g (all three lanes
-// accumulate its 8-point system, each carries three eigenvector rows) and
-// splits its Sampson count; results: hCnt[h] (-1 when fundamental_8pt gives
-// None) and the hypothesis F in loF[h]. Kept out of line so that its
-// registers are allocated on their own.
-__device__ __noinline__ void hyp_round(const double2* x1, const double2* x2, uint32_t n,
-                                       double th2, HypMem& hm, int round) {
+// Hypotheses of one batch in device scratch: samples, F and raw counts.
+struct PreBatch {
+    uint32_t idx[kHypPerBatch][8];
+    int cnt[64];
+    double F[kHypPerBatch][9];
+};
+constexpr uint32_t kPreWords = (uint32_t)((sizeof(PreBatch) + 255) / 256) * 64;  // 256-byte multiple
+
+// One round of the solver groups (lanes 3g..3g+2 form group g): groups
+// [0, nlo) take the LO candidate records hand[g] (A^T A in slots 0..44, the
+// Hartley scales and offsets in 45..50; results: slot 51 = 1 / 0 for Some /
+// None, 52..60 = F), the next hcount groups hypotheses hfirst, hfirst + 1,
+// ... of `pre` (8-point system from the samples, then the Sampson count split
+// over the group's lanes; results: pre->cnt[h], -1 for None, and
+// pre->F[h]). Both kinds share one solve, so a mixed round costs one solve.
+// Out of line: its registers are allocated on their own.
+__device__ __noinline__ void group_solve(const double2* x1, const double2* x2, uint32_t n,
+                                         double th2, double* hand, int nlo, PreBatch* pre,
+                                         int hfirst, int hcount) {
     const int g = solver_group(), part = solver_part(), gbase = 3 * g;
-    const int h = kGroups * round + g;
     if (g >= kGroups) return;
-    PtsIdx8 s{x1, x2, hm.idx[h]};
-    double t1[9], t2[9], ata[45], F[9];
+    const bool is_lo = g < nlo;
+    if (!is_lo && g - nlo >= hcount) return;
+    const int h = hfirst + g - nlo;
+    double* hd = hand + (size_t)g * 64;
+    double ata[45], t1[9], t2[9], F[9];
+    bool ok = true;
+    if (is_lo) {
+#pragma unroll
+        for (int k = 0; k < 45; ++k) ata[k] = hd[k];
+        t1[0] = hd[45]; t1[1] = 0.0; t1[2] = hd[46];
+        t1[3] = 0.0; t1[4] = hd[45]; t1[5] = hd[47];
+        t1[6] = 0.0; t1[7] = 0.0; t1[8] = 1.0;
+        t2[0] = hd[48]; t2[1] = 0.0; t2[2] = hd[49];
+        t2[3] = 0.0; t2[4] = hd[48]; t2[5] = hd[50];
+        t2[6] = 0.0; t2[7] = 0.0; t2[8] = 1.0;
+    } else {
+        PtsIdx8 s{x1, x2, pre->idx[h]};
+        ok = accumulate_8pt(s, 8, t1, t2, ata);
+    }
+    ok = ok && solve_8pt_group(ata, t1, t2, F);
+    if (is_lo) {
+        if (part == 0) {
+            hd[51] = ok ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) hd[52 + k] = F[k];
+        }
+        return;
+    }
     int cnt = -1;
-    if (accumulate_8pt(s, 8, t1, t2, ata) && solve_8pt_group(ata, t1, t2, F)) {
+    if (ok) {
         uint32_t c = 0;
         for (uint32_t i = part; i < n; i += 3) c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
         cnt = (int)(__shfl(c, gbase) + __shfl(c, gbase + 1) + __shfl(c, gbase + 2));
         if (part == 0) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) hm.loF[h][k] = F[k];
+            for (int k = 0; k < 9; ++k) pre->F[h][k] = F[k];
         }
     }
-    if (part == 0) hm.hCnt[h] = cnt;
-}
-
-// The 8-point tail of one LO candidate from its hand-off record (A^T A in
-// slots 0..44, the Hartley scales and offsets in 45..50), solved by the
-// calling solver group; slot 51 receives 1 or 0 (Some / None), 52..60 F.
-__device__ __noinline__ void lo_group_solve(double* hd) {
-    double ata[45], F[9];
-#pragma unroll
-    for (int k = 0; k < 45; ++k) ata[k] = hd[k];
-    const double t1[9] = {hd[45], 0.0, hd[46], 0.0, hd[45], hd[47], 0.0, 0.0, 1.0};
-    const double t2[9] = {hd[48], 0.0, hd[49], 0.0, hd[48], hd[50], 0.0, 0.0, 1.0};
-    const bool ok = solve_8pt_group(ata, t1, t2, F);
-    if (solver_part() == 0) {
-        hd[51] = ok ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) hd[52 + k] = F[k];
-    }
+    if (part == 0) pre->cnt[h] = cnt;
 }
 
 constexpr uint32_t kHandStride = 64;                         // doubles per candidate record
@@ -997,11 +1018,13 @@ constexpr uint32_t kHandWords = 2 * kGroups * kHandStride;   // 32-bit words
 // lanes solve up to 32 candidates' 8-point systems at once, then each
 // candidate's Sampson pass gives its count at x1 and its next selection.
 // Results: loF[h], loCnt[h] and the inlier bitset lo + h * W; list (n
-// words) and hand (kGroups candidate records) are device scratch.
+// words) and hand (kGroups candidate records) are device scratch. Solver
+// groups that a round leaves free solve the next batch's hypotheses of `pre`
+// (reserved through the counter *pre_next).
 __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, uint64_t cand,
                                             double th2, uint32_t* lo, uint32_t* alt,
                                             uint32_t* selb, uint32_t W, uint32_t* list,
-                                            double* hand) {
+                                            double* hand, PreBatch* pre, uint32_t* pre_next) {
     const double2* x1 = t.x1;
     const double2* x2 = t.x2;
     const uint32_t n = t.n;
@@ -1057,8 +1080,15 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                 filled |= 1ull << h;
                 ++rank;
             }
+            // the groups left free take the next batch's hypotheses, in order
+            uint32_t hfirst = 0;
+            if (lane == 0) hfirst = atomicAdd(pre_next, (uint32_t)(kGroups - rank));
+            hfirst = (uint32_t)__shfl((int)hfirst, 0);
+            const int hcount = hfirst < (uint32_t)kHypPerBatch
+                                   ? min(kGroups - rank, kHypPerBatch - (int)hfirst)
+                                   : 0;
             wave_sync();
-            if (solver_group() < rank) lo_group_solve(hand + (size_t)solver_group() * kHandStride);
+            group_solve(x1, x2, n, th2, hand, rank, pre, (int)hfirst, hcount);
             wave_sync();
             rank = 0;
             for (uint64_t m = filled; m; m &= m - 1, ++rank) {
@@ -1124,9 +1154,11 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
     if (t.n_dev) t.n = min(*t.n_dev, t.n);
     const uint32_t n = t.n;
     const uint32_t W = words(n);
-    // device scratch: kWaves hand-off record sets, the bitsets, kWaves index lists
+    // device scratch: kWaves hand-off record sets, the batch hypotheses, the bitsets,
+    // kWaves index lists
     double* hand = reinterpret_cast<double*>(t.scratch + (size_t)wave * kHandWords);
-    uint32_t* masks = t.scratch + (size_t)kWaves * kHandWords;
+    PreBatch* pre = reinterpret_cast<PreBatch*>(t.scratch + (size_t)kWaves * kHandWords);
+    uint32_t* masks = t.scratch + (size_t)kWaves * kHandWords + kPreWords;
     uint32_t* lo_masks = masks;                                 // NL * W
     uint32_t* alt_masks = masks + (size_t)NL * W;               // NL * W
     uint32_t* sel_masks = masks + (size_t)2 * NL * W;           // NL * W
@@ -1156,6 +1188,8 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
         sh.iters = cfg.max_iters;
         sh.it = 0;
         sh.result = 0;
+        sh.pre_drawn = 0;
+        sh.pre_next = 0;
         for (int k = 0; k < 6; ++k) sh.poolSlot[k] = k;
     }
     if (lead && lane < 11) pc[lane] = 0;
@@ -1183,21 +1217,34 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
         lap(3);
         if (lead) {
             if (lane == 0) pc[8] += 1;
-            // matching.rs:971-979: eight distinct draws per hypothesis
-            lcg_samples<8>(smp, jump, n, kHypPerBatch, &pm.hyp.idx[0][0], 8);
-            if (lane >= kHypPerBatch) pm.hyp.hCnt[lane] = -1;
+            // matching.rs:971-979: eight distinct draws per hypothesis (unless the previous
+            // batch already drew them)
+            if (!sh.pre_drawn) {
+                lcg_samples<8>(smp, jump, n, kHypPerBatch, &pre->idx[0][0], 8);
+                if (lane == 0) sh.pre_next = 0;
+            }
         }
         pair_sync();
         lap(0);
-        if (!big) {
-            for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, pm.hyp, round);
-        } else if (wave < 3) {
-            hyp_round(t.x1, t.x2, n, th2, pm.hyp, wave);
+        {
+            // the batch's hypotheses that the previous batch's LO rounds did not solve
+            const int done = min((int)sh.pre_next, kHypPerBatch);
+            const int rounds = (kHypPerBatch - done + kGroups - 1) / kGroups;
+            for (int r = big ? wave : 0; r < rounds; r += big ? kWaves : 1) {
+                const int first = done + kGroups * r;
+                group_solve(t.x1, t.x2, n, th2, hand, 0, pre, first,
+                            min(kGroups, kHypPerBatch - first));
+            }
         }
         pair_sync();
         lap(1);
         if (lead) {
-            const int cnt = pm.hyp.hCnt[lane];
+            const int cnt = lane < kHypPerBatch ? pre->cnt[lane] : -1;
+            pm.hyp.hCnt[lane] = cnt;
+            if (cnt >= 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) pm.hyp.loF[lane][k] = pre->F[lane][k];
+            }
             // LO candidates: bound = max(best count now, raw counts >= 8 earlier in the batch)
             int v = cnt >= 8 ? cnt : 0;
             int incl = v;
@@ -1216,6 +1263,14 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
                 sh.cmask = cm;
             }
             if (!cand) pm.hyp.loCnt[lane] = -1;
+            // the next batch's samples: the stream continues where this batch's ended; free
+            // solver groups of this batch's LO rounds solve them
+            wave_sync();
+            lcg_samples<8>(smp, jump, n, kHypPerBatch, &pre->idx[0][0], 8);
+            if (lane == 0) {
+                sh.pre_next = 0;
+                sh.pre_drawn = 1;
+            }
         }
         pair_sync();
         // a large pair's waves take every kWaves-th candidate
@@ -1227,7 +1282,8 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
                 if (r % kWaves == wave) sub |= m & (~m + 1);
             mine = sub;
         }
-        local_opt_batch(t, pm.hyp, mine, th2, lo_masks, alt_masks, sel_masks, W, list_buf, hand);
+        local_opt_batch(t, pm.hyp, mine, th2, lo_masks, alt_masks, sel_masks, W, list_buf, hand,
+                        pre, &sh.pre_next);
         pair_sync();
         lap(2);
         // replay in order (lane 0)
@@ -1649,11 +1705,13 @@ void ransac_probe_kernel(const RansacTask* __restrict__ tasks, const RansacCfgDe
     const double th2 = cfg.threshold_px * cfg.threshold_px;
     const LcgLane jump = lcg_lane_jump();
     LcgStream smp{cfg.seed ^ 0x9E3779B97F4A7C15ull, 0u, (uint32_t)NL};
-    lcg_samples<8>(smp, jump, n, kHypPerBatch, &s_pm[0].hyp.idx[0][0], 8);
+    PreBatch* pre = reinterpret_cast<PreBatch*>(t.scratch + (size_t)kWaves * kHandWords);
+    lcg_samples<8>(smp, jump, n, kHypPerBatch, &pre->idx[0][0], 8);
     wave_sync();
-    for (int round = 0; round < 3; ++round) hyp_round(t.x1, t.x2, n, th2, s_pm[0].hyp, round);
+    for (int round = 0; round < 3; ++round)
+        group_solve(t.x1, t.x2, n, th2, nullptr, 0, pre, kGroups * round, kGroups);
     wave_sync();
-    int best = lane < kHypPerBatch ? s_pm[0].hyp.hCnt[lane] : 0;
+    int best = lane < kHypPerBatch ? pre->cnt[lane] : 0;
 #pragma unroll
     for (int off = 1; off < NL; off <<= 1) {
         const int o = __shfl_xor(best, off);
@@ -1721,9 +1779,10 @@ void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev
 }
 
 uint32_t ransac_scratch_words(uint32_t n) {
-    // kWaves sets of LO hand-off records, then the bitsets and kWaves index
-    // lists; a multiple of 64 words keeps every pair's scratch 256-byte aligned
-    const uint32_t w = kWaves * kHandWords + kMaskSets * words(n);
+    // kWaves sets of LO hand-off records, the batch hypotheses, then the
+    // bitsets and kWaves index lists; a multiple of 64 words keeps every
+    // pair's scratch 256-byte aligned
+    const uint32_t w = kWaves * kHandWords + kPreWords + kMaskSets * words(n);
     return (w + 63u) & ~63u;
 }
 
