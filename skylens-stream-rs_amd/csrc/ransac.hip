@@ -312,14 +312,27 @@ __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double
                                 uint32_t* ns) {
     const int lane = lane_id();
     uint32_t cnt = 0, cs = 0;
+    // the next chunk's points are loaded while this chunk's errors are computed
+    double2 p = make_double2(0.0, 0.0), q = p;
+    if ((uint32_t)lane < n) {
+        p = x1[lane];
+        q = x2[lane];
+    }
     for (uint32_t base = 0; base < n; base += NL) {
-        const uint32_t i = base + lane;
+        const uint32_t i = base + lane, inext = i + NL;
+        double2 pn = make_double2(0.0, 0.0), qn = pn;
+        if (inext < n) {
+            pn = x1[inext];
+            qn = x2[inext];
+        }
         bool a = false, b = false;
         if (i < n) {
-            const double d = sampson_error(F, x1[i], x2[i]);
+            const double d = sampson_error(F, p, q);
             a = d < th2;
             b = d < t2;
         }
+        p = pn;
+        q = qn;
         const uint64_t ma = __ballot(a), mb = __ballot(b);
         if (lane < 2 && base + 32u * lane < n) {
             const uint32_t w = (base >> 5) + lane;
@@ -798,8 +811,18 @@ __device__ __noinline__ void hom_round(const double2* x1, const double2* x2, con
     homography_accumulate(s4, 4, t1, t2, ata);
     int c = -1;
     if (homography_finish_group(ata, t1, t2, hh) && inv3(hh, hi)) {
-        uint32_t cc = 0;
-        for (uint32_t r = part; r < ns; r += 3) {
+        // ranks part, part + 3, ...: four per step, loads issued together
+        uint32_t cc = 0, r = part;
+        for (; r + 9 < ns; r += 12) {
+            const uint32_t i0 = list[r], i1 = list[r + 3], i2 = list[r + 6], i3 = list[r + 9];
+            const double2 a0 = x1[i0], b0 = x2[i0], a1 = x1[i1], b1 = x2[i1];
+            const double2 a2 = x1[i2], b2 = x2[i2], a3 = x1[i3], b3 = x2[i3];
+            cc += homography_error(hh, hi, a0, b0) < th ? 1u : 0u;
+            cc += homography_error(hh, hi, a1, b1) < th ? 1u : 0u;
+            cc += homography_error(hh, hi, a2, b2) < th ? 1u : 0u;
+            cc += homography_error(hh, hi, a3, b3) < th ? 1u : 0u;
+        }
+        for (; r < ns; r += 3) {
             const uint32_t i = list[r];
             cc += homography_error(hh, hi, x1[i], x2[i]) < th ? 1u : 0u;
         }
@@ -998,8 +1021,17 @@ __device__ __noinline__ void group_solve(const double2* x1, const double2* x2, u
     }
     int cnt = -1;
     if (ok) {
-        uint32_t c = 0;
-        for (uint32_t i = part; i < n; i += 3) c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
+        // points part, part + 3, ...: four per step, loads issued together
+        uint32_t c = 0, i = part;
+        for (; i + 9 < n; i += 12) {
+            const double2 a0 = x1[i], b0 = x2[i], a1 = x1[i + 3], b1 = x2[i + 3];
+            const double2 a2 = x1[i + 6], b2 = x2[i + 6], a3 = x1[i + 9], b3 = x2[i + 9];
+            c += sampson_error(F, a0, b0) < th2 ? 1u : 0u;
+            c += sampson_error(F, a1, b1) < th2 ? 1u : 0u;
+            c += sampson_error(F, a2, b2) < th2 ? 1u : 0u;
+            c += sampson_error(F, a3, b3) < th2 ? 1u : 0u;
+        }
+        for (; i < n; i += 3) c += sampson_error(F, x1[i], x2[i]) < th2 ? 1u : 0u;
         cnt = (int)(__shfl(c, gbase) + __shfl(c, gbase + 1) + __shfl(c, gbase + 2));
         if (part == 0) {
 #pragma unroll

@@ -651,8 +651,9 @@ __device__ __forceinline__ double sampson_error(const double* F, double2 p, doub
     mtv3(F, b, ftb);
     double e = b[0] * fa[0] + b[1] * fa[1] + b[2] * fa[2];
     double den = fa[0] * fa[0] + fa[1] * fa[1] + ftb[0] * ftb[0] + ftb[1] * ftb[1];
-    if (den <= 0.0) return INFINITY;
-    return e * e / den;
+    // branch-free: the quotient is discarded when den <= 0
+    const double err = e * e / den;
+    return den <= 0.0 ? (double)INFINITY : err;
 }
 
 }  // namespace rdev
