@@ -109,18 +109,11 @@ static inline uint64_t rnd(uint64_t* st, uint64_t m) {
 
 /* ------------------------------------------------------------ eigen / SVD */
 
-/* sqrt(a^2 + b^2) without undue overflow: the larger magnitude times
- * sqrt(1 + ratio^2). */
-static inline double pythag(double a, double b) {
-    const double aa = fabs(a), ab = fabs(b);
-    if (aa > ab) {
-        const double r = ab / aa;
-        return aa * sqrt(1.0 + r * r);
-    }
-    if (ab == 0.0) return 0.0;
-    const double r = aa / ab;
-    return ab * sqrt(1.0 + r * r);
-}
+/* sqrt(a^2 + b^2), formed directly: the operands (shifts and couplings of
+ * the tridiagonal form of a Hartley-normalised 9x9 A^T A, entries of order
+ * the point count) stay many decades away from overflow, and a sum that
+ * underflows to 0 is the r == 0 case of the sweep. */
+static inline double hyp2(double a, double b) { return sqrt(a * a + b * b); }
 
 /* Householder tridiagonalisation of the symmetric 9x9 `a` (row-major, full,
  * overwritten; Golub & Van Loan, Matrix Computations 4th ed., Alg. 8.3.1),
@@ -180,8 +173,9 @@ static void tridiag9(double* a, double* q, double* d, double* e) {
 #define QL_MAX_ITERS 30
 
 /* Implicit QL with shifts on the symmetric tridiagonal (d, e) (Wilkinson &
- * Reinsch, Handbook for Automatic Computation II, tql2), rotating the rows
- * of q. Eigenvalue l is split off once e[l] is negligible against its
+ * Reinsch, Handbook for Automatic Computation II, tql2; each rotation's
+ * radius is sqrt(f^2 + g^2) and its cosine and sine two divisions by it),
+ * rotating the rows of q. Eigenvalue l is split off once e[l] is negligible against its
  * neighbours; at most QL_MAX_ITERS sweeps per eigenvalue. */
 static void tql9(double* d, double* e, double* q) {
     for (int l = 0; l < 9; ++l) {
@@ -193,13 +187,13 @@ static void tql9(double* d, double* e, double* q) {
             }
             if (m == l) break;
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = pythag(g, 1.0);
+            double r = hyp2(g, 1.0);
             g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
             double s = 1.0, c = 1.0, p = 0.0;
             int underflow = 0;
             for (int i = m - 1; i >= l; --i) {
                 const double f = s * e[i], b = c * e[i];
-                r = pythag(f, g);
+                r = hyp2(f, g);
                 e[i + 1] = r;
                 if (r == 0.0) {
                     d[i + 1] = d[i + 1] - p;

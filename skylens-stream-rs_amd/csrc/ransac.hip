@@ -306,12 +306,14 @@ __device__ __forceinline__ void ut_rc(int k, int& r, int& c) {
 
 // Sampson errors of F over all n points, one pass: the bitset {d < th2}
 // into `at` and its size (returned); when `sel` is set, the bitset {d < t2}
-// into `sel` and its size into *ns.
+// into `sel` (overwritten) and its size into *ns, and when `changed` is set
+// too, whether `sel` differed from the new bitset into *changed.
 __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double2* x2, uint32_t n,
                                 double th2, uint32_t* at, double t2, uint32_t* sel,
-                                uint32_t* ns) {
+                                uint32_t* ns, bool* changed = nullptr) {
     const int lane = lane_id();
     uint32_t cnt = 0, cs = 0;
+    bool diff = false;
     // the next chunk's points are loaded while this chunk's errors are computed
     double2 p = make_double2(0.0, 0.0), q = p;
     if ((uint32_t)lane < n) {
@@ -337,13 +339,18 @@ __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double
         if (lane < 2 && base + 32u * lane < n) {
             const uint32_t w = (base >> 5) + lane;
             at[w] = lane ? (uint32_t)(ma >> 32) : (uint32_t)ma;
-            if (sel) sel[w] = lane ? (uint32_t)(mb >> 32) : (uint32_t)mb;
+            if (sel) {
+                const uint32_t nw = lane ? (uint32_t)(mb >> 32) : (uint32_t)mb;
+                if (changed) diff |= sel[w] != nw;
+                sel[w] = nw;
+            }
         }
         cnt += (uint32_t)__popcll(ma);
         cs += (uint32_t)__popcll(mb);
     }
     wave_sync();
     if (ns) *ns = cs;
+    if (changed) *changed = __any(diff);
     return cnt;
 }
 
@@ -1136,8 +1143,10 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                 const bool sw = (swapped >> h) & 1u;
                 uint32_t* spare = (sw ? lo : alt) + (size_t)h * W;
                 uint32_t ns = 0;
+                bool changed = true;
                 const uint32_t gc = coop_masks2(gh, x1, x2, n, th2, spare, t2n,
-                                                r < 5 ? selb + (size_t)h * W : nullptr, &ns);
+                                                r < 5 ? selb + (size_t)h * W : nullptr, &ns,
+                                                &changed);
                 const int cur = hm.loCnt[h];
                 wave_sync();
                 if (lane == 0) hm.loNs[h] = ns;
@@ -1149,6 +1158,11 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                     }
                     swapped ^= 1ull << h;
                 }
+                // From round 2 on the selection thresholds of this and every later round are
+                // equal (kNM[3..5] = 1): an unchanged selection makes the next round solve
+                // this round's system again, giving the same F, a count that cannot beat the
+                // current one and the same selection, so every remaining round is a no-op.
+                if (r >= 2 && !changed) active &= ~(1ull << h);
                 wave_sync();
             }
         }

@@ -167,17 +167,9 @@ __host__ __device__ constexpr int ut(int r, int c) {
     return r <= c ? r * 9 - r * (r - 1) / 2 + (c - r) : c * 9 - c * (c - 1) / 2 + (r - c);
 }
 
-// sqrt(a^2 + b^2) without undue overflow.
-__device__ __forceinline__ double pythag(double a, double b) {
-    const double aa = fabs(a), ab = fabs(b);
-    if (aa > ab) {
-        const double r = ab / aa;
-        return aa * sqrt(1.0 + r * r);
-    }
-    if (ab == 0.0) return 0.0;
-    const double r = aa / ab;
-    return ab * sqrt(1.0 + r * r);
-}
+// sqrt(a^2 + b^2), formed directly (the operands of the QL sweep stay far
+// from overflow; an underflow to 0 is the sweep's r == 0 case).
+__device__ __forceinline__ double hyp2(double a, double b) { return sqrt(a * a + b * b); }
 
 // Tridiagonalises `a` (upper triangle, 45, overwritten); q: rows r0..r0+R-1
 // of H_0 ... H_6; d: diagonal, e[i]: (i, i+1) coupling, e[8] = 0.
@@ -277,7 +269,7 @@ __device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) 
             });
             if (m == l) break;
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = pythag(g, 1.0);
+            double r = hyp2(g, 1.0);
             double dm = d[8];
 #pragma unroll
             for (int j = l; j < 8; ++j) dm = (j == m) ? d[j] : dm;
@@ -288,7 +280,7 @@ __device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) 
                 constexpr int i = 7 - decltype(ic)::value;
                 if (i < m && !underflow) {
                     const double f = s * e[i], b = c * e[i];
-                    r = pythag(f, g);
+                    r = hyp2(f, g);
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] = d[i + 1] - p;
