@@ -1,22 +1,26 @@
-// ransac_fundamental on gfx950: one 64-lane workgroup per image pair.
+// ransac_fundamental on gfx950.
 //
 // Reference: crates/core/src/matching.rs:941-1104 (and the helpers it calls).
 // The sequential semantics (adaptive iteration count, local-optimisation
 // trigger, 5-entry Jaccard pool, stable sort) are replayed exactly while the
 // heavy work runs across lanes:
-//   * hypotheses in batches of 64: lane 0 draws the seeded samples
-//     (matching.rs:956-979), every lane solves one 8-point F (FP64) and counts
-//     its Sampson inliers;
+//   * a persistent launch of four-wave workgroups takes pairs from a queue in
+//     descending order of a first-batch cost estimate; the costliest pairs run
+//     on a whole workgroup, the others on one wave each;
+//   * hypotheses in batches of 63: the lead wave draws the seeded samples
+//     (matching.rs:956-979), solver groups of three lanes solve one 8-point F
+//     each (FP64) and count its Sampson inliers;
 //   * local optimisation for hypothesis k depends only on F_k and the data
 //     (matching.rs:993-1015), so every hypothesis that *could* trigger it runs
-//     it in its own lane: cnt >= 8 and 4 cnt >= max(best count at batch
-//     start, raw counts >= 8 earlier in the batch), a superset of the real
-//     triggers because the best count is never below an earlier raw count;
-//   * lane 0 replays the batch in order with the precomputed results; pool
-//     Jaccard tests use inlier bitsets;
-//   * the pool's Sampson-LM trajectories (matching.rs:1041-1057) run in
-//     parallel lanes and are merged in pool order; the final refits, gates
-//     and GRIC model selection follow the reference.
+//     it: cnt >= 8 and 4 cnt >= max(best count at batch start, raw counts
+//     >= 8 earlier in the batch), a superset of the real triggers because the
+//     best count is never below an earlier raw count; point sums run
+//     wave-cooperatively in the host's order, the 8-point solves of all
+//     candidates (and of the next batch's hypotheses) share solver rounds;
+//   * the lead wave replays the batch in order with the precomputed results;
+//     pool Jaccard tests use inlier bitsets;
+//   * the pool's Sampson-LM trajectories (matching.rs:1041-1057), the final
+//     refits, gates and GRIC model selection follow the reference.
 // All FP64 routines match the host restatement operation for operation.
 #pragma clang fp contract(off)
 
@@ -44,6 +48,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+__device__ __forceinline__ void load9(const double* src, double* dst) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dst[i] = src[i];
+}
+
 __constant__ double kNM[6] = {3.0, 2.0, 1.5, 1.0, 1.0, 1.0};
 
 __host__ __device__ __forceinline__ uint32_t words(uint32_t n) { return (n + 31) / 32; }
@@ -172,8 +181,11 @@ __device__ uint64_t parallax_needed(uint64_t n) {
 
 constexpr int kStageW = 10;  // doubles staged per point
 
+constexpr int kLmSpec = 8;  // LM tries of one iteration evaluated side by side
+
 struct Coop {
     double stage[NL][kStageW];
+    double cand[kLmSpec][9];  // the candidate F of each LM try
 };
 __shared__ Coop s_cp[kWaves];
 
@@ -504,6 +516,46 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
     return cost;
 }
 
+// coop_lm_cost of the kLmSpec candidates coop().cand[0..kLmSpec): lane k
+// (and every lane congruent to k) returns candidate k's sum.
+__device__ double coop_lm_costs(double s1, double s2, const double* t1, const double* t2,
+                                const double2* x1, const double2* x2, const uint32_t* list,
+                                uint32_t len) {
+    const int lane = lane_id();
+    const double q1 = s1 * s1, q2 = s2 * s2;
+    double cost = 0.0;
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t slot = lane, pi = list[base + lane];
+            const double2 p = x1[pi], q = x2[pi];
+            double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
+            mv3(t1, pa, a);
+            mv3(t2, pb, b);
+#pragma unroll
+            for (int k = 0; k < kLmSpec; ++k) {
+                double g[9];
+                load9(coop().cand[k], g);
+                double ga[3], gtb[3];
+                mv3(g, a, ga);
+                mtv3(g, b, gtb);
+                double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) +
+                             q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
+                double e;
+                if (den > 0.0)
+                    e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
+                else
+                    e = 0.0;
+                coop().stage[slot][k] = e * e;
+            }
+        }
+        wave_sync();
+        cost = stage_sum(lane & (kLmSpec - 1), cnt, cost);
+        wave_sync();
+    }
+    return cost;
+}
+
 // J^T J (lanes 0..44, upper-triangle slots) and J^T r (lanes 45..53) of the
 // Sampson residuals over `sel` (matching.rs:585-618, :690-706).
 __device__ double coop_normal_eq(const double* g, double s1, double s2,
@@ -579,6 +631,7 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
     if (!rank2_unit(g0, g)) return;
     double cost = coop_lm_cost(g, s1, s2, t1, t2, x1, x2, list, len);
     double lambda = 1e-3;
+    const int lane = lane_id();
     for (int it = 0; it < iters; ++it) {
         const double ne = coop_normal_eq(g, s1, s2, t1, t2, x1, x2, list, len);
         double jtj[45], jtr[9];
@@ -586,39 +639,51 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
         for (int k = 0; k < 45; ++k) jtj[k] = readlane_d(ne, k);
 #pragma unroll
         for (int k = 0; k < 9; ++k) jtr[k] = readlane_d(ne, 45 + k);
-        bool improved = false;
-        for (int tries = 0; tries < 8; ++tries) {
-            double m[45];
+        // The host's up to kLmSpec tries run one after another, each failure (no
+        // Cholesky factor, no rank-2 unit F, no lower cost) multiplying lambda by
+        // 10. Here lane k takes try k with lambda 10^k-fold (multiplied up in the
+        // same order), the costs of all tries are summed side by side, and the
+        // first try in order that succeeds is the one the host accepts.
+        const int k = lane & (kLmSpec - 1);
+        double lam = lambda;
+        for (int j = 0; j < k; ++j) lam *= 10.0;
+        double m[45];
 #pragma unroll
-            for (int i = 0; i < 45; ++i) m[i] = jtj[i];
+        for (int i = 0; i < 45; ++i) m[i] = jtj[i];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) m[ut(k, k)] = m[ut(k, k)] + lambda * (jtj[ut(k, k)] + 1e-12);
-            double nrhs[9], d[9];
+        for (int i = 0; i < 9; ++i) m[ut(i, i)] = m[ut(i, i)] + lam * (jtj[ut(i, i)] + 1e-12);
+        double nrhs[9], d[9], cand[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) nrhs[k] = -jtr[k];
-            if (!cholesky_solve9(m, nrhs, d)) {
-                lambda *= 10.0;
-                continue;
-            }
-            double gs[9], cand[9];
+        for (int i = 0; i < 9; ++i) nrhs[i] = -jtr[i];
+        bool ok = cholesky_solve9(m, nrhs, d);
+        if (ok) {
+            double gs[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
-            if (!rank2_unit(gs, cand)) {
-                lambda *= 10.0;
-                continue;
-            }
-            double c = coop_lm_cost(cand, s1, s2, t1, t2, x1, x2, list, len);
-            if (c < cost) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) g[k] = cand[k];
-                double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
-                cost = c;
-                lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;
-                improved = rel > 1e-10;
-                break;
-            }
-            lambda *= 10.0;
+            for (int i = 0; i < 9; ++i) gs[i] = g[i] + d[i];
+            ok = rank2_unit(gs, cand);
         }
+        if (!ok) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) cand[i] = 0.0;
+        }
+        if (lane < kLmSpec) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) coop().cand[lane][i] = cand[i];
+        }
+        wave_sync();
+        const double c = coop_lm_costs(s1, s2, t1, t2, x1, x2, list, len);
+        const uint64_t win = __ballot(lane < kLmSpec && ok && c < cost);
+        bool improved = false;
+        if (win) {
+            const int kb = __ffsll((unsigned long long)win) - 1;
+            const double cb = readlane_d(c, kb), lb = readlane_d(lam, kb);
+            load9(coop().cand[kb], g);
+            const double rel = (cost - cb) / (cost > 1e-300 ? cost : 1e-300);
+            cost = cb;
+            lambda = lb * 0.1 > 1e-9 ? lb * 0.1 : 1e-9;
+            improved = rel > 1e-10;
+        }
+        wave_sync();
         if (!improved) break;
     }
     double t2t[9], fp[9];
@@ -971,11 +1036,6 @@ struct Shared {
     int es_n;
     double redA[NL];
 };
-
-__device__ __forceinline__ void load9(const double* src, double* dst) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) dst[i] = src[i];
-}
 
 // Hypotheses of one batch in device scratch: samples, F and raw counts.
 struct PreBatch {
