@@ -277,13 +277,22 @@ __device__ __forceinline__ uint64_t bits64(const uint32_t* bits, uint32_t base, 
 }
 
 // acc = acc + stage[j][u] for the cnt staged points, in order. The loads of
-// a group of eight are issued before its dependent adds.
+// a group of eight are issued while the previous group is being added.
 __device__ __forceinline__ double stage_sum(int u, uint32_t cnt, double acc) {
     uint32_t j = 0;
-    for (; j + 8 <= cnt; j += 8) {
+    if (cnt >= 8) {
         double v[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = coop().stage[j + e][u];
+        for (int e = 0; e < 8; ++e) v[e] = coop().stage[e][u];
+        for (j = 8; j + 8 <= cnt; j += 8) {
+            double nx[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) nx[e] = coop().stage[j + e][u];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc = acc + v[e];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = nx[e];
+        }
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc = acc + v[e];
     }
@@ -291,19 +300,68 @@ __device__ __forceinline__ double stage_sum(int u, uint32_t cnt, double acc) {
     return acc;
 }
 
-// acc = acc + stage[j][u] * stage[j][w] for the cnt staged points, in order.
+// acc = acc + stage[j][u] * stage[j][w] for the cnt staged points, in order,
+// pipelined like stage_sum.
 __device__ __forceinline__ double stage_dot(int u, int w, uint32_t cnt,
                                             double acc) {
     uint32_t j = 0;
-    for (; j + 8 <= cnt; j += 8) {
+    if (cnt >= 8) {
         double v[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = coop().stage[j + e][u] * coop().stage[j + e][w];
+        for (int e = 0; e < 8; ++e) v[e] = coop().stage[e][u] * coop().stage[e][w];
+        for (j = 8; j + 8 <= cnt; j += 8) {
+            double a[8], b[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                a[e] = coop().stage[j + e][u];
+                b[e] = coop().stage[j + e][w];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc = acc + v[e];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = a[e] * b[e];
+        }
 #pragma unroll
         for (int e = 0; e < 8; ++e) acc = acc + v[e];
     }
     for (; j < cnt; ++j) acc = acc + coop().stage[j][u] * coop().stage[j][w];
     return acc;
+}
+
+// One pass over the points list[0..len) in chunks of NL: stage(slot, p, q)
+// for each lane holding a point of the chunk (slot = lane), then, once the
+// wave has staged, reduce(cnt) over the chunk's cnt points. The next chunk's
+// points and the indices two chunks ahead are loaded while a chunk is
+// processed.
+template <class Stage, class Reduce>
+__device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
+                                          const uint32_t* list, uint32_t len, Stage&& stage,
+                                          Reduce&& reduce) {
+    const uint32_t lane = (uint32_t)lane_id();
+    double2 p = make_double2(0.0, 0.0), q = p;
+    if (lane < len) {
+        const uint32_t i0 = list[lane];
+        p = x1[i0];
+        q = x2[i0];
+    }
+    uint32_t inext = lane + NL < len ? list[lane + NL] : 0u;
+    for (uint32_t base = 0; base < len; base += NL) {
+        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+        double2 pn = make_double2(0.0, 0.0), qn = pn;
+        if (base + NL + lane < len) {
+            pn = x1[inext];
+            qn = x2[inext];
+        }
+        const uint32_t j2 = base + 2 * NL + lane;
+        const uint32_t inext2 = j2 < len ? list[j2] : 0u;
+        if (lane < cnt) stage(lane, p, q);
+        wave_sync();
+        reduce(cnt);
+        wave_sync();
+        p = pn;
+        q = qn;
+        inext = inext2;
+    }
 }
 
 // (row, col) of upper-triangle slot k (ut order), k < 45.
@@ -402,29 +460,22 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
                           uint32_t len, double* t1, double* t2) {
     const int lane = lane_id();
     double acc = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 a = x1[pi], b = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 a, double2 b) {
             coop().stage[slot][0] = a.x;
             coop().stage[slot][1] = a.y;
             coop().stage[slot][2] = b.x;
             coop().stage[slot][3] = b.y;
-        }
-        wave_sync();
-        acc = stage_sum(lane & 3, cnt, acc);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            acc = stage_sum(lane & 3, cnt, acc);
+        });
     const double nc = (double)(len > 0 ? len : 1);
     const double c1x = readlane_d(acc, 0) / nc, c1y = readlane_d(acc, 1) / nc;
     const double c2x = readlane_d(acc, 2) / nc, c2y = readlane_d(acc, 3) / nc;
     acc = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 a = x1[pi], b = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 a, double2 b) {
             const double ax = a.x - c1x, ay = a.y - c1y, bx = b.x - c2x, by = b.y - c2y;
             coop().stage[slot][0] = ax * ax;
             coop().stage[slot][1] = ax * ay;
@@ -434,11 +485,10 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
             coop().stage[slot][5] = by * by;
             coop().stage[slot][6] = norm2(ax, ay);
             coop().stage[slot][7] = norm2(bx, by);
-        }
-        wave_sync();
-        acc = stage_sum(lane & 7, cnt, acc);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            acc = stage_sum(lane & 7, cnt, acc);
+        });
     double q[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) q[k] = readlane_d(acc, k);
@@ -453,11 +503,8 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
     int r, c;
     ut_rc(lane < 45 ? lane : 0, r, c);
     double acc = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 p = x1[pi], q = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 p, double2 q) {
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -471,11 +518,10 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
             row[6] = a[0];
             row[7] = a[1];
             row[8] = 1.0;
-        }
-        wave_sync();
-        acc = stage_dot(r, c, cnt, acc);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            acc = stage_dot(r, c, cnt, acc);
+        });
     return acc;
 }
 
@@ -489,11 +535,8 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
     const int lane = lane_id();
     const double q1 = s1 * s1, q2 = s2 * s2;
     double cost = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 p = x1[pi], q = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 p, double2 q) {
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -508,11 +551,10 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
             else
                 e = 0.0;
             coop().stage[slot][0] = e * e;
-        }
-        wave_sync();
-        cost = stage_sum(0, cnt, cost);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            cost = stage_sum(0, cnt, cost);
+        });
     return cost;
 }
 
@@ -524,11 +566,8 @@ __device__ double coop_lm_costs(double s1, double s2, const double* t1, const do
     const int lane = lane_id();
     const double q1 = s1 * s1, q2 = s2 * s2;
     double cost = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 p = x1[pi], q = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 p, double2 q) {
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -548,11 +587,10 @@ __device__ double coop_lm_costs(double s1, double s2, const double* t1, const do
                     e = 0.0;
                 coop().stage[slot][k] = e * e;
             }
-        }
-        wave_sync();
-        cost = stage_sum(lane & (kLmSpec - 1), cnt, cost);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            cost = stage_sum(lane & (kLmSpec - 1), cnt, cost);
+        });
     return cost;
 }
 
@@ -571,11 +609,8 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
         w = 9;
     }
     double acc = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 p = x1[pi], q = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 p, double2 q) {
             double pa[3] = {p.x, p.y, 1.0}, pb[3] = {q.x, q.y, 1.0}, a[3], b[3];
             mv3(t1, pa, a);
             mv3(t2, pb, b);
@@ -601,11 +636,10 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
                         row[3 * k + l] = b[k] * a[l] * inv - c * (uu[k] * a[l] + b[k] * vv[l]);
                 row[9] = e * inv;
             }
-        }
-        wave_sync();
-        acc = stage_dot(u, w, cnt, acc);
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            acc = stage_dot(u, w, cnt, acc);
+        });
     return acc;
 }
 
@@ -796,11 +830,8 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
     ut_rc(lane < 45 ? lane : 0, r, c);
     const int a1 = kDltR1[r], b1 = kDltR1[c], a2 = kDltR2[r], b2 = kDltR2[c];
     double acc = 0.0;
-    for (uint32_t base = 0; base < len; base += NL) {
-        const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        if ((uint32_t)lane < cnt) {
-            const uint32_t slot = lane, pi = list[base + lane];
-            const double2 pp = x1[pi], qq = x2[pi];
+    list_pass(x1, x2, list, len,
+        [&](uint32_t slot, double2 pp, double2 qq) {
             double p3[3] = {pp.x, pp.y, 1.0}, v[3];
             mv3(t1, p3, v);
             const double x = v[0] / v[2], y = v[1] / v[2];
@@ -818,14 +849,24 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
             row[7] = w * x;
             row[8] = w * y;
             row[9] = w;
-        }
-        wave_sync();
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const double* row = coop().stage[j];
-            acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
-        }
-        wave_sync();
-    }
+        },
+        [&](uint32_t cnt) {
+            uint32_t j = 0;
+            for (; j + 8 <= cnt; j += 8) {
+                double v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const double* row = coop().stage[j + e];
+                    v[e] = row[a1] * row[b1] + row[a2] * row[b2];
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc = acc + v[e];
+            }
+            for (; j < cnt; ++j) {
+                const double* row = coop().stage[j];
+                acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
+            }
+        });
     double ata[45];
 #pragma unroll
     for (int k = 0; k < 45; ++k) ata[k] = readlane_d(acc, k);
