@@ -52,7 +52,7 @@ struct Plan {
     std::vector<uint64_t> view_off;
     std::vector<uint32_t> pairs;
     std::vector<uint64_t> match_base;  // n_pairs + 1
-    int tile = kTile128x128;
+    int tile = kTile256x64;
     uint32_t tbits = 1;
     bool lowp = false;
     uint32_t n_blocks = 0;
@@ -122,7 +122,7 @@ int Plan::build() {
     lowp = desc.precision == SKH_PREC_BF16;
     for (uint32_t v = 0; v < V; ++v) max_view_rows = std::max(max_view_rows, rows(v));
     if (lowp && max_view_rows > 131072) lowp = false;  // tile ids would erode the key precision
-    tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile128x128;
+    tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile256x64;
     const uint32_t bm = lowp ? lowp_block_rows(tile) : 64u;
     tbits = tile_bits(max_view_rows);
 

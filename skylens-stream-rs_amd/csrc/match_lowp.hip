@@ -40,13 +40,13 @@ __device__ __forceinline__ uint32_t bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(uint32_t h) { return __uint_as_float(h << 16); }
 
-// min(a, -p) as one v_min_f32 with a source negation. fminf would first
-// canonicalise its inputs (one more v_max per element); keys of pairs that
-// reach a decision are never NaN, so the plain instruction is exact here.
-__device__ __forceinline__ float min_neg(float a, float p) {
-    float r;
-    asm("v_min_f32 %0, %1, -%2" : "=v"(r) : "v"(a), "v"(p));
-    return r;
+// (k1, k2) <- the two smallest of {k1, k2, -p} (k1 <= k2): v_med3_f32 then
+// v_min_f32, both with a source negation, updating the registers in place.
+// fminf / the med3 builtin would first canonicalise their inputs (one more
+// v_max per operand); keys of pairs that reach a decision are never NaN, so
+// the plain instructions are exact here.
+__device__ __forceinline__ void top2_neg(float& k1, float& k2, float p) {
+    asm("v_med3_f32 %1, %0, %1, -%2\n\tv_min_f32 %0, %0, -%2" : "+v"(k1), "+v"(k2) : "v"(p));
 }
 
 // Rounds a positive float up by a relative 2^-20 so that bounds computed in
@@ -162,9 +162,13 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
     constexpr int BM = 4 * WR;          // rows per workgroup
     constexpr int BN = 32 * CB;         // columns per chunk
     constexpr int LDS_STRIDE = 136;     // bf16 per LDS row: 128 + 8 pad (272 B)
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
-    __shared__ uint32_t Bn[BN];
-    __shared__ float2 Cs[4][BN];
+    constexpr int PF = BN * 16 / 256;   // 16-byte pieces of a chunk per thread
+    // Two b chunks in LDS: chunk ch + 1 is fetched from global memory into
+    // registers while chunk ch is multiplied, then stored into the other
+    // buffer; one barrier per chunk. The column partials alternate the same way.
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
+    __shared__ uint32_t Bn[2][BN];
+    __shared__ float2 Cs[2][4][BN];
 
     const RowBlock blk = blocks[blockIdx.x];
     if (blk.pair == kNone) return;  // padding of the XCD-interleaved work list
@@ -203,18 +207,38 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
             m2[rb][r] = INFINITY;
         }
 
-    const uint32_t nchunks = (t.nb + BN - 1) / BN;
-    for (uint32_t ch = 0; ch < nchunks; ++ch) {
-        const uint32_t col0 = ch * BN;
-        __syncthreads();
-        // Previous chunk's column partials: merge the 4 waves, store.
-        if (ch > 0 && tid < BN) {
-            const uint32_t col = col0 - BN + tid;
+    // chunk loader: thread tid fetches pieces e = tid + 256 k (column e / 16,
+    // 16-byte segment e % 16) and, for tid < BN, the column's packed norm
+    uint4 pf[PF];
+    uint32_t pfn = kPadNorm;
+    auto fetch = [&](uint32_t col0) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int e = tid + 256 * k, c = e >> 4, seg = e & 15;
+            const uint32_t col = col0 + c;
+            pf[k] = col < t.nb ? *reinterpret_cast<const uint4*>(t.qb + (size_t)col * kDescLen + 8 * seg)
+                               : make_uint4(0, 0, 0, 0);
+        }
+        if (tid < BN) pfn = (col0 + tid < t.nb) ? t.nrm_b[col0 + tid] : kPadNorm;
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int e = tid + 256 * k, c = e >> 4, seg = e & 15;
+            *reinterpret_cast<uint4*>(&Bs[buf][c * LDS_STRIDE + 8 * seg]) = pf[k];
+        }
+        if (tid < BN) Bn[buf][tid] = pfn;
+    };
+    // merge of the 4 waves' column partials of chunk `ch` (buffer ch & 1)
+    auto merge_cols = [&](uint32_t ch) {
+        if (tid < BN) {
+            const uint32_t col = ch * BN + tid;
+            const int cb = ch & 1;
             if (col < t.nb) {
-                float a1 = Cs[0][tid].x, a2 = Cs[0][tid].y;
+                float a1 = Cs[cb][0][tid].x, a2 = Cs[cb][0][tid].y;
 #pragma unroll
                 for (int w = 1; w < 4; ++w) {
-                    float b1 = Cs[w][tid].x, b2 = Cs[w][tid].y;
+                    float b1 = Cs[cb][w][tid].x, b2 = Cs[cb][w][tid].y;
                     float n2 = fminf(fmaxf(a1, b1), fminf(a2, b2));
                     a1 = fminf(a1, b1);
                     a2 = n2;
@@ -223,24 +247,26 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
                     make_float2(a1, a2);
             }
         }
-        // b chunk -> LDS (bf16 rows, padded stride), packed norms.
-        for (int e = tid; e < BN * 16; e += 256) {
-            const int c = e >> 4, seg = e & 15;
-            const uint32_t col = col0 + c;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (col < t.nb) v = *reinterpret_cast<const uint4*>(t.qb + (size_t)col * kDescLen + 8 * seg);
-            *reinterpret_cast<uint4*>(&Bs[c * LDS_STRIDE + 8 * seg]) = v;
-        }
-        if (tid < BN) Bn[tid] = (col0 + tid < t.nb) ? t.nrm_b[col0 + tid] : kPadNorm;
-        __syncthreads();
+    };
+
+    const uint32_t nchunks = (t.nb + BN - 1) / BN;
+    if (nchunks > 0) {
+        fetch(0);
+        store(0);
+    }
+    __syncthreads();
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nchunks) fetch((ch + 1) * BN);
+        if (ch > 0) merge_cols(ch - 1);
 
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
-            const uint16_t* brow = &Bs[(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
+            const uint16_t* brow = &Bs[buf][(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
             bf16x8 bfrag[9];
 #pragma unroll
             for (int s = 0; s < 8; ++s) bfrag[s] = load_frag(brow + 16 * s);
-            bfrag[8] = tail_frag(Bn[cb * 32 + l32], false, lo_half);
+            bfrag[8] = tail_frag(Bn[buf][cb * 32 + l32], false, lo_half);
             const uint32_t tile = (ch * CB + cb) & tmask;
             float c1 = INFINITY, c2 = INFINITY;
 #pragma unroll
@@ -255,10 +281,8 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
                     // key = -(acc with the tile id in its low bits) ~ D/2: one v_and_or_b32
                     // (keep_v is a VGPR so the SGPR tile id fits gfx9's constant bus).
                     const float p = __uint_as_float((__float_as_uint(acc[r]) & keep_v) | tile);
-                    m2[rb][r] = __builtin_amdgcn_fmed3f(m1[rb][r], m2[rb][r], -p);
-                    m1[rb][r] = min_neg(m1[rb][r], p);
-                    c2 = __builtin_amdgcn_fmed3f(c1, c2, -p);
-                    c1 = min_neg(c1, p);
+                    top2_neg(m1[rb][r], m2[rb][r], p);
+                    top2_neg(c1, c2, p);
                 }
             }
             // Merge the two half-waves (same column, other 16*RB rows).
@@ -267,25 +291,12 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
             const float n2 = fminf(fmaxf(c1, o1), fminf(c2, o2));
             c1 = fminf(c1, o1);
             c2 = n2;
-            if (lo_half) Cs[wave][cb * 32 + l32] = make_float2(c1, c2);
+            if (lo_half) Cs[buf][wave][cb * 32 + l32] = make_float2(c1, c2);
         }
+        if (ch + 1 < nchunks) store(buf ^ 1);
+        __syncthreads();
     }
-    __syncthreads();
-    if (nchunks > 0 && tid < BN) {
-        const uint32_t col = (nchunks - 1) * BN + tid;
-        if (col < t.nb) {
-            float a1 = Cs[0][tid].x, a2 = Cs[0][tid].y;
-#pragma unroll
-            for (int w = 1; w < 4; ++w) {
-                float b1 = Cs[w][tid].x, b2 = Cs[w][tid].y;
-                float n2 = fminf(fmaxf(a1, b1), fminf(a2, b2));
-                a1 = fminf(a1, b1);
-                a2 = n2;
-            }
-            out.col_part[(size_t)t.part_base + (size_t)(blk.row0 / BM) * t.nb + col] =
-                make_float2(a1, a2);
-        }
-    }
+    if (nchunks > 0) merge_cols(nchunks - 1);
 
     // Rows: resolve each lane's winning column, merge the 32 lanes of a half.
 #pragma unroll

@@ -25,7 +25,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     if (nb > kLowpMaxCols)
         return ratio_match_dev(d_a, na, d_b, nb, ratio, mutual, SKH_PREC_FP32, d_out_ij, d_out_len);
 
-    const int tile = tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile128x128);
+    const int tile = tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile256x64);
     const uint32_t bm = lowp_block_rows(tile);
     const uint32_t nblk = (uint32_t)((na + bm - 1) / bm);
     const uint32_t tbits = tile_bits(nb);
