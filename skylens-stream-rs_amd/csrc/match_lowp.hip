@@ -370,83 +370,100 @@ __device__ __forceinline__ double pair_eps(const LowpTask& t, uint32_t tbits) {
 }
 
 // Certain decisions; uncertain rows go to the exact queue.
-__global__ void decide_kernel(const LowpTask* __restrict__ tasks, LowpOut out, float ratio,
-                              int mutual, uint32_t tbits) {
-    const LowpTask t = tasks[blockIdx.y];
-    const bool bad = (t.sa->bad | t.sb->bad) != 0;
-    const double eps = pair_eps(t, tbits);
-    const float r2 = ratio * ratio;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.na;
-         i += gridDim.x * blockDim.x) {
-        const uint32_t g = t.row_base + i;
-        uint32_t verdict = kNone;
-        bool queue = false, need_row = true;
-        uint32_t j1 = kNone;
-        if (t.nb < 2) {
-            verdict = kNone;  // no second neighbour (matching.rs:206): never a match
-        } else if (bad) {
-            queue = true;
-        } else {
-            const double e1 = 2.0 * (double)out.row_k1[g];
-            const double e2 = 2.0 * (double)out.row_k2[g];
-            j1 = out.row_j1[g];
-            const bool j1_certain = (e2 - e1) > 2.0 * eps && j1 < t.nb;
-            const float lo1 = __double2float_rd(e1 - eps), hi1 = __double2float_ru(e1 + eps);
-            const float lo2 = __double2float_rd(e2 - eps), hi2 = __double2float_ru(e2 + eps);
-            const bool reject = lo1 >= r2 * hi2;
-            const bool accept = hi1 < r2 * fmaxf(lo2, 0.0f);
-            if (reject) {
-                verdict = kNone;
-            } else if (accept && j1_certain) {
-                if (!mutual) {
-                    verdict = j1;
-                } else {
-                    const float2 c = out.col_top2[t.col_base + j1];
-                    const double c1 = 2.0 * (double)c.x, c2 = 2.0 * (double)c.y;
-                    if (c2 - e1 > 2.0 * eps)
-                        verdict = j1;
-                    else if (e1 - c1 > 2.0 * eps)
-                        verdict = kNone;
-                    else {
-                        queue = true;
-                        need_row = false;
-                    }
-                }
+// Verdict of row i from the approximate top-2 and the pair's error bound eps: verdict =
+// the match or kNone; queue = the exact re-score decides; need_row = it needs the row scan
+// (else only the column check of the certain j1).
+__device__ __forceinline__ void decide_row(const LowpTask& t, const LowpOut& out, bool bad,
+                                           double eps, float r2, int mutual, uint32_t i,
+                                           uint32_t& verdict, bool& queue, bool& need_row,
+                                           uint32_t& j1) {
+    const uint32_t g = t.row_base + i;
+    verdict = kNone;
+    queue = false;
+    need_row = true;
+    j1 = kNone;
+    if (t.nb < 2) {
+        verdict = kNone;  // no second neighbour (matching.rs:206): never a match
+    } else if (bad) {
+        queue = true;
+    } else {
+        const double e1 = 2.0 * (double)out.row_k1[g];
+        const double e2 = 2.0 * (double)out.row_k2[g];
+        j1 = out.row_j1[g];
+        const bool j1_certain = (e2 - e1) > 2.0 * eps && j1 < t.nb;
+        const float lo1 = __double2float_rd(e1 - eps), hi1 = __double2float_ru(e1 + eps);
+        const float lo2 = __double2float_rd(e2 - eps), hi2 = __double2float_ru(e2 + eps);
+        const bool reject = lo1 >= r2 * hi2;
+        const bool accept = hi1 < r2 * fmaxf(lo2, 0.0f);
+        if (reject) {
+            verdict = kNone;
+        } else if (accept && j1_certain) {
+            if (!mutual) {
+                verdict = j1;
             } else {
-                queue = true;
+                const float2 c = out.col_top2[t.col_base + j1];
+                const double c1 = 2.0 * (double)c.x, c2 = 2.0 * (double)c.y;
+                if (c2 - e1 > 2.0 * eps)
+                    verdict = j1;
+                else if (e1 - c1 > 2.0 * eps)
+                    verdict = kNone;
+                else {
+                    queue = true;
+                    need_row = false;
+                }
             }
-        }
-        out.row_match[g] = verdict;
-        if (queue) {
-            const uint32_t slot = atomicAdd(out.queue_len, 1u);
-            out.queue[slot] = make_uint4(blockIdx.y, i, need_row ? kNone : j1, 0u);
+        } else {
+            queue = true;
         }
     }
 }
 
-// Exact squared distance, the reference's formula (8 lanes, no FMA).
-__device__ __forceinline__ float exact_dist(const float* __restrict__ p, const float* __restrict__ q) {
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int c = 0; c < kDescLen; c += 8) {
-        float4 p0 = *reinterpret_cast<const float4*>(p + c);
-        float4 p1 = *reinterpret_cast<const float4*>(p + c + 4);
-        float4 q0 = *reinterpret_cast<const float4*>(q + c);
-        float4 q1 = *reinterpret_cast<const float4*>(q + c + 4);
-        float d;
-        d = p0.x - q0.x; acc[0] = acc[0] + d * d;
-        d = p0.y - q0.y; acc[1] = acc[1] + d * d;
-        d = p0.z - q0.z; acc[2] = acc[2] + d * d;
-        d = p0.w - q0.w; acc[3] = acc[3] + d * d;
-        d = p1.x - q1.x; acc[4] = acc[4] + d * d;
-        d = p1.y - q1.y; acc[5] = acc[5] + d * d;
-        d = p1.z - q1.z; acc[6] = acc[6] + d * d;
-        d = p1.w - q1.w; acc[7] = acc[7] + d * d;
+// One workgroup per pair. Pass 1 writes the certain verdicts and counts the queued rows; one
+// reservation then gives the pair a contiguous run of queue entries, which pass 2 fills (the
+// verdicts are recomputed, identically). The re-score groups consecutive entries, so a pair's
+// queued rows share its streamed rows.
+__global__ __launch_bounds__(256) void decide_kernel(const LowpTask* __restrict__ tasks,
+                                                     LowpOut out, float ratio, int mutual,
+                                                     uint32_t tbits) {
+    __shared__ uint32_t wave_tot[4];
+    __shared__ uint32_t run_base;
+    const LowpTask t = tasks[blockIdx.y];
+    const bool bad = (t.sa->bad | t.sb->bad) != 0;
+    const double eps = pair_eps(t, tbits);
+    const float r2 = ratio * ratio;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t mine = 0;
+    for (uint32_t i = tid; i < t.na; i += blockDim.x) {
+        uint32_t verdict, j1;
+        bool queue, need_row;
+        decide_row(t, out, bad, eps, r2, mutual, i, verdict, queue, need_row, j1);
+        out.row_match[t.row_base + i] = verdict;
+        mine += queue ? 1u : 0u;
     }
-    float s = 0.f;
+    // exclusive prefix of `mine` over the workgroup
+    uint32_t incl = mine;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s = s + acc[k];
-    return s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= (uint32_t)off) incl += o;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+        before += w < wave ? wave_tot[w] : 0u;
+        total += wave_tot[w];
+    }
+    if (tid == 0) run_base = total ? atomicAdd(out.queue_len, total) : 0u;
+    __syncthreads();
+    if (!total) return;
+    uint32_t slot = run_base + before + incl - mine;
+    for (uint32_t i = tid; i < t.na; i += blockDim.x) {
+        uint32_t verdict, j1;
+        bool queue, need_row;
+        decide_row(t, out, bad, eps, r2, mutual, i, verdict, queue, need_row, j1);
+        if (queue) out.queue[slot++] = make_uint4(blockIdx.y, i, need_row ? kNone : j1, 0u);
+    }
 }
 
 struct Top2x {
@@ -471,98 +488,241 @@ __device__ __forceinline__ Top2x merge2(Top2x a, Top2x b) {
 
 // Exact argmin of column j over all rows of a (lowest i among equal d;
 // d = inf and NaN never win, matching.rs:188-203). Returns kNone if none.
-__device__ uint32_t exact_col_argmin(const LowpTask& t, uint32_t j, unsigned long long* red) {
-    __shared__ __attribute__((aligned(16))) float qrow[kDescLen];
-    __syncthreads();
-    if (threadIdx.x < kDescLen) qrow[threadIdx.x] = t.b[(size_t)j * kDescLen + threadIdx.x];
-    __syncthreads();
-    unsigned long long best = kColEmpty;
-    for (uint32_t i = threadIdx.x; i < t.na; i += blockDim.x) {
-        const float d = exact_dist(t.a + (size_t)i * kDescLen, qrow);
-        if (d < INFINITY) {
-            const unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | i;
-            best = key < best ? key : best;
+constexpr int kRescoreGroup = 8;  // queued rows of one pair re-scored together
+
+// Exact squared distances, the reference's formula (matching.rs:146-155: eight running sums,
+// lane k taking components k, k + 8, ..., no FMA, then the eight lanes added in order), from
+// the streamed row x to the first m rows of ys (LDS): d[r] = |x - ys[r]|^2. A difference and
+// its negation square to the same value, so x may be either side.
+__device__ __forceinline__ void exact_dist_group(const float* __restrict__ x,
+                                                 const float (*ys)[kDescLen], int m,
+                                                 float* d) {
+    float acc[kRescoreGroup][8];
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[r][k] = 0.f;
+#pragma unroll 2
+    for (int c = 0; c < kDescLen; c += 8) {
+        const float4 x0 = *reinterpret_cast<const float4*>(x + c);
+        const float4 x1 = *reinterpret_cast<const float4*>(x + c + 4);
+#pragma unroll
+        for (int r = 0; r < kRescoreGroup; ++r) {
+            if (r < m) {
+                const float4 y0 = *reinterpret_cast<const float4*>(&ys[r][c]);
+                const float4 y1 = *reinterpret_cast<const float4*>(&ys[r][c + 4]);
+                float e;
+                e = x0.x - y0.x; acc[r][0] = acc[r][0] + e * e;
+                e = x0.y - y0.y; acc[r][1] = acc[r][1] + e * e;
+                e = x0.z - y0.z; acc[r][2] = acc[r][2] + e * e;
+                e = x0.w - y0.w; acc[r][3] = acc[r][3] + e * e;
+                e = x1.x - y1.x; acc[r][4] = acc[r][4] + e * e;
+                e = x1.y - y1.y; acc[r][5] = acc[r][5] + e * e;
+                e = x1.z - y1.z; acc[r][6] = acc[r][6] + e * e;
+                e = x1.w - y1.w; acc[r][7] = acc[r][7] + e * e;
+            }
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(best, off);
-        best = o < best ? o : best;
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r) {
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s = s + acc[r][k];
+        d[r] = s;
     }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
-    __syncthreads();
-    unsigned long long b = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[w] < b ? red[w] : b;
-    __syncthreads();
-    return b == kColEmpty ? kNone : (uint32_t)(b & 0xffffffffu);
 }
 
-// Exact row top-2 of row i against all of b (matching.rs:182-187).
-__device__ Top2x exact_row_top2(const LowpTask& t, uint32_t i, Top2x* red) {
-    __shared__ __attribute__((aligned(16))) float arow[kDescLen];
-    __syncthreads();
-    if (threadIdx.x < kDescLen) arow[threadIdx.x] = t.a[(size_t)i * kDescLen + threadIdx.x];
-    __syncthreads();
-    Top2x st{kNone, INFINITY, INFINITY};
+// Exact row top-2 (matching.rs:182-187) of the m rows in ys against all of b: thread k
+// takes columns k, k + 256, ... in order, then the 256 states merge (lanes, then waves).
+__device__ void exact_row_top2_group(const LowpTask& t, const float (*ys)[kDescLen], int m,
+                                     Top2x (*red)[4], Top2x* res) {
+    Top2x st[kRescoreGroup];
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r) st[r] = Top2x{kNone, INFINITY, INFINITY};
     for (uint32_t j = threadIdx.x; j < t.nb; j += blockDim.x) {
-        const float d = exact_dist(arow, t.b + (size_t)j * kDescLen);
-        if (d < st.d1) {
-            st.d2 = st.d1;
-            st.d1 = d;
-            st.j = j;
-        } else if (d < st.d2) {
-            st.d2 = d;
+        float d[kRescoreGroup];
+        exact_dist_group(t.b + (size_t)j * kDescLen, ys, m, d);
+#pragma unroll
+        for (int r = 0; r < kRescoreGroup; ++r) {
+            if (d[r] < st[r].d1) {
+                st[r].d2 = st[r].d1;
+                st[r].d1 = d[r];
+                st[r].j = j;
+            } else if (d[r] < st[r].d2) {
+                st[r].d2 = d[r];
+            }
         }
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        Top2x o;
-        o.j = __shfl_xor(st.j, off);
-        o.d1 = __shfl_xor(st.d1, off);
-        o.d2 = __shfl_xor(st.d2, off);
-        st = merge2(st, o);
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r) {
+        if (r >= m) break;
+        Top2x x = st[r];
+        for (int off = 32; off >= 1; off >>= 1) {
+            Top2x o;
+            o.j = __shfl_xor(x.j, off);
+            o.d1 = __shfl_xor(x.d1, off);
+            o.d2 = __shfl_xor(x.d2, off);
+            x = merge2(x, o);
+        }
+        if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = x;
     }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = st;
     __syncthreads();
-    Top2x r = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = merge2(r, red[w]);
+    for (int r = 0; r < m; ++r) {
+        Top2x x = red[r][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x = merge2(x, red[r][w]);
+        res[r] = x;
+    }
     __syncthreads();
-    return r;
 }
 
+// Exact column argmin (lowest row index among equal distances) of the m b rows in ys over all
+// of a; kNone when every distance is infinite.
+__device__ void exact_col_argmin_group(const LowpTask& t, const float (*ys)[kDescLen], int m,
+                                       unsigned long long (*red)[4], uint32_t* res) {
+    unsigned long long best[kRescoreGroup];
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r) best[r] = kColEmpty;
+    for (uint32_t i = threadIdx.x; i < t.na; i += blockDim.x) {
+        float d[kRescoreGroup];
+        exact_dist_group(t.a + (size_t)i * kDescLen, ys, m, d);
+#pragma unroll
+        for (int r = 0; r < kRescoreGroup; ++r) {
+            if (d[r] < INFINITY) {
+                const unsigned long long key = ((unsigned long long)__float_as_uint(d[r]) << 32) | i;
+                best[r] = key < best[r] ? key : best[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r) {
+        if (r >= m) break;
+        unsigned long long b = best[r];
+        for (int off = 32; off >= 1; off >>= 1) {
+            const unsigned long long o = __shfl_xor(b, off);
+            b = o < b ? o : b;
+        }
+        if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    for (int r = 0; r < m; ++r) {
+        unsigned long long b = red[r][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = red[r][w] < b ? red[r][w] : b;
+        res[r] = b == kColEmpty ? kNone : (uint32_t)(b & 0xffffffffu);
+    }
+    __syncthreads();
+}
+
+// Queued rows, kRescoreGroup consecutive queue entries per step; consecutive entries of one
+// pair (the decide kernel reserves a wave's entries together) share each streamed row of b
+// (row scans) or of a (column scans).
 __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict__ tasks,
                                                       LowpOut out, float ratio, int mutual,
                                                       uint32_t tbits) {
-    __shared__ Top2x red_t[4];
-    __shared__ unsigned long long red_c[4];
+    __shared__ __attribute__((aligned(16))) float ys[kRescoreGroup][kDescLen];
+    __shared__ Top2x red_t[kRescoreGroup][4];
+    __shared__ unsigned long long red_c[kRescoreGroup][4];
+    __shared__ Top2x row_res[kRescoreGroup];
+    __shared__ uint32_t col_res[kRescoreGroup];
     const float r2 = ratio * ratio;
     const uint32_t n = *out.queue_len;
-    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
-        const uint4 item = out.queue[q];
-        const LowpTask t = tasks[item.x];
-        const uint32_t i = item.y;
-        const bool bad = (t.sa->bad | t.sb->bad) != 0;
-        const double eps = pair_eps(t, tbits);
-        uint32_t j1 = item.z;
-        bool accept;
-        bool col_needed = true;
-        if (j1 == kNone) {
-            const Top2x st = exact_row_top2(t, i, red_t);
-            j1 = st.j;
-            accept = st.j != kNone && isfinite(st.d2) && !(st.d1 >= r2 * st.d2);
-            if (accept && mutual && !bad) {
-                const float2 c = out.col_top2[t.col_base + j1];
-                const double d1 = st.d1;
-                if ((2.0 * (double)c.y) - d1 > eps) {
-                    col_needed = false;  // row i is the unique column winner
-                } else if (d1 - 2.0 * (double)c.x > eps) {
-                    accept = false;  // another row is strictly closer
-                    col_needed = false;
+    for (uint32_t base = blockIdx.x * kRescoreGroup; base < n;
+         base += gridDim.x * kRescoreGroup) {
+        const uint32_t kend = min(base + (uint32_t)kRescoreGroup, n);
+        uint32_t k0 = base;
+        while (k0 < kend) {
+            // the run [k0, k1) of entries of one pair
+            const uint32_t pair = out.queue[k0].x;
+            uint32_t k1 = k0 + 1;
+            while (k1 < kend && out.queue[k1].x == pair) ++k1;
+            const LowpTask t = tasks[pair];
+            const bool bad = (t.sa->bad | t.sb->bad) != 0;
+            const double eps = pair_eps(t, tbits);
+            const int m = (int)(k1 - k0);
+            uint32_t row_i[kRescoreGroup], j1[kRescoreGroup];
+            bool accept[kRescoreGroup], col_needed[kRescoreGroup];
+            int row_slot[kRescoreGroup];
+            int mr = 0;
+#pragma unroll
+            for (int r = 0; r < kRescoreGroup; ++r) {
+                row_i[r] = kNone;
+                j1[r] = kNone;
+                accept[r] = false;
+                col_needed[r] = false;
+                row_slot[r] = -1;
+                if (r < m) {
+                    const uint4 item = out.queue[k0 + r];
+                    row_i[r] = item.y;
+                    j1[r] = item.z;
+                    col_needed[r] = true;
+                    if (item.z == kNone) row_slot[r] = mr++;
+                    else accept[r] = true;  // ratio accepted with certainty, j1 certain
                 }
             }
-        } else {
-            accept = true;  // ratio accepted with certainty, j1 certain
+            // row scans of the entries that need one
+            if (mr > 0) {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kRescoreGroup; ++r)
+                    if (row_slot[r] >= 0 && threadIdx.x < kDescLen)
+                        ys[row_slot[r]][threadIdx.x] = t.a[(size_t)row_i[r] * kDescLen + threadIdx.x];
+                __syncthreads();
+                exact_row_top2_group(t, ys, mr, red_t, row_res);
+#pragma unroll
+                for (int r = 0; r < kRescoreGroup; ++r) {
+                    if (row_slot[r] < 0) continue;
+                    const Top2x st = row_res[row_slot[r]];
+                    j1[r] = st.j;
+                    accept[r] = st.j != kNone && isfinite(st.d2) && !(st.d1 >= r2 * st.d2);
+                    if (accept[r] && mutual && !bad) {
+                        const float2 c = out.col_top2[t.col_base + st.j];
+                        const double d1 = st.d1;
+                        if ((2.0 * (double)c.y) - d1 > eps) {
+                            col_needed[r] = false;  // row i is the unique column winner
+                        } else if (d1 - 2.0 * (double)c.x > eps) {
+                            accept[r] = false;  // another row is strictly closer
+                            col_needed[r] = false;
+                        }
+                    }
+                }
+            }
+            // column scans of the accepted entries whose mutual check is open (one per
+            // distinct j1)
+            int col_slot[kRescoreGroup];
+            bool col_first[kRescoreGroup];
+            int mc = 0;
+#pragma unroll
+            for (int r = 0; r < kRescoreGroup; ++r) {
+                col_slot[r] = -1;
+                col_first[r] = false;
+                if (r < m && accept[r] && mutual && col_needed[r]) {
+#pragma unroll
+                    for (int q = 0; q < r; ++q)
+                        if (col_slot[r] < 0 && col_slot[q] >= 0 && j1[q] == j1[r]) col_slot[r] = col_slot[q];
+                    if (col_slot[r] < 0) {
+                        col_slot[r] = mc++;
+                        col_first[r] = true;
+                    }
+                }
+            }
+            if (mc > 0) {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kRescoreGroup; ++r)
+                    if (col_first[r] && threadIdx.x < kDescLen)
+                        ys[col_slot[r]][threadIdx.x] = t.b[(size_t)j1[r] * kDescLen + threadIdx.x];
+                __syncthreads();
+                exact_col_argmin_group(t, ys, mc, red_c, col_res);
+#pragma unroll
+                for (int r = 0; r < kRescoreGroup; ++r)
+                    if (col_slot[r] >= 0) accept[r] = col_res[col_slot[r]] == row_i[r];
+            }
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int r = 0; r < kRescoreGroup; ++r)
+                    if (r < m) out.row_match[t.row_base + row_i[r]] = accept[r] ? j1[r] : kNone;
+            }
+            k0 = k1;
         }
-        if (accept && mutual && col_needed) accept = exact_col_argmin(t, j1, red_c) == i;
-        if (threadIdx.x == 0) out.row_match[t.row_base + i] = accept ? j1 : kNone;
     }
 }
 
@@ -621,7 +781,7 @@ void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
                    int mutual, uint32_t tbits, hipStream_t s) {
     if (!n_pairs) return;
-    hipLaunchKernelGGL(decide_kernel, dim3(8, n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
+    hipLaunchKernelGGL(decide_kernel, dim3(1, n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
                        tbits);
 }
 
