@@ -328,16 +328,37 @@ __device__ __forceinline__ double stage_dot(int u, int w, uint32_t cnt,
     return acc;
 }
 
+// The points of one index list (up to kCacheN of them) kept in the wave's LDS between
+// passes: the first pass over a list gathers them from global memory and fills the cache,
+// the passes that follow over the same list read it (no gathers, no L2 round trips).
+constexpr uint32_t kCacheN = 128;
+struct PtCache {
+    double2 p[kCacheN], q[kCacheN];
+};
+__shared__ PtCache s_pt[kWaves];
+enum CacheMode { kNoCache = 0, kFill = 1, kUse = 2 };
+
 // One pass over the points list[0..len) in chunks of NL: stage(slot, p, q)
 // for each lane holding a point of the chunk (slot = lane), then, once the
-// wave has staged, reduce(cnt) over the chunk's cnt points. The next chunk's
-// points and the indices two chunks ahead are loaded while a chunk is
-// processed.
+// wave has staged, reduce(cnt) over the chunk's cnt points. Without the cache,
+// the next chunk's points and the indices two chunks ahead are loaded while a
+// chunk is processed. kFill / kUse require len <= kCacheN.
 template <class Stage, class Reduce>
 __device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
                                           const uint32_t* list, uint32_t len, Stage&& stage,
-                                          Reduce&& reduce) {
+                                          Reduce&& reduce, CacheMode mode = kNoCache) {
     const uint32_t lane = (uint32_t)lane_id();
+    PtCache& pc = s_pt[wave_id()];
+    if (mode == kUse) {
+        for (uint32_t base = 0; base < len; base += NL) {
+            const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
+            if (lane < cnt) stage(lane, pc.p[base + lane], pc.q[base + lane]);
+            wave_sync();
+            reduce(cnt);
+            wave_sync();
+        }
+        return;
+    }
     double2 p = make_double2(0.0, 0.0), q = p;
     if (lane < len) {
         const uint32_t i0 = list[lane];
@@ -354,7 +375,13 @@ __device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
         }
         const uint32_t j2 = base + 2 * NL + lane;
         const uint32_t inext2 = j2 < len ? list[j2] : 0u;
-        if (lane < cnt) stage(lane, p, q);
+        if (lane < cnt) {
+            if (mode == kFill) {
+                pc.p[base + lane] = p;
+                pc.q[base + lane] = q;
+            }
+            stage(lane, p, q);
+        }
         wave_sync();
         reduce(cnt);
         wave_sync();
@@ -455,10 +482,11 @@ __device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
 
 // collinear_and_normalizers over the bitset `sel` (len points): fills the
 // Hartley normalisers t1, t2 and returns whether either image is nearly
-// collinear.
+// collinear. Leaves the points in the wave's point cache when len <= kCacheN.
 __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* list,
                           uint32_t len, double* t1, double* t2) {
     const int lane = lane_id();
+    const bool cached = len <= kCacheN;
     double acc = 0.0;
     list_pass(x1, x2, list, len,
         [&](uint32_t slot, double2 a, double2 b) {
@@ -469,7 +497,8 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
         },
         [&](uint32_t cnt) {
             acc = stage_sum(lane & 3, cnt, acc);
-        });
+        },
+        cached ? kFill : kNoCache);
     const double nc = (double)(len > 0 ? len : 1);
     const double c1x = readlane_d(acc, 0) / nc, c1y = readlane_d(acc, 1) / nc;
     const double c2x = readlane_d(acc, 2) / nc, c2y = readlane_d(acc, 3) / nc;
@@ -488,7 +517,8 @@ __device__ bool coop_norm(const double2* x1, const double2* x2, const uint32_t* 
         },
         [&](uint32_t cnt) {
             acc = stage_sum(lane & 7, cnt, acc);
-        });
+        },
+        cached ? kUse : kNoCache);
     double q[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) q[k] = readlane_d(acc, k);
@@ -521,7 +551,8 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
         },
         [&](uint32_t cnt) {
             acc = stage_dot(r, c, cnt, acc);
-        });
+        },
+        len <= kCacheN ? kUse : kNoCache);
     return acc;
 }
 
@@ -554,7 +585,8 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
         },
         [&](uint32_t cnt) {
             cost = stage_sum(0, cnt, cost);
-        });
+        },
+        len <= kCacheN ? kUse : kNoCache);
     return cost;
 }
 
@@ -590,7 +622,8 @@ __device__ double coop_lm_costs(double s1, double s2, const double* t1, const do
         },
         [&](uint32_t cnt) {
             cost = stage_sum(lane & (kLmSpec - 1), cnt, cost);
-        });
+        },
+        len <= kCacheN ? kUse : kNoCache);
     return cost;
 }
 
@@ -639,7 +672,8 @@ __device__ double coop_normal_eq(const double* g, double s1, double s2,
         },
         [&](uint32_t cnt) {
             acc = stage_dot(u, w, cnt, acc);
-        });
+        },
+        len <= kCacheN ? kUse : kNoCache);
     return acc;
 }
 
@@ -866,7 +900,8 @@ __device__ bool coop_homography_dlt(const double2* x1, const double2* x2, uint32
                 const double* row = coop().stage[j];
                 acc = acc + (row[a1] * row[b1] + row[a2] * row[b2]);
             }
-        });
+        },
+        len <= kCacheN ? kUse : kNoCache);
     double ata[45];
 #pragma unroll
     for (int k = 0; k < 45; ++k) ata[k] = readlane_d(acc, k);
