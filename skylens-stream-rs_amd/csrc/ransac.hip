@@ -469,13 +469,24 @@ __device__ uint32_t coop_count_of(const double* F, const double2* x1, const doub
 }
 
 // rank -> index of the points of an n-bit bitset, in index order.
+// The bitset's words are loaded 64 at a time, one per lane, and handed to the
+// chunks through readlane: one load latency per 2048 points.
 __device__ void coop_list(const uint32_t* bits, uint32_t n, uint32_t* list) {
     const int lane = lane_id();
+    const uint32_t W = words(n);
     uint32_t done = 0;
-    for (uint32_t base = 0; base < n; base += NL) {
-        const uint64_t m = bits64(bits, base, n);
-        if ((m >> lane) & 1u) list[done + (uint32_t)__popcll(m & lanes_below())] = base + lane;
-        done += (uint32_t)__popcll(m);
+    for (uint32_t w0 = 0; w0 < W; w0 += NL) {
+        const uint32_t mine = w0 + (uint32_t)lane < W ? bits[w0 + lane] : 0u;
+        const uint32_t wend = W - w0 < (uint32_t)NL ? W - w0 : (uint32_t)NL;
+        for (uint32_t c = 0; c < wend; c += 2) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)c);
+            const uint32_t hi =
+                c + 1 < wend ? (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)c + 1) : 0u;
+            const uint64_t m = (uint64_t)lo | ((uint64_t)hi << 32);
+            if ((m >> lane) & 1u)
+                list[done + (uint32_t)__popcll(m & lanes_below())] = (w0 + c) * 32u + (uint32_t)lane;
+            done += (uint32_t)__popcll(m);
+        }
     }
     wave_sync();
 }
