@@ -12,6 +12,13 @@ equal the exact path's; the plain FP32 GPU path is timed beside it, and the
 CPU oracle (the restated reference algorithm) is timed on a bounded sample of
 the same pairs on the host cores.
 
+Steps are issued as a stream of batches would be: two plans over the same
+inputs on two HIP streams, consecutive steps alternating between them, so
+that one step's matching fills the CUs the previous step's RANSAC frees as
+its last pairs finish. Every step is a complete run over the whole batch;
+the latency of one isolated step is reported beside the throughput, and both
+plans' results are checked to be identical.
+
 Prints one JSON line (see the field notes in the repository documentation).
 """
 import argparse
@@ -137,68 +144,94 @@ def main():
     log(f"workload {args.workload}: {len(sc.views)} views, {P} pairs, "
         f"{sc.offsets[-1]} features, built in {time.time() - t0:.1f}s")
 
-    ctx = skylens_hip.Context(0)
+    # Two contexts (two streams) each hold one plan over the same views and pairs; consecutive
+    # steps alternate between them, so that step k + 1's matching runs on the CUs that step k's
+    # RANSAC frees while its last pairs finish. Each step is a complete run of the path over the
+    # whole batch; a plan's own runs stay ordered on its stream.
+    ctxs = [skylens_hip.Context(0), skylens_hip.Context(0)]
     d_desc = torch.from_numpy(sc.desc).cuda()
     d_kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
     torch.cuda.synchronize()
 
-    def make_plan(prec):
-        p = skylens_hip.Plan(ctx, sc.offsets, d_desc.data_ptr(), d_kp.data_ptr(), sc.pairs, 0.8,
-                             True, prec, True, 0.0)
-        p.set_timing(True)
-        return p
+    def make_plans(prec, offsets, dd, dk, pairs):
+        return [skylens_hip.Plan(c, offsets, dd.data_ptr(), dk.data_ptr(), pairs, 0.8, True, prec,
+                                 True, 0.0) for c in ctxs]
 
-    def timed(plan, steps, warmup):
+    def timed(plans, steps, warmup):
+        """(seconds for `steps` steps, distance-kernel ms, match ms, RANSAC ms, isolated step ms)."""
         for _ in range(warmup):
-            plan.run()
-        plan.sync()
+            for p in plans:
+                p.run()
+        for p in plans:
+            p.sync()
         torch.cuda.synchronize()
-        match_ms, ransac_ms = [], []
         t = time.perf_counter()
-        for _ in range(steps):
-            plan.run()
-            plan.sync()
-            match_ms.append(plan.stage_ms(1))
-            ransac_ms.append(plan.stage_ms(2))
+        for k in range(steps):
+            plans[k % len(plans)].run()
+        for p in plans:
+            p.sync()
         torch.cuda.synchronize()
         el = time.perf_counter() - t
-        # the distance kernel alone, on the same inputs and stream, HIP events
-        kern = plan.time_distance(3)
-        return el, kern, float(np.mean(match_ms)), float(np.mean(ransac_ms))
+        # outside the timed region: one isolated step (latency), then one with stage events for
+        # the matching / RANSAC split, then the distance kernel alone (HIP events, plan stream)
+        p = plans[0]
+        t = time.perf_counter()
+        p.run()
+        p.sync()
+        iso = (time.perf_counter() - t) * 1e3
+        p.set_timing(True)
+        p.run()
+        p.sync()
+        match_ms, ransac_ms = p.stage_ms(1), p.stage_ms(2)
+        p.set_timing(False)
+        kern = p.time_distance(3)
+        return el, kern, match_ms, ransac_ms, iso
 
-    plan = make_plan(args.precision)
-    log(f"plan {args.precision} built")
+    plans = make_plans(args.precision, sc.offsets, d_desc, d_kp, sc.pairs)
+    log(f"plans {args.precision} built")
     if args.kernel_only:
+        plan = plans[0]
         plan.run()
         plan.sync()
         log(f"distance kernel {plan.time_distance(3):.3f} ms")
-        plan.close()
-        ctx.close()
+        for p in plans:
+            p.close()
+        for c in ctxs:
+            c.close()
         return
-    el, kern_ms, match_ms, ransac_ms = timed(plan, args.steps, args.warmup)
-    res = plan.results()
+    el, kern_ms, match_ms, ransac_ms, iso_ms = timed(plans, args.steps, args.warmup)
+    res = plans[0].results()
+    same_plans = same_results(res, plans[1].results(), P)
     ms_step = el / args.steps * 1e3
     value = P * args.steps / el
     n_found = int(np.sum(res.found))
     n_matches = int(np.sum(res.match_len))
     log(f"{args.precision}: {ms_step:.2f} ms/step ({match_ms:.2f} match, {ransac_ms:.2f} ransac, "
-        f"distance kernel {kern_ms:.2f}), {value:.0f} pairs/s, {n_matches} matches, "
-        f"{n_found}/{P} verified")
+        f"distance kernel {kern_ms:.2f}; isolated step {iso_ms:.2f} ms), {value:.0f} pairs/s, "
+        f"{n_matches} matches, {n_found}/{P} verified, plans agree {same_plans}")
 
-    extra = {}
+    extra = {"steps_issue": "consecutive steps alternate between two plans on two streams",
+             "isolated_step_ms": round(iso_ms, 3), "plans_agree": bool(same_plans)}
     if args.precision != "fp32" and args.fp32_steps > 0:
-        plan32 = make_plan("fp32")
-        el32, k32, m32, r32 = timed(plan32, args.fp32_steps, 1)
-        r32res = plan32.results()
+        plans32 = make_plans("fp32", sc.offsets, d_desc, d_kp, sc.pairs)
+        el32, k32, m32, r32, iso32 = timed(plans32, args.fp32_steps, 1)
+        r32res = plans32[0].results()
         same = same_results(r32res, res, P)
         v32 = P * args.fp32_steps / el32
-        extra = {"fp32_gpu_pairs_per_s": round(v32, 2), "speedup_vs_fp32_gpu": round(value / v32, 3),
-                 "fp32_ms_per_step": round(el32 / args.fp32_steps * 1e3, 2),
-                 "fp32_distance_kernel_ms": round(k32, 3),
-                 "identical_to_fp32_gpu": bool(same)}
+        extra.update({"fp32_gpu_pairs_per_s": round(v32, 2),
+                      "speedup_vs_fp32_gpu": round(value / v32, 3),
+                      "fp32_ms_per_step": round(el32 / args.fp32_steps * 1e3, 2),
+                      "fp32_isolated_step_ms": round(iso32, 3),
+                      "fp32_distance_kernel_ms": round(k32, 3),
+                      "identical_to_fp32_gpu": bool(same)})
         log(f"fp32: {el32 / args.fp32_steps * 1e3:.2f} ms/step ({m32:.2f} match, {r32:.2f} ransac, "
-            f"exact kernel {k32:.2f}), {v32:.0f} pairs/s, identical={same}")
-        plan32.close()
+            f"exact kernel {k32:.2f}; isolated step {iso32:.2f} ms), {v32:.0f} pairs/s, "
+            f"identical={same}")
+        for p in plans32:
+            p.close()
+
+    for p in plans:
+        p.close()
 
     if not args.no_uneven and args.workload != "uneven":
         # The 4000-pair uneven batch (view sizes log-uniform in [200, 8000]), on which the
@@ -207,18 +240,19 @@ def main():
         du = torch.from_numpy(su.desc).cuda()
         ku = torch.from_numpy(np.ascontiguousarray(su.kp)).cuda()
         res_u = {}
-        for prec in (args.precision, "fp32"):
-            pu = skylens_hip.Plan(ctx, su.offsets, du.data_ptr(), ku.data_ptr(), su.pairs, 0.8,
-                                  True, prec, True, 0.0)
-            pu.set_timing(True)
-            el_u, _, m_u, r_u = timed(pu, 2, 1)
-            res_u[prec] = (len(su.pairs) * 2 / el_u, m_u, r_u, pu.results())
-            pu.close()
+        for prec, steps in ((args.precision, 4), ("fp32", 2)):
+            pu = make_plans(prec, su.offsets, du, ku, su.pairs)
+            el_u, _, m_u, r_u, iso_u = timed(pu, steps, 1)
+            res_u[prec] = (len(su.pairs) * steps / el_u, m_u, r_u, pu[0].results(), iso_u)
+            for p in pu:
+                p.close()
         same_u = same_results(res_u[args.precision][3], res_u["fp32"][3], len(su.pairs))
         extra["uneven_batch"] = {
             "pairs": len(su.pairs), f"{args.precision}_pairs_per_s": round(res_u[args.precision][0], 2),
             "fp32_pairs_per_s": round(res_u["fp32"][0], 2),
             "speedup_vs_fp32_gpu": round(res_u[args.precision][0] / res_u["fp32"][0], 3),
+            f"{args.precision}_isolated_step_ms": round(res_u[args.precision][4], 3),
+            "fp32_isolated_step_ms": round(res_u["fp32"][4], 3),
             f"{args.precision}_match_ms": round(res_u[args.precision][1], 3),
             "fp32_match_ms": round(res_u["fp32"][1], 3),
             "ransac_ms": round(res_u[args.precision][2], 3), "identical_to_fp32_gpu": bool(same_u)}
@@ -266,8 +300,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    plan.close()
-    ctx.close()
+    for c in ctxs:
+        c.close()
 
 
 if __name__ == "__main__":
