@@ -110,3 +110,33 @@ def test_batch_empty_and_tiny_views(ctx):
             b = desc[offsets[j]:offsets[j + 1]]
             assert [tuple(map(int, x)) for x in r.matches(p)] == ob.ratio_match(a, b, 0.8, True)
         plan.close()
+
+
+def test_two_plans_interleaved(ctx):
+    """Two plans on two contexts (two streams), runs issued alternately with no
+    synchronisation in between (the way bench.py issues steps): every run's
+    results equal an isolated run's, byte for byte."""
+    import torch
+
+    sc = _scene(11, 8, 150, 900)
+    prec = skylens_hip.available_precisions()[-1]
+    (ref,) = _run_plan(ctx, sc, prec, steps=1)
+    desc = torch.from_numpy(sc.desc).cuda()
+    kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
+    ctx2 = skylens_hip.Context(0)
+    plans = [skylens_hip.Plan(c, sc.offsets, desc.data_ptr(), kp.data_ptr(), sc.pairs, 0.8, True,
+                              prec) for c in (ctx, ctx2)]
+    for k in range(6):
+        plans[k % 2].run()
+    P = len(sc.pairs)
+    for plan in plans:
+        r = plan.results()
+        assert np.array_equal(r.match_len, ref.match_len)
+        assert all(np.array_equal(r.matches(p), ref.matches(p)) for p in range(P))
+        assert np.array_equal(r.found, ref.found)
+        for p in range(P):
+            if ref.found[p] == 1:
+                assert np.array_equal(r.F[p].view(np.uint64), ref.F[p].view(np.uint64)), f"pair {p}"
+                assert np.array_equal(r.inlier_mask(p), ref.inlier_mask(p)), f"pair {p}"
+        plan.close()
+    ctx2.close()
