@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--workload", default="full", choices=["full", "uneven", "fixed"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="plans (each on its own context and stream) that consecutive steps "
+                         "alternate between; 1 for profiler runs that need isolated kernels")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true",
@@ -148,7 +151,7 @@ def main():
     # steps alternate between them, so that step k + 1's matching runs on the CUs that step k's
     # RANSAC frees while its last pairs finish. Each step is a complete run of the path over the
     # whole batch; a plan's own runs stay ordered on its stream.
-    ctxs = [skylens_hip.Context(0), skylens_hip.Context(0)]
+    ctxs = [skylens_hip.Context(0) for _ in range(args.streams)]
     d_desc = torch.from_numpy(sc.desc).cuda()
     d_kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
     torch.cuda.synchronize()
@@ -201,7 +204,7 @@ def main():
         return
     el, kern_ms, match_ms, ransac_ms, iso_ms = timed(plans, args.steps, args.warmup)
     res = plans[0].results()
-    same_plans = same_results(res, plans[1].results(), P)
+    same_plans = same_results(res, plans[-1].results(), P)
     ms_step = el / args.steps * 1e3
     value = P * args.steps / el
     n_found = int(np.sum(res.found))
@@ -210,7 +213,8 @@ def main():
         f"distance kernel {kern_ms:.2f}; isolated step {iso_ms:.2f} ms), {value:.0f} pairs/s, "
         f"{n_matches} matches, {n_found}/{P} verified, plans agree {same_plans}")
 
-    extra = {"steps_issue": "consecutive steps alternate between two plans on two streams",
+    extra = {"steps_issue": ("consecutive steps alternate between two plans on two streams"
+                             if len(ctxs) == 2 else "one plan, one stream"),
              "isolated_step_ms": round(iso_ms, 3), "plans_agree": bool(same_plans)}
     if args.precision != "fp32" and args.fp32_steps > 0:
         plans32 = make_plans("fp32", sc.offsets, d_desc, d_kp, sc.pairs)
