@@ -247,11 +247,21 @@ static int cmp_double(const void* x, const void* y) {
 
 /* One-sided Jacobi SVD of F (3x3): W = F V with orthogonal columns, then the
  * column of least norm is dropped and F' = W' V^T (matching.rs:550-553,
- * :577-580: zero the smallest singular value, recompose). */
+ * :577-580: zero the smallest singular value, recompose). The rotations start
+ * from the identity (V0 = NULL: W = F) or from an orthogonal basis V0
+ * (row-major, column j = basis vector j: W = F V0, V = V0), e.g. the basis of
+ * a nearby matrix, which leaves fewer sweeps; Vout (if set) receives V. */
 #define SVD3_MAX_SWEEPS 30
-static void rank2_recompose(const double* F, double* out) {
+static void rank2_recompose_v(const double* F, const double* V0, double* out, double* Vout) {
     double w[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    memcpy(w, F, sizeof w);
+    if (V0) {
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j)
+                w[3 * r + j] = F[3 * r] * V0[j] + F[3 * r + 1] * V0[3 + j] + F[3 * r + 2] * V0[6 + j];
+        memcpy(v, V0, sizeof v);
+    } else {
+        memcpy(w, F, sizeof w);
+    }
     static const int PP[3] = {0, 0, 1}, QQ[3] = {1, 2, 2};
     for (int sweep = 0; sweep < SVD3_MAX_SWEEPS; ++sweep) {
         int rotated = 0;
@@ -280,6 +290,7 @@ static void rank2_recompose(const double* F, double* out) {
         }
         if (!rotated) break;
     }
+    if (Vout) memcpy(Vout, v, sizeof v);
     double sig[3];
     for (int c = 0; c < 3; ++c) sig[c] = w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c];
     int m = 0;
@@ -296,15 +307,18 @@ static void rank2_recompose(const double* F, double* out) {
         }
 }
 
-/* matching.rs:576-583. */
-static int rank2_unit(const double* f, double* out) {
+static void rank2_recompose(const double* F, double* out) { rank2_recompose_v(F, NULL, out, NULL); }
+
+/* matching.rs:576-583 (V0 / Vout as in rank2_recompose_v). */
+static int rank2_unit_v(const double* f, const double* V0, double* out, double* Vout) {
     double g[9];
-    rank2_recompose(f, g);
+    rank2_recompose_v(f, V0, g, Vout);
     double n = fro3(g);
     if (!(isfinite(n) && n > 0.0)) return 0;
     for (int i = 0; i < 9; ++i) out[i] = g[i] / n;
     return 1;
 }
+
 
 /* --------------------------------------------------------------- geometry */
 
@@ -508,7 +522,10 @@ int orc_refine_sampson(const double* F, const double* x1, const double* x2, size
     tr3(t2i, t2it);
     mm3(t2it, F, tmp);
     mm3(tmp, t1i, g0);
-    if (!rank2_unit(g0, g)) { free(a); free(b); return 0; }
+    /* V: the right singular basis of g; each damped step's rank-2 projection starts its
+     * Jacobi rotations from it (g + d is close to g) */
+    double V[9];
+    if (!rank2_unit_v(g0, NULL, g, V)) { free(a); free(b); return 0; }
     double cost = sampson_cost(g, s1, s2, a, b, n);
     double lambda = 1e-3;
     uint64_t taken = 0;
@@ -537,11 +554,13 @@ int orc_refine_sampson(const double* F, const double* x1, const double* x2, size
             if (!cholesky_solve9(m, nrhs, d)) { lambda *= 10.0; continue; }
             double gs[9], cand[9];
             for (int k = 0; k < 9; ++k) gs[k] = g[k] + d[k];
-            if (!rank2_unit(gs, cand)) { lambda *= 10.0; continue; }
+            double Vc[9];
+            if (!rank2_unit_v(gs, V, cand, Vc)) { lambda *= 10.0; continue; }
             double c = sampson_cost(cand, s1, s2, a, b, n);
             g_lm_evals++;
             if (c < cost) {
                 memcpy(g, cand, sizeof g);
+                memcpy(V, Vc, sizeof V);
                 double rel = (cost - c) / (cost > 1e-300 ? cost : 1e-300);
                 cost = c;
                 lambda = lambda * 0.1 > 1e-9 ? lambda * 0.1 : 1e-9;

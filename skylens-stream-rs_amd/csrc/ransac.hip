@@ -707,7 +707,10 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
     tr3(t2i, t2it);
     mm3(t2it, F, tmp);
     mm3(tmp, t1i, g0);
-    if (!rank2_unit(g0, g)) return;
+    // V: the right singular basis of g; each damped step's rank-2 projection starts its
+    // Jacobi rotations from it (g + d is close to g)
+    double V[9];
+    if (!rank2_unit_v(g0, nullptr, g, V)) return;
     double cost = coop_lm_cost(g, s1, s2, t1, t2, x1, x2, list, len);
     double lambda = 1e-3;
     const int lane = lane_id();
@@ -731,7 +734,7 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
         for (int i = 0; i < 45; ++i) m[i] = jtj[i];
 #pragma unroll
         for (int i = 0; i < 9; ++i) m[ut(i, i)] = m[ut(i, i)] + lam * (jtj[ut(i, i)] + 1e-12);
-        double nrhs[9], d[9], cand[9];
+        double nrhs[9], d[9], cand[9], Vc[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) nrhs[i] = -jtr[i];
         bool ok = cholesky_solve9(m, nrhs, d);
@@ -739,7 +742,7 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
             double gs[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) gs[i] = g[i] + d[i];
-            ok = rank2_unit(gs, cand);
+            ok = rank2_unit_v(gs, V, cand, Vc);
         }
         if (!ok) {
 #pragma unroll
@@ -757,6 +760,8 @@ __device__ __noinline__ void coop_refine_sampson(const double* F, const double2*
             const int kb = __ffsll((unsigned long long)win) - 1;
             const double cb = readlane_d(c, kb), lb = readlane_d(lam, kb);
             load9(coop().cand[kb], g);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) V[i] = readlane_d(Vc[i], kb);
             const double rel = (cost - cb) / (cost > 1e-300 ? cost : 1e-300);
             cost = cb;
             lambda = lb * 0.1 > 1e-9 ? lb * 0.1 : 1e-9;

@@ -316,10 +316,25 @@ __device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) 
 
 // --------------------------------------------------------------- 3x3 rank 2
 
-__device__ __forceinline__ void rank2_recompose(const double* F, double* out) {
+// One-sided Jacobi on W = F V (columns orthogonalised), the least-norm
+// column dropped, F' = W' V^T. The rotations start from the identity (V0 =
+// nullptr: W = F) or from the orthogonal basis V0 (row-major, column j =
+// basis vector j: W = F V0, V = V0); Vout (if set) receives V.
+__device__ __forceinline__ void rank2_recompose_v(const double* F, const double* V0, double* out,
+                                                  double* Vout) {
     double w[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (V0) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) w[i] = F[i];
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                w[3 * r + j] = F[3 * r] * V0[j] + F[3 * r + 1] * V0[3 + j] + F[3 * r + 2] * V0[6 + j];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v[i] = V0[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) w[i] = F[i];
+    }
     for (int sweep = 0; sweep < 30; ++sweep) {
         bool rotated = false;
 #pragma unroll
@@ -349,6 +364,10 @@ __device__ __forceinline__ void rank2_recompose(const double* F, double* out) {
         }
         if (!rotated) break;
     }
+    if (Vout) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Vout[i] = v[i];
+    }
     double sig[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) sig[c] = w[c] * w[c] + w[3 + c] * w[3 + c] + w[6 + c] * w[6 + c];
@@ -369,9 +388,14 @@ __device__ __forceinline__ void rank2_recompose(const double* F, double* out) {
         }
 }
 
-__device__ __forceinline__ bool rank2_unit(const double* f, double* out) {
+__device__ __forceinline__ void rank2_recompose(const double* F, double* out) {
+    rank2_recompose_v(F, nullptr, out, nullptr);
+}
+
+__device__ __forceinline__ bool rank2_unit_v(const double* f, const double* V0, double* out,
+                                             double* Vout) {
     double g[9];
-    rank2_recompose(f, g);
+    rank2_recompose_v(f, V0, g, Vout);
     double n = fro3(g);
     if (!(isfinite(n) && n > 0.0)) return false;
 #pragma unroll
