@@ -12,7 +12,8 @@
  *   - symmetric 9x9 eigen: Householder tridiagonalisation and implicit QL
  *     (the scheme of nalgebra's SymmetricEigen: reduction to tridiagonal
  *     form, then shifted implicit iterations);
- *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958);
+ *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958),
+ *     inside the Sampson LM started from the current F's singular basis;
  *   - Cholesky: right-looking column factorisation, forward/back solves;
  *   - 3x3 inverse: cofactor expansion.
  * The GPU performs the same operations in the same order and produces the
