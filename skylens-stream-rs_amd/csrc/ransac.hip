@@ -40,6 +40,13 @@ constexpr int kWaves = 4;  // waves per workgroup: four small pairs, or one larg
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Orders this wave's LDS and device-memory accesses before the following
 // ones: the lanes of a wave exchange data through LDS and device scratch,
 // and the waves of a workgroup may work on different pairs.
@@ -149,22 +156,59 @@ __device__ __forceinline__ uint64_t ipow_adaptive(double w, int sample, double c
     return v;
 }
 
-__device__ double binomial_tail(uint64_t n, uint64_t k, double p) {
-    if (k == 0) return 1.0;
-    if (k > n || p <= 0.0) return 0.0;
-    if (p >= 1.0) return 1.0;
-    double lp = log(p), lq = log1p(-p);
-    double log_c = 0.0, sum = 0.0;
-    for (uint64_t i = 0; i <= n; ++i) {
-        if (i > 0) log_c += log((double)(n - i + 1)) - log((double)i);
-        if (i >= k) sum += exp(log_c + (double)i * lp + (double)(n - i) * lq);
+// Binomial tail P(X >= k), X ~ B(n, p) (matching.rs:843-865), for a k per
+// lane; n and p are the same in every lane, and every lane of the wave must
+// take part. The host loop's operands are formed 64 indices at a time, lane
+// j taking index i0 + j (the ln C(n, i) increment, then the term
+// exp(ln C(n, i) + i lp + (n - i) lq)); both running sums (ln C(n, i) and
+// the tail) then add them in index order, as the host does, every lane
+// reading them lane by lane -- so each lane's tail is bit-identical to the
+// host's for its k, while the logarithms and exponentials run in parallel.
+__device__ double binomial_tails(uint64_t n, uint64_t k, double p) {
+    const int lane = lane_id();
+    double triv = 1.0;
+    bool trivial = true;
+    if (k == 0)
+        triv = 1.0;
+    else if (k > n || p <= 0.0)
+        triv = 0.0;
+    else if (p >= 1.0)
+        triv = 1.0;
+    else
+        trivial = false;
+    if (__ballot(!trivial) == 0) return triv;
+    const double lp = log(p), lq = log1p(-p);
+    double lc = 0.0, sum = 0.0;
+    for (uint64_t i0 = 0; i0 <= n; i0 += NL) {
+        const uint64_t i = i0 + (uint64_t)lane;
+        double inc = 0.0;
+        if (i > 0 && i <= n) inc = log((double)(n - i + 1)) - log((double)i);
+        const uint32_t cnt = n + 1 - i0 < (uint64_t)NL ? (uint32_t)(n + 1 - i0) : (uint32_t)NL;
+        double mine = 0.0;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const double v = readlane_d(inc, (int)j);
+            if (i0 + j > 0) lc = lc + v;
+            mine = j == (uint32_t)lane ? lc : mine;
+        }
+        double term = 0.0;
+        if (i <= n) term = exp(mine + (double)i * lp + (double)(n - i) * lq);
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const double v = readlane_d(term, (int)j);
+            if (i0 + j >= k) sum = sum + v;
+        }
     }
-    return sum < 1.0 ? sum : 1.0;
+    return trivial ? triv : (sum < 1.0 ? sum : 1.0);
 }
 
+// matching.rs:405-409: the first k in 2..=n whose tail at rate 0.001 is
+// below 1e-6 (n + 1 if none), 64 candidate k at a time. Wave-uniform.
 __device__ uint64_t parallax_needed(uint64_t n) {
-    for (uint64_t k = 2; k <= n; ++k)
-        if (binomial_tail(n, k, 0.001) < 1e-6) return k;
+    for (uint64_t k0 = 2; k0 <= n; k0 += NL) {
+        const uint64_t k = k0 + (uint64_t)lane_id();
+        const double tail = binomial_tails(n, k, 0.001);
+        const uint64_t hit = __ballot(k <= n && tail < 1e-6);
+        if (hit) return k0 + (uint64_t)__builtin_ctzll(hit);
+    }
     return n + 1;
 }
 
@@ -193,13 +237,6 @@ __device__ __forceinline__ Coop& coop() { return s_cp[wave_id()]; }
 
 __device__ __forceinline__ uint64_t lanes_below() {
     return (1ull << lane_id()) - 1ull;
-}
-
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // ---------------------------------------------------------------- sampler
@@ -1649,7 +1686,7 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
                 p = p < 0.0 ? 0.0 : (p > 1.0 ? 1.0 : p);
             }
             double chance = (uint64_t)ic <= 8 ? 1.0
-                                              : binomial_tail(n > 8 ? n - 8 : 0, (uint64_t)ic - 8, p);
+                                              : binomial_tails(n > 8 ? n - 8 : 0, (uint64_t)ic - 8, p);
             if (chance > cfg.max_chance_prob) ok = false;
         }
         if (ok) {
@@ -1690,32 +1727,56 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
             }
         }
         wave_sync();
-        int best = 0;
-        for (int it = lane; it < 300; it += NL) {
-            const uint32_t ri = pm.draws[2 * it], rj = pm.draws[2 * it + 1];
-            const uint32_t pi = inl_list[ri], pj = inl_list[rj];
-            const double2 a = side ? t.x2[pi] : t.x1[pi];
-            const double2 b = side ? t.x2[pj] : t.x1[pj];
-            const double dx = b.x - a.x, dy = b.y - a.y;
-            const double len = norm2(dx, dy);
-            if (ri == rj || isnan(len) || len <= 0.0) continue;
-            const double nx = -dy / len, ny = dx / len;
-            int c = 0;
-            for (uint32_t r = 0; r < ns; ++r) {
-                const uint32_t i = inl_list[r];
-                const double2 q = side ? t.x2[i] : t.x1[i];
-                const double ex = q.x - a.x, ey = q.y - a.y;
-                if (fabs(ex * nx + ey * ny) < cfg.threshold_px) ++c;
+        // The side fails iff some valid line keeps fewer than 8 of the ns
+        // inliers off it (ns - best < 8). The off-line count of a line only
+        // grows with the points tested, so a line is settled once 8 points
+        // lie off it: the wave tests the inliers in rank order 64 at a time
+        // (one per lane, ballot + popcount) and leaves a line at the first
+        // chunk that brings its off-line count to 8 -- the first chunk for
+        // almost every line. Lines are prepared 64 at a time, lane L taking
+        // line 64b + L, and read from that lane.
+        const double2* xs = side ? t.x2 : t.x1;
+        double2 q0 = make_double2(0.0, 0.0);
+        if ((uint32_t)lane < ns) q0 = xs[inl_list[lane]];
+        bool fail = false;
+        for (int b0 = 0; b0 < 300 && !fail; b0 += NL) {
+            double ax = 0.0, ay = 0.0, nx = 0.0, ny = 0.0;
+            bool valid = false;
+            const int it = b0 + lane;
+            if (it < 300) {
+                const uint32_t ri = pm.draws[2 * it], rj = pm.draws[2 * it + 1];
+                const double2 a = xs[inl_list[ri]];
+                const double2 bp = xs[inl_list[rj]];
+                const double dx = bp.x - a.x, dy = bp.y - a.y;
+                const double len = norm2(dx, dy);
+                valid = !(ri == rj || isnan(len) || len <= 0.0);
+                ax = a.x;
+                ay = a.y;
+                nx = -dy / len;
+                ny = dx / len;
             }
-            best = best > c ? best : c;
+            uint64_t vmask = __ballot(valid);
+            while (vmask) {
+                const int l = __builtin_ctzll(vmask);
+                vmask &= vmask - 1;
+                const double lax = readlane_d(ax, l), lay = readlane_d(ay, l);
+                const double lnx = readlane_d(nx, l), lny = readlane_d(ny, l);
+                uint32_t off = 0;
+                for (uint32_t r0 = 0; r0 < ns; r0 += NL) {
+                    const uint32_t r = r0 + (uint32_t)lane;
+                    const double2 q = r0 == 0 ? q0 : (r < ns ? xs[inl_list[r]] : q0);
+                    const double ex = q.x - lax, ey = q.y - lay;
+                    const bool o = r < ns && !(fabs(ex * lnx + ey * lny) < cfg.threshold_px);
+                    off += (uint32_t)__popcll(__ballot(o));
+                    if (off >= 8) break;
+                }
+                if (off < 8) {
+                    fail = true;
+                    break;
+                }
+            }
         }
-        sh.lineBest[lane] = best;
-        wave_sync();
-        if (lane == 0) {
-            int b = 0;
-            for (int l = 0; l < NL; ++l) b = b > sh.lineBest[l] ? b : sh.lineBest[l];
-            sh.result = (int)(ns - (uint32_t)b) >= 8;
-        }
+        if (lane == 0) sh.result = fail ? 0 : 1;
         wave_sync();
         if (!sh.result) {
             line_ok = false;
