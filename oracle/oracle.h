@@ -105,9 +105,11 @@ void orc_lm_work(uint64_t* iters, uint64_t* evals);
 void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
                            uint64_t* lm_converged, uint64_t* lm_stalled_at_start);
 
-/* symmetric 9x9 eigen-decomposition used by the 8-point and DLT solvers
- * (stands in for nalgebra SymmetricEigen, matching.rs:539, two_view.rs:638). */
-void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor);
+/* Eigen-solve of the symmetric 9x9 A used by the 8-point and DLT solvers
+ * (stands in for nalgebra SymmetricEigen, matching.rs:539-541,
+ * two_view.rs:638-639): the nine eigenvalues and a unit eigenvector of the
+ * first smallest one. */
+void orc_eigen_min9(const double* A, double* evals, double* evec);
 
 /* ---- test-geometry generators (matching.rs:1165-1220, camera.rs:41-52,123-159) ---- */
 /* two_cameras(): K from_hfov(960,540,70deg), poses as in matching.rs:1165-1178.

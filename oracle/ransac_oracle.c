@@ -173,12 +173,13 @@ static void tridiag9(double* a, double* q, double* d, double* e) {
 
 #define QL_MAX_ITERS 30
 
-/* Implicit QL with shifts on the symmetric tridiagonal (d, e) (Wilkinson &
- * Reinsch, Handbook for Automatic Computation II, tql2; each rotation's
- * radius is sqrt(f^2 + g^2) and its cosine and sine two divisions by it),
- * rotating the rows of q. Eigenvalue l is split off once e[l] is negligible against its
- * neighbours; at most QL_MAX_ITERS sweeps per eigenvalue. */
-static void tql9(double* d, double* e, double* q) {
+/* Eigenvalues of the symmetric tridiagonal (d, e) by implicit QL with shifts
+ * (Wilkinson & Reinsch, Handbook for Automatic Computation II, tql1: the
+ * eigenvalue form of tql2). Each rotation's radius is r = sqrt(f^2 + g^2),
+ * its cosine and sine are g and f times 1 / r. Eigenvalue l is split off
+ * once e[l] is negligible against its neighbours; at most QL_MAX_ITERS
+ * sweeps per eigenvalue. */
+static void tql9_values(double* d, double* e) {
     for (int l = 0; l < 9; ++l) {
         for (int iter = 0; iter < QL_MAX_ITERS; ++iter) {
             int m = l;
@@ -202,18 +203,14 @@ static void tql9(double* d, double* e, double* q) {
                     underflow = 1;
                     break;
                 }
-                s = f / r;
-                c = g / r;
+                const double ri = 1.0 / r;
+                s = f * ri;
+                c = g * ri;
                 g = d[i + 1] - p;
                 r = (d[i] - g) * s + 2.0 * c * b;
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
-                for (int k = 0; k < 9; ++k) {
-                    const double fz = q[9 * k + i + 1];
-                    q[9 * k + i + 1] = s * q[9 * k + i] + c * fz;
-                    q[9 * k + i] = c * q[9 * k + i] - s * fz;
-                }
             }
             if (underflow) continue;
             d[l] = d[l] - p;
@@ -223,16 +220,111 @@ static void tql9(double* d, double* e, double* q) {
     }
 }
 
-void orc_sym_eigen9(const double* A, double* evals, double* evecs_colmajor) {
-    double a[81], q[81], d[9], e[9];
-    memcpy(a, A, sizeof a);
-    tridiag9(a, q, d, e);
-    tql9(d, e, q);
-    for (int k = 0; k < 9; ++k) {
-        evals[k] = d[k];
-        for (int i = 0; i < 9; ++i) evecs_colmajor[9 * k + i] = q[9 * i + k];
+/* x / max|x_i|, then divided by its 2-norm (sums in index order). */
+static void unit9(double* x) {
+    double mx = 0.0;
+    for (int i = 0; i < 9; ++i) mx = fmax(mx, fabs(x[i]));
+    const double sm = 1.0 / mx;
+    double ss = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        x[i] = x[i] * sm;
+        ss = ss + x[i] * x[i];
+    }
+    const double sn = 1.0 / sqrt(ss);
+    for (int i = 0; i < 9; ++i) x[i] = x[i] * sn;
+}
+
+/* Unit eigenvector of the symmetric tridiagonal (d, e) for its eigenvalue
+ * lam by inverse iteration (Wilkinson, The Algebraic Eigenvalue Problem,
+ * 1965, ch. 5.54; EISPACK tinvit): T - lam I = P L U by Gaussian
+ * elimination with partial pivoting (LAPACK dgttrf's layout: U has two
+ * superdiagonals), every pivot of U smaller than eps3 = 2^-52 |T| in
+ * magnitude replaced by eps3 with its sign (|T| = the largest absolute row
+ * sum); then x1 = U^-1 (1, ..., 1) (the start vector b with L^-1 P b = 1),
+ * x2 = (P L U)^-1 unit(x1), x = unit(x2). */
+static void tinv9(const double* d, const double* e, double lam, double* x) {
+    double tn = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        double rs = fabs(d[i]);
+        if (i < 8) rs = rs + fabs(e[i]);
+        if (i > 0) rs = rs + fabs(e[i - 1]);
+        tn = fmax(tn, rs);
+    }
+    const double eps3 = tn * 0x1p-52;
+    double dd[9], du[8], du2[7], dl[8];
+    int swp[8];
+    for (int i = 0; i < 9; ++i) dd[i] = d[i] - lam;
+    for (int i = 0; i < 8; ++i) du[i] = dl[i] = e[i];
+    for (int i = 0; i < 8; ++i) {
+        if (fabs(dd[i]) >= fabs(dl[i])) {
+            swp[i] = 0;
+            const double fact = dd[i] != 0.0 ? dl[i] / dd[i] : 0.0;
+            dl[i] = fact;
+            dd[i + 1] = dd[i + 1] - fact * du[i];
+            if (i < 7) du2[i] = 0.0;
+        } else {
+            swp[i] = 1;
+            const double fact = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = fact;
+            const double t = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = t - fact * dd[i + 1];
+            if (i < 7) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -fact * du[i + 1];
+            }
+        }
+    }
+    double ui[9];
+    for (int i = 0; i < 9; ++i) {
+        double u = dd[i];
+        if (fabs(u) < eps3) u = u >= 0.0 ? eps3 : -eps3;
+        ui[i] = 1.0 / u;
+    }
+    double b[9];
+    for (int i = 0; i < 9; ++i) b[i] = 1.0;
+    for (int it = 0; it < 2; ++it) {
+        if (it == 1)
+            for (int i = 0; i < 8; ++i) {
+                if (swp[i]) {
+                    const double t = b[i];
+                    b[i] = b[i + 1];
+                    b[i + 1] = t - dl[i] * b[i];
+                } else {
+                    b[i + 1] = b[i + 1] - dl[i] * b[i];
+                }
+            }
+        x[8] = b[8] * ui[8];
+        x[7] = (b[7] - du[7] * x[8]) * ui[7];
+        for (int i = 6; i >= 0; --i) x[i] = (b[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) * ui[i];
+        unit9(x);
+        memcpy(b, x, sizeof b);
     }
 }
+
+/* The eigenvalues of the symmetric 9x9 A (ev, in the order the QL sweep
+ * leaves them) and a unit eigenvector of the first smallest one (vec):
+ * tridiagonalisation T = Q^T A Q, eigenvalues of T by QL, the eigenvector
+ * of T by inverse iteration, vec = Q x. */
+static int imin9(const double* e);
+static void eigen_min9(const double* A, double* ev, double* vec) {
+    double a[81], q[81], d[9], e[9], d0[9], e0[9], x[9];
+    memcpy(a, A, sizeof a);
+    tridiag9(a, q, d, e);
+    memcpy(d0, d, sizeof d0);
+    memcpy(e0, e, sizeof e0);
+    tql9_values(d, e);
+    tinv9(d0, e0, d[imin9(d)], x);
+    for (int i = 0; i < 9; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 9; ++j) s = s + q[9 * i + j] * x[j];
+        vec[i] = s;
+    }
+    memcpy(ev, d, sizeof d);
+}
+
+void orc_eigen_min9(const double* A, double* evals, double* evec) { eigen_min9(A, evals, evec); }
 
 static int imin9(const double* e) {
     int k = 0;
@@ -387,15 +479,14 @@ int orc_fundamental_8pt(const double* x1, const double* x2, size_t n, double* F)
         for (int r = 0; r < 9; ++r)
             for (int c = 0; c < 9; ++c) ata[9 * r + c] = ata[9 * r + c] + row[r] * row[c];
     }
-    double ev[9], vec[81];
-    orc_sym_eigen9(ata, ev, vec);
-    int k = imin9(ev);
+    double ev[9], vec[9];
+    eigen_min9(ata, ev, vec);
     double sorted[9];
     memcpy(sorted, ev, sizeof sorted);
     qsort(sorted, 9, sizeof(double), cmp_double);
     if (sorted[1] <= 1e-10 * sorted[8]) return 0;
     double fn[9];
-    for (int i = 0; i < 9; ++i) fn[i] = vec[9 * k + i];
+    for (int i = 0; i < 9; ++i) fn[i] = vec[i];
     double r2[9];
     rank2_recompose(fn, r2);
     double t2t[9], tmp[9], f[9];
@@ -694,11 +785,10 @@ static int homography_dlt_core(const double* a, const double* b, size_t n, doubl
             for (int c = 0; c < 9; ++c)
                 ata[9 * r + c] = ata[9 * r + c] + (r1[r] * r1[c] + r2[r] * r2[c]);
     }
-    double ev[9], vec[81];
-    orc_sym_eigen9(ata, ev, vec);
-    int k = imin9(ev);
+    double ev[9], vec[9];
+    eigen_min9(ata, ev, vec);
     for (int i = 0; i < 9; ++i) {
-        H[i] = vec[9 * k + i];
+        H[i] = vec[i];
         if (!isfinite(H[i])) return 0;
     }
     return 1;

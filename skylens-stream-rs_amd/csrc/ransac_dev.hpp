@@ -253,10 +253,10 @@ __device__ __forceinline__ void tridiag9_rows(double* a, int r0, double (*q)[9],
     e[8] = 0.0;
 }
 
-// Implicit QL with shifts on (d, e) (tql2 scheme), rotating the R rows of
-// q; at most 30 sweeps per eigenvalue.
-template <int R>
-__device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) {
+// Eigenvalues of the symmetric tridiagonal (d, e) by implicit QL with
+// shifts (tql1 scheme: tql2 without eigenvectors); each rotation's cosine
+// and sine are g and f times 1 / r; at most 30 sweeps per eigenvalue.
+__device__ __forceinline__ void tql9_values(double* d, double* e) {
     sfor<0, 9>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
         for (int iter = 0; iter < 30; ++iter) {
@@ -288,19 +288,14 @@ __device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) 
                         for (int j = l; j < 9; ++j) e[j] = (j == m) ? 0.0 : e[j];
                         underflow = true;
                     } else {
-                        s = f / r;
-                        c = g / r;
+                        const double ri = 1.0 / r;
+                        s = f * ri;
+                        c = g * ri;
                         g = d[i + 1] - p;
                         r = (d[i] - g) * s + 2.0 * c * b;
                         p = s * r;
                         d[i + 1] = g + p;
                         g = c * r - b;
-#pragma unroll
-                        for (int k = 0; k < R; ++k) {
-                            const double fz = q[k][i + 1];
-                            q[k][i + 1] = s * q[k][i] + c * fz;
-                            q[k][i] = c * q[k][i] - s * fz;
-                        }
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -312,6 +307,101 @@ __device__ __forceinline__ void tql9_rows(double* d, double* e, double (*q)[9]) 
             for (int j = l; j < 9; ++j) e[j] = (j == m) ? 0.0 : e[j];
         }
     });
+}
+
+// x / max|x_i|, then divided by its 2-norm (sums in index order).
+__device__ __forceinline__ void unit9(double* x) {
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mx = fmax(mx, fabs(x[i]));
+    const double sm = 1.0 / mx;
+    double ss = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        x[i] = x[i] * sm;
+        ss = ss + x[i] * x[i];
+    }
+    const double sn = 1.0 / sqrt(ss);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) x[i] = x[i] * sn;
+}
+
+// Unit eigenvector of the symmetric tridiagonal (d, e) for its eigenvalue
+// lam by inverse iteration (Wilkinson 1965, ch. 5.54; EISPACK tinvit):
+// T - lam I = P L U by Gaussian elimination with partial pivoting (U with
+// two superdiagonals), every pivot of U below eps3 = 2^-52 |T| in magnitude
+// raised to eps3 with its sign (|T|: largest absolute row sum); then
+// x1 = U^-1 (1, ..., 1), x2 = (P L U)^-1 unit(x1), x = unit(x2).
+__device__ __forceinline__ void tinv9(const double* d, const double* e, double lam, double* x) {
+    double tn = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double rs = fabs(d[i]);
+        if (i < 8) rs = rs + fabs(e[i]);
+        if (i > 0) rs = rs + fabs(e[i - 1]);
+        tn = fmax(tn, rs);
+    }
+    const double eps3 = tn * 0x1p-52;
+    double dd[9], du[8], du2[7], dl[8];
+    bool swp[8];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dd[i] = d[i] - lam;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) du[i] = dl[i] = e[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const bool keep = fabs(dd[i]) >= fabs(dl[i]);
+        swp[i] = !keep;
+        if (keep) {
+            const double fact = dd[i] != 0.0 ? dl[i] / dd[i] : 0.0;
+            dl[i] = fact;
+            dd[i + 1] = dd[i + 1] - fact * du[i];
+            if (i < 7) du2[i] = 0.0;
+        } else {
+            const double fact = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = fact;
+            const double t = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = t - fact * dd[i + 1];
+            if (i < 7) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -fact * du[i + 1];
+            }
+        }
+    }
+    double ui[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double u = dd[i];
+        if (fabs(u) < eps3) u = u >= 0.0 ? eps3 : -eps3;
+        ui[i] = 1.0 / u;
+    }
+    double b[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) b[i] = 1.0;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        if (it == 1) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (swp[i]) {
+                    const double t = b[i];
+                    b[i] = b[i + 1];
+                    b[i + 1] = t - dl[i] * b[i];
+                } else {
+                    b[i + 1] = b[i + 1] - dl[i] * b[i];
+                }
+            }
+        }
+        x[8] = b[8] * ui[8];
+        x[7] = (b[7] - du[7] * x[8]) * ui[7];
+#pragma unroll
+        for (int i = 6; i >= 0; --i) x[i] = (b[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) * ui[i];
+        unit9(x);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) b[i] = x[i];
+    }
 }
 
 // --------------------------------------------------------------- 3x3 rank 2
@@ -581,8 +671,8 @@ __device__ bool accumulate_8pt(const S& s, uint32_t len, double* t1, double* t2,
 }
 
 // Lanes 3g, 3g+1, 3g+2 of a wave form solver group g (g < kGroups); a group splits the
-// nine eigenvector rows of a 9x9 eigen-solve three per lane while every lane
-// of the group rotates the same matrix, which keeps both in registers.
+// nine rows of a 9x9 eigen-solve's orthogonal basis three per lane while every
+// lane of the group reduces the same matrix, which keeps both in registers.
 constexpr int kGroups = 21;
 
 __device__ __forceinline__ int solver_group() { return (int)(threadIdx.x & 63u) / 3; }
@@ -592,15 +682,22 @@ __device__ __forceinline__ int solver_part() {
 }
 
 // Eigen-solve of the upper triangle `a` (45, overwritten) by a solver group:
-// d receives the nine eigenvalues and every lane of the group the
-// eigenvector of the smallest one (first minimum) in vk. All three lanes of
-// the group must be active with the same `a`.
+// tridiagonalisation T = Q^T A Q (each lane accumulating three rows of Q),
+// the eigenvalues of T by QL into d, the eigenvector x of the smallest one
+// (first minimum) by inverse iteration on T, and Q x into vk in every lane
+// of the group. All three lanes of the group must be active with the same
+// `a`.
 __device__ __forceinline__ int eigen_min_group(double* a, double* d, double* vk) {
     const int part = solver_part();
     const int base = 3 * solver_group();
-    double q[3][9], e[9];
+    double q[3][9], e[9], d0[9], e0[9];
     tridiag9_rows<3>(a, 3 * part, q, d, e);
-    tql9_rows<3>(d, e, q);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        d0[i] = d[i];
+        e0[i] = e[i];
+    }
+    tql9_values(d, e);
     // first minimum (the value is tracked so that no array is indexed at run time)
     int k = 0;
     double dmin = d[0];
@@ -611,12 +708,15 @@ __device__ __forceinline__ int eigen_min_group(double* a, double* d, double* vk)
             k = i;
         }
     }
+    double x[9];
+    tinv9(d0, e0, dmin, x);
+    // this lane's three rows of Q x
     double mine[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) acc = (j == k) ? q[i][j] : acc;
+        for (int j = 0; j < 9; ++j) acc = acc + q[i][j] * x[j];
         mine[i] = acc;
     }
 #pragma unroll

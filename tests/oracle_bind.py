@@ -171,7 +171,7 @@ _SIGS_R = {
     "orc_select_two_view_model": (ctypes.c_int, [_vp, _vp, _sz, _vp, _d, _vp]),
     "orc_ransac_fundamental": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "orc_ransac_last_stats": (None, [_vp, _vp, _vp, _vp, _vp]),
-    "orc_sym_eigen9": (None, [_vp, _vp, _vp]),
+    "orc_eigen_min9": (None, [_vp, _vp, _vp]),
     "orc_two_cameras": (None, [_vp, _vp]),
     "orc_project": (ctypes.c_int, [_vp, _vp, _vp]),
     "orc_fundamental_from_cameras": (None, [_vp, _vp, _vp]),
@@ -279,12 +279,12 @@ def ransac_last_stats():
                     [int(x[0]) for x in v]))
 
 
-def sym_eigen9(A):
+def eigen_min9(A):
     A = np.ascontiguousarray(A, np.float64).reshape(81)
     ev = np.zeros(9)
-    vec = np.zeros(81)
-    lib().orc_sym_eigen9(P(A), P(ev), P(vec))
-    return ev, vec.reshape(9, 9).T  # columns are eigenvectors
+    vec = np.zeros(9)
+    lib().orc_eigen_min9(P(A), P(ev), P(vec))
+    return ev, vec
 
 
 def two_cameras():

@@ -54,16 +54,39 @@ def test_parallax_needed_values():
         assert ob.binomial_tail(n, k - 1, 0.001) >= 1e-6
 
 
-def test_sym_eigen_against_lapack():
+def test_eigen_min_against_lapack():
+    # eigenvalues against LAPACK; the vector is a unit eigenvector of the
+    # smallest one (A v = lambda v), the same line as LAPACK's when the gap
+    # to the next eigenvalue is wide
     rng = np.random.default_rng(2)
-    for trial in range(40):
+    for trial in range(60):
         X = rng.standard_normal((9 if trial % 2 else 12, 9)) * rng.uniform(0.1, 10, 9)
         A = X.T @ X
-        ev, V = ob.sym_eigen9(A)
-        ref = np.linalg.eigvalsh(A)
+        ev, v = ob.eigen_min9(A)
+        ref, V = np.linalg.eigh(A)
         assert np.allclose(np.sort(ev), ref, rtol=1e-10, atol=1e-10 * ref[-1])
-        assert np.allclose(V.T @ V, np.eye(9), atol=1e-12)
-        assert np.allclose(A @ V, V * ev, atol=1e-9 * ref[-1])
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-13
+        assert np.linalg.norm(A @ v - ref[0] * v) < 1e-9 * ref[-1]
+        if ref[1] - ref[0] > 1e-3 * ref[-1]:
+            assert min(np.linalg.norm(v - V[:, 0]), np.linalg.norm(v + V[:, 0])) < 1e-6
+
+
+def test_eigen_min_null_vectors():
+    # rank-deficient A = X^T X (8 rows): the vector spans the null space of X,
+    # also for a tridiagonal form whose eliminations pivot (graded rows) and
+    # for a block-diagonal A whose tridiagonal form splits
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        X = rng.standard_normal((8, 9)) * (10.0 ** rng.uniform(-2, 2, 9))
+        A = X.T @ X
+        ev, v = ob.eigen_min9(A)
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-13
+        assert np.linalg.norm(X @ v) < 1e-9 * np.linalg.norm(X)
+        assert np.min(ev) < 1e-9 * np.max(ev)
+    D = np.diag([5.0, 1.0, 7.0, 2.0, 3.0, 0.5, 9.0, 4.0, 6.0])
+    ev, v = ob.eigen_min9(D)
+    assert np.allclose(np.sort(ev), np.sort(np.diag(D)), rtol=0, atol=1e-14)
+    assert abs(abs(v[5]) - 1.0) < 1e-14 and np.linalg.norm(np.delete(v, 5)) < 1e-14
 
 
 # matching.rs:1222-1235
