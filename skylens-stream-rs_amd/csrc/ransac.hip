@@ -1242,8 +1242,8 @@ constexpr uint32_t kHandWords = 2 * kGroups * kHandStride;   // 32-bit words
 
 // Local optimisation (matching.rs:993-1015) of every candidate in `cand`
 // (hypothesis lanes), stage by stage: for each stage the candidates' point
-// sums run wave-cooperatively one candidate after another, then pairs of
-// lanes solve up to 32 candidates' 8-point systems at once, then each
+// sums run wave-cooperatively one candidate after another, then the solver
+// groups (three lanes each) solve up to 21 candidates' 8-point systems at once, then each
 // candidate's Sampson pass gives its count at x1 and its next selection.
 // Results: loF[h], loCnt[h] and the inlier bitset lo + h * W; list (n
 // words) and hand (kGroups candidate records) are device scratch. Solver
