@@ -1521,29 +1521,39 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
                         pre, &sh.pre_next);
         pair_sync();
         lap(2);
-        // replay in order (lane 0)
-        if (lead && lane == 0) {
-            for (int h = 0; h < kHypPerBatch && sh.it < sh.iters; ++h) {
-                sh.it++;
+        // replay in order: the lead wave walks the batch's hypotheses with
+        // wave-uniform control flow (the loop state in registers, every lane
+        // holding the same values); the Jaccard popcounts run over the bitset
+        // words one per lane, lane 0 alone updates the pool order in LDS.
+        if (lead) {
+            uint64_t it = sh.it, iters = sh.iters;
+            int best_cnt = sh.haveBest ? sh.bestCnt : 0;
+            int have_best = sh.haveBest, npool = sh.npool, result = sh.result;
+            for (int h = 0; h < kHypPerBatch && it < iters; ++h) {
+                it++;
                 const int c0 = pm.hyp.hCnt[h];
                 if (c0 < 0) continue;
-                const int best_cnt = sh.haveBest ? sh.bestCnt : 0;
                 if (!(c0 >= 8 && 4 * c0 >= best_cnt)) continue;
                 // loCnt[h] >= 0 here: the candidate bound never exceeds best_cnt
                 const int c = pm.hyp.loCnt[h];
                 if (c < 0) {  // cannot happen: candidates are a superset of triggers
-                    sh.result = -1;
-                    sh.it = sh.iters;
+                    result = -1;
+                    it = iters;
                     break;
                 }
                 const uint32_t* inl = lo_masks + (size_t)h * W;
                 int same = -1;
-                for (int j = 0; j < sh.npool && same < 0; ++j) {
+                for (int j = 0; j < npool && same < 0; ++j) {
                     const uint32_t* pmk = pool_masks + (size_t)sh.poolSlot[j] * W;
                     uint32_t both = 0, either = 0;
-                    for (uint32_t w0 = 0; w0 < W; ++w0) {
-                        both += __popc(pmk[w0] & inl[w0]);
-                        either += __popc(pmk[w0] | inl[w0]);
+                    for (uint32_t w0 = (uint32_t)lane; w0 < W; w0 += NL) {
+                        both += (uint32_t)__popc(pmk[w0] & inl[w0]);
+                        either += (uint32_t)__popc(pmk[w0] | inl[w0]);
+                    }
+#pragma unroll
+                    for (int off = NL / 2; off > 0; off >>= 1) {
+                        both += (uint32_t)__shfl_xor((int)both, off);
+                        either += (uint32_t)__shfl_xor((int)either, off);
                     }
                     if (10 * both > 7 * either) same = j;
                 }
@@ -1551,41 +1561,52 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
                 if (same >= 0) {
                     if (sh.poolCnt[same] < c) target = same;
                 } else {
-                    target = sh.npool++;
+                    target = npool++;
                 }
                 if (target >= 0) {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) sh.poolF[target][i] = pm.hyp.loF[h][i];
-                    sh.poolCnt[target] = c;
+                    if (lane < 9) sh.poolF[target][lane] = pm.hyp.loF[h][lane];
+                    if (lane == 0) sh.poolCnt[target] = c;
                     uint32_t* dst = pool_masks + (size_t)sh.poolSlot[target] * W;
-                    for (uint32_t w0 = 0; w0 < W; ++w0) dst[w0] = inl[w0];
+                    for (uint32_t w0 = (uint32_t)lane; w0 < W; w0 += NL) dst[w0] = inl[w0];
                 }
-                // stable insertion sort, count descending
-                for (int a = 1; a < sh.npool; ++a) {
-                    double xf[9];
+                wave_sync();
+                if (lane == 0) {
+                    // stable insertion sort, count descending
+                    for (int a = 1; a < npool; ++a) {
+                        double xf[9];
 #pragma unroll
-                    for (int i = 0; i < 9; ++i) xf[i] = sh.poolF[a][i];
-                    const int xc = sh.poolCnt[a], xs = sh.poolSlot[a];
-                    int b = a - 1;
-                    while (b >= 0 && sh.poolCnt[b] < xc) {
+                        for (int i = 0; i < 9; ++i) xf[i] = sh.poolF[a][i];
+                        const int xc = sh.poolCnt[a], xs = sh.poolSlot[a];
+                        int b = a - 1;
+                        while (b >= 0 && sh.poolCnt[b] < xc) {
 #pragma unroll
-                        for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = sh.poolF[b][i];
-                        sh.poolCnt[b + 1] = sh.poolCnt[b];
-                        sh.poolSlot[b + 1] = sh.poolSlot[b];
-                        --b;
+                            for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = sh.poolF[b][i];
+                            sh.poolCnt[b + 1] = sh.poolCnt[b];
+                            sh.poolSlot[b + 1] = sh.poolSlot[b];
+                            --b;
+                        }
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = xf[i];
+                        sh.poolCnt[b + 1] = xc;
+                        sh.poolSlot[b + 1] = xs;
                     }
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) sh.poolF[b + 1][i] = xf[i];
-                    sh.poolCnt[b + 1] = xc;
-                    sh.poolSlot[b + 1] = xs;
                 }
-                if (sh.npool > 5) sh.npool = 5;  // the dropped slot stays at index 5
+                wave_sync();
+                if (npool > 5) npool = 5;  // the dropped slot stays at index 5
                 if (c <= best_cnt) continue;
-                sh.iters = ipow_adaptive((double)c / (double)n, 8, cfg.confidence, cfg.max_iters);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) sh.bestF[i] = pm.hyp.loF[h][i];
-                sh.bestCnt = c;
-                sh.haveBest = 1;
+                iters = ipow_adaptive((double)c / (double)n, 8, cfg.confidence, cfg.max_iters);
+                if (lane < 9) sh.bestF[lane] = pm.hyp.loF[h][lane];
+                best_cnt = c;
+                have_best = 1;
+            }
+            wave_sync();
+            if (lane == 0) {
+                sh.it = it;
+                sh.iters = iters;
+                sh.bestCnt = best_cnt;
+                sh.haveBest = have_best;
+                sh.npool = npool;
+                sh.result = result;
             }
         }
         pair_sync();
