@@ -343,7 +343,9 @@ static int cmp_double(const void* x, const void* y) {
  * :577-580: zero the smallest singular value, recompose). The rotations start
  * from the identity (V0 = NULL: W = F) or from an orthogonal basis V0
  * (row-major, column j = basis vector j: W = F V0, V = V0), e.g. the basis of
- * a nearby matrix, which leaves fewer sweeps; Vout (if set) receives V. */
+ * a nearby matrix, which leaves fewer sweeps; Vout (if set) receives V. A
+ * column pair is rotated while gamma^2 > 1e-30 alpha beta (|gamma| above
+ * 1e-15 of the geometric mean of the two squared column norms). */
 #define SVD3_MAX_SWEEPS 30
 static void rank2_recompose_v(const double* F, const double* V0, double* out, double* Vout) {
     double w[9], v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -364,7 +366,7 @@ static void rank2_recompose_v(const double* F, const double* V0, double* out, do
             double beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
             double gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
             double c = 1.0, s = 0.0;
-            if (gamma != 0.0 && fabs(gamma) > 1e-15 * sqrt(alpha * beta)) {
+            if (gamma != 0.0 && gamma * gamma > 1e-30 * (alpha * beta)) {
                 double zeta = (beta - alpha) / (2.0 * gamma);
                 double t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 if (zeta < 0.0) t = -t;
@@ -535,7 +537,9 @@ void orc_sampson_residual_jacobian(const double* g, double s1, double s2, const 
     *r = e * inv;
 }
 
-/* matching.rs:621-641, summed into the cost sum_e e^2 in order. */
+/* matching.rs:621-641, summed into the cost sum_e e^2 in order; each
+ * squared residual e^2 = (b . G a)^2 / den is formed without the square
+ * root of the residual itself. */
 static double sampson_cost(const double* g, double s1, double s2, const double* a,
                            const double* b, size_t n) {
     double q1 = s1 * s1, q2 = s2 * s2;
@@ -545,22 +549,23 @@ static double sampson_cost(const double* g, double s1, double s2, const double* 
         mv3(g, a + 3 * i, ga);
         mtv3(g, b + 3 * i, gtb);
         double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-        double e;
+        double e2 = 0.0;
         if (den > 0.0) {
             const double* bb = b + 3 * i;
-            e = (bb[0] * ga[0] + bb[1] * ga[1] + bb[2] * ga[2]) / sqrt(den);
-        } else {
-            e = 0.0;
+            const double num = bb[0] * ga[0] + bb[1] * ga[1] + bb[2] * ga[2];
+            e2 = num * num / den;
         }
-        cost = cost + e * e;
+        cost = cost + e2;
     }
     return cost;
 }
 
 /* 9x9 Cholesky (nalgebra's column-by-column, right-looking update) and solve
- * m x = rhs. Returns 0 if a pivot is not positive. */
+ * m x = rhs; every division by a diagonal entry L_jj is a multiplication by
+ * its reciprocal (one division per column). Returns 0 if a pivot is not
+ * positive. */
 static int cholesky_solve9(const double* m_in, const double* rhs, double* x) {
-    double L[81];
+    double L[81], rd[9];
     memcpy(L, m_in, sizeof L);
     for (int j = 0; j < 9; ++j) {
         for (int k = 0; k < j; ++k) {
@@ -571,18 +576,19 @@ static int cholesky_solve9(const double* m_in, const double* rhs, double* x) {
         if (!(diag > 0.0)) return 0;
         double den = sqrt(diag);
         L[9 * j + j] = den;
-        for (int i = j + 1; i < 9; ++i) L[9 * i + j] = L[9 * i + j] / den;
+        rd[j] = 1.0 / den;
+        for (int i = j + 1; i < 9; ++i) L[9 * i + j] = L[9 * i + j] * rd[j];
     }
     double y[9];
     memcpy(y, rhs, sizeof y);
     for (int i = 0; i < 9; ++i) {
-        y[i] = y[i] / L[9 * i + i];
+        y[i] = y[i] * rd[i];
         for (int k = i + 1; k < 9; ++k) y[k] = y[k] - L[9 * k + i] * y[i];
     }
     for (int i = 8; i >= 0; --i) {
         double dot = 0.0;
         for (int k = i + 1; k < 9; ++k) dot = dot + L[9 * k + i] * y[k];
-        y[i] = (y[i] - dot) / L[9 * i + i];
+        y[i] = (y[i] - dot) * rd[i];
     }
     memcpy(x, y, sizeof y);
     return 1;

@@ -77,12 +77,14 @@ struct Pts4 {
 // ---------------------------------------------------------------- Cholesky
 
 // Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs
-// (right-looking Cholesky, then forward and back substitution). Fully
+// (right-looking Cholesky, then forward and back substitution; divisions by
+// a diagonal entry are multiplications by its reciprocal). Fully
 // expanded at compile time: every index is a constant, so L stays in
 // registers. A non-positive pivot makes it return false (the remaining
 // columns are computed but unused).
 __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
     bool ok = true;
+    double rd[9];  // reciprocals of the diagonal
     sfor<0, 9>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         sfor<0, j>([&](auto kc) {
@@ -97,9 +99,10 @@ __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
         ok = ok && diag > 0.0;
         const double den = sqrt(diag);
         L[ut(j, j)] = den;
+        rd[j] = 1.0 / den;
         sfor<j + 1, 9>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
-            L[ut(i, j)] = L[ut(i, j)] / den;
+            L[ut(i, j)] = L[ut(i, j)] * rd[j];
         });
     });
     if (!ok) return false;
@@ -108,7 +111,7 @@ __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
     for (int i = 0; i < 9; ++i) y[i] = rhs[i];
     sfor<0, 9>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        y[i] = y[i] / L[ut(i, i)];
+        y[i] = y[i] * rd[i];
         sfor<i + 1, 9>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             y[k] = y[k] - L[ut(k, i)] * y[i];
@@ -121,7 +124,7 @@ __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
             constexpr int k = decltype(kc)::value;
             dot = dot + L[ut(k, i)] * y[k];
         });
-        y[i] = (y[i] - dot) / L[ut(i, i)];
+        y[i] = (y[i] - dot) * rd[i];
     });
 #pragma unroll
     for (int i = 0; i < 9; ++i) x[i] = y[i];
@@ -607,7 +610,8 @@ __device__ double coop_ata(const double2* x1, const double2* x2, const uint32_t*
 // ---------------------------------------------------------------- LM refine
 
 // Sum of squared Sampson residuals of the normalised g over `sel`
-// (matching.rs:620-641 summed as the host does).
+// (matching.rs:620-641 summed as the host does; each squared residual is
+// (b . g a)^2 / den).
 __device__ double coop_lm_cost(const double* g, double s1, double s2, const double* t1,
                                const double* t2, const double2* x1, const double2* x2,
                                const uint32_t* list, uint32_t len) {
@@ -624,12 +628,12 @@ __device__ double coop_lm_cost(const double* g, double s1, double s2, const doub
             mtv3(g, b, gtb);
             double den =
                 q2 * (ga[0] * ga[0] + ga[1] * ga[1]) + q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-            double e;
-            if (den > 0.0)
-                e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
-            else
-                e = 0.0;
-            coop().stage[slot][0] = e * e;
+            double e2 = 0.0;
+            if (den > 0.0) {
+                const double num = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
+                e2 = num * num / den;
+            }
+            coop().stage[slot][0] = e2;
         },
         [&](uint32_t cnt) {
             cost = stage_sum(0, cnt, cost);
@@ -660,12 +664,12 @@ __device__ double coop_lm_costs(double s1, double s2, const double* t1, const do
                 mtv3(g, b, gtb);
                 double den = q2 * (ga[0] * ga[0] + ga[1] * ga[1]) +
                              q1 * (gtb[0] * gtb[0] + gtb[1] * gtb[1]);
-                double e;
-                if (den > 0.0)
-                    e = (b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2]) / sqrt(den);
-                else
-                    e = 0.0;
-                coop().stage[slot][k] = e * e;
+                double e2 = 0.0;
+                if (den > 0.0) {
+                    const double num = b[0] * ga[0] + b[1] * ga[1] + b[2] * ga[2];
+                    e2 = num * num / den;
+                }
+                coop().stage[slot][k] = e2;
             }
         },
         [&](uint32_t cnt) {

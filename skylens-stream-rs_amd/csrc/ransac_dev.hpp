@@ -434,7 +434,7 @@ __device__ __forceinline__ void rank2_recompose_v(const double* F, const double*
             double beta = w[q] * w[q] + w[3 + q] * w[3 + q] + w[6 + q] * w[6 + q];
             double gamma = w[p] * w[q] + w[3 + p] * w[3 + q] + w[6 + p] * w[6 + q];
             double c = 1.0, s = 0.0;
-            if (gamma != 0.0 && fabs(gamma) > 1e-15 * sqrt(alpha * beta)) {
+            if (gamma != 0.0 && gamma * gamma > 1e-30 * (alpha * beta)) {
                 double zeta = (beta - alpha) / (2.0 * gamma);
                 double t = 1.0 / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 if (zeta < 0.0) t = -t;
