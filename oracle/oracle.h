@@ -93,6 +93,11 @@ void orc_normalizer(const double* p, size_t n, double* T);
 int orc_refine_sampson(const double* F, const double* x1, const double* x2, size_t n, size_t iters,
                        double* out, uint64_t* taken, int* stop);
 uint64_t orc_homography_support(const double* x1, const double* x2, size_t n, double th_px);
+/* homography_dlt (matching.rs:251-266) of n correspondences, and the sampled
+ * hypotheses' closed-form four-point solution; H row-major, unit Frobenius
+ * norm. Return 0 for no homography. */
+int orc_homography_dlt(const double* x1, const double* x2, size_t n, double* H);
+int orc_homography_4pt(const double* x1, const double* x2, double* H);
 int orc_select_two_view_model(const double* x1, const double* x2, size_t n, const double* F,
                               double sigma_px, orc_model_sel* out);
 /* Returns 1 = Some(FundamentalFit). F row-major, inliers n bytes. */

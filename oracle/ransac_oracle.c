@@ -800,8 +800,49 @@ static int homography_dlt_core(const double* a, const double* b, size_t n, doubl
     return 1;
 }
 
+/* The homography of four correspondences of normalised points (a_i -> b_i,
+ * i < 4), the exact solution of their DLT system (8 equations for H's 8
+ * degrees of freedom: the null vector that homography_dlt_core finds as the
+ * smallest eigenvector of A^T A, two_view.rs:607-642), in closed form
+ * (Hartley & Zisserman, Multiple View Geometry, 2nd ed., ex. 4.1 / the
+ * projective basis construction): with p_i = (a_i, 1), q_i = (b_i, 1),
+ * M = [p0 p1 p2] and N = [q0 q1 q2], adj(M) has the rows p1 x p2, p2 x p0,
+ * p0 x p1; l = adj(M) p3 and m = adj(N) q3 are the basis weights up to
+ * det(M) and det(N), and H = N diag(m_i / l_i) adj(M) up to scale
+ * (H p_j ~ q_j for j < 3 and H p3 ~ N m ~ q3). H[r][c] = sum_i (q_i[r] w_i)
+ * adj_i[c] in index order, w_i = m_i / l_i. Returns 0 (no hypothesis) if
+ * some l_i is 0 or H is not finite: three of the four points collinear. */
+static void cross3(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+static int homography_4pt_core(const double* a, const double* b, double* H) {
+    double p[4][3], q[4][3], am[3][3], an[3][3], l[3], m[3], w[3];
+    for (int i = 0; i < 4; ++i) {
+        p[i][0] = a[2 * i]; p[i][1] = a[2 * i + 1]; p[i][2] = 1.0;
+        q[i][0] = b[2 * i]; q[i][1] = b[2 * i + 1]; q[i][2] = 1.0;
+    }
+    cross3(p[1], p[2], am[0]); cross3(p[2], p[0], am[1]); cross3(p[0], p[1], am[2]);
+    cross3(q[1], q[2], an[0]); cross3(q[2], q[0], an[1]); cross3(q[0], q[1], an[2]);
+    for (int i = 0; i < 3; ++i) {
+        l[i] = am[i][0] * p[3][0] + am[i][1] * p[3][1] + am[i][2] * p[3][2];
+        m[i] = an[i][0] * q[3][0] + an[i][1] * q[3][1] + an[i][2] * q[3][2];
+        if (l[i] == 0.0) return 0;
+        w[i] = m[i] / l[i];
+    }
+    for (int r = 0; r < 3; ++r) {
+        const double t0 = q[0][r] * w[0], t1 = q[1][r] * w[1], t2 = q[2][r] * w[2];
+        for (int c = 0; c < 3; ++c) {
+            H[3 * r + c] = t0 * am[0][c] + t1 * am[1][c] + t2 * am[2][c];
+            if (!isfinite(H[3 * r + c])) return 0;
+        }
+    }
+    return 1;
+}
+
 /* matching.rs:251-266: Hartley-normalised pixel DLT. */
-static int homography_dlt(const double* x1, const double* x2, size_t n, double* H) {
+static int homography_dlt_x(const double* x1, const double* x2, size_t n, double* H, int four) {
     if (n < 4) return 0;
     double t1[9], t2[9];
     orc_normalizer(x1, n, t1);
@@ -817,7 +858,7 @@ static int homography_dlt(const double* x1, const double* x2, size_t n, double* 
         b[2 * i] = v[0] / v[2]; b[2 * i + 1] = v[1] / v[2];
     }
     double hn[9];
-    int ok = homography_dlt_core(a, b, n, hn);
+    int ok = four ? homography_4pt_core(a, b, hn) : homography_dlt_core(a, b, n, hn);
     free(a);
     free(b);
     if (!ok) return 0;
@@ -829,6 +870,23 @@ static int homography_dlt(const double* x1, const double* x2, size_t n, double* 
     if (!(isfinite(nn) && nn > 0.0)) return 0;
     for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
     return 1;
+}
+
+static int homography_dlt(const double* x1, const double* x2, size_t n, double* H) {
+    return homography_dlt_x(x1, x2, n, H, 0);
+}
+
+/* The sampled hypotheses' homography (matching.rs:334-337): homography_dlt of
+ * the four samples, its DLT system solved in closed form. */
+static int homography_4pt(const double* x1, const double* x2, double* H) {
+    return homography_dlt_x(x1, x2, 4, H, 1);
+}
+
+int orc_homography_dlt(const double* x1, const double* x2, size_t n, double* H) {
+    return homography_dlt(x1, x2, n, H);
+}
+int orc_homography_4pt(const double* x1, const double* x2, double* H) {
+    return homography_4pt(x1, x2, H);
 }
 
 /* matching.rs:269-287. */
@@ -883,7 +941,7 @@ static int fit_homography(const double* x1, const double* x2, size_t n, double t
             s2[2 * e] = x2[2 * idx[e]]; s2[2 * e + 1] = x2[2 * idx[e] + 1];
         }
         double h[9], hi[9];
-        if (!homography_dlt(s1, s2, 4, h)) continue;
+        if (!homography_4pt(s1, s2, h)) continue;
         if (!inv3(h, hi)) continue;
         size_t c = count_h(h, hi, x1, x2, n, th_px);
         if (!have || c > bc) {

@@ -854,32 +854,74 @@ struct PtsRank4 {
     __device__ double2 q(uint32_t i) const { return x2[list[rank[i]]]; }
 };
 
-// The DLT normal matrix of len points of `s` (one lane).
+// The sampled hypotheses' homography (matching.rs:334-337): homography_dlt of
+// the four samples of `s` with its DLT system (8 equations for 8 degrees of
+// freedom) solved in closed form, the host's homography_4pt step for step:
+// Hartley normalisers, normalised points p_i, q_i, adj(M) rows p1 x p2,
+// p2 x p0, p0 x p1 (M = [p0 p1 p2], likewise N), weights l = adj(M) p3,
+// m = adj(N) q3, Hn = N diag(m_i / l_i) adj(M), then T2^-1 Hn T1 over its
+// Frobenius norm. False if some l_i is 0 or a result is not finite. One lane.
+__device__ __forceinline__ void cross3(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
 template <class S>
-__device__ void homography_accumulate(const S& s, uint32_t len, double* t1, double* t2,
-                                      double* ata) {
-    normalizer(s, false, t1, len);
-    normalizer(s, true, t2, len);
+__device__ bool homography_4pt(const S& s, double* H) {
+    double t1[9], t2[9];
+    normalizer(s, false, t1, 4);
+    normalizer(s, true, t2, 4);
+    double p[4][3], q[4][3];
 #pragma unroll
-    for (int i = 0; i < 45; ++i) ata[i] = 0.0;
-    for (uint32_t i = 0; i < s.n; ++i) {
-        if (!s.take(i)) continue;
-        double2 pp = s.p(i), qq = s.q(i);
+    for (int i = 0; i < 4; ++i) {
+        const double2 pp = s.p(i), qq = s.q(i);
         double p3[3] = {pp.x, pp.y, 1.0}, v[3];
         mv3(t1, p3, v);
-        const double x = v[0] / v[2], y = v[1] / v[2];
+        p[i][0] = v[0] / v[2];
+        p[i][1] = v[1] / v[2];
+        p[i][2] = 1.0;
         double q3[3] = {qq.x, qq.y, 1.0};
         mv3(t2, q3, v);
-        const double u = v[0] / v[2], w = v[1] / v[2];
-        double r1[9] = {-x, -y, -1.0, 0.0, 0.0, 0.0, u * x, u * y, u};
-        double r2[9] = {0.0, 0.0, 0.0, -x, -y, -1.0, w * x, w * y, w};
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-#pragma unroll
-            for (int c = r; c < 9; ++c)
-                ata[ut(r, c)] = ata[ut(r, c)] + (r1[r] * r1[c] + r2[r] * r2[c]);
-        __builtin_amdgcn_sched_barrier(0);
+        q[i][0] = v[0] / v[2];
+        q[i][1] = v[1] / v[2];
+        q[i][2] = 1.0;
     }
+    double am[3][3], an[3][3], w[3];
+    cross3(p[1], p[2], am[0]);
+    cross3(p[2], p[0], am[1]);
+    cross3(p[0], p[1], am[2]);
+    cross3(q[1], q[2], an[0]);
+    cross3(q[2], q[0], an[1]);
+    cross3(q[0], q[1], an[2]);
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double l = am[i][0] * p[3][0] + am[i][1] * p[3][1] + am[i][2] * p[3][2];
+        const double m = an[i][0] * q[3][0] + an[i][1] * q[3][1] + an[i][2] * q[3][2];
+        ok = ok && l != 0.0;
+        w[i] = m / l;
+    }
+    if (!ok) return false;
+    double hn[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double t0 = q[0][r] * w[0], tt1 = q[1][r] * w[1], tt2 = q[2][r] * w[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            hn[3 * r + c] = t0 * am[0][c] + tt1 * am[1][c] + tt2 * am[2][c];
+            ok = ok && isfinite(hn[3 * r + c]);
+        }
+    }
+    if (!ok) return false;
+    double t2i[9], tmp[9], h[9];
+    if (!inv3(t2, t2i)) return false;
+    mm3(t2i, hn, tmp);
+    mm3(tmp, t1, h);
+    const double nn = fro3(h);
+    if (!(isfinite(nn) && nn > 0.0)) return false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = h[i] / nn;
+    return true;
 }
 
 // Eigen-solve and denormalisation of a DLT normal matrix by a solver group.
@@ -1012,10 +1054,9 @@ __device__ __noinline__ void hom_round(const double2* x1, const double2* x2, con
     const int h = kGroups * round + g;
     if (g >= kGroups) return;
     PtsRank4 s4{x1, x2, list, sh.idx[h]};
-    double t1[9], t2[9], ata[45], hh[9], hi[9];
-    homography_accumulate(s4, 4, t1, t2, ata);
+    double hh[9], hi[9];
     int c = -1;
-    if (homography_finish_group(ata, t1, t2, hh) && inv3(hh, hi)) {
+    if (homography_4pt(s4, hh) && inv3(hh, hi)) {
         // ranks part, part + 3, ...: four per step, loads issued together
         uint32_t cc = 0, r = part;
         for (; r + 9 < ns; r += 12) {

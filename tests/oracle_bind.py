@@ -168,6 +168,8 @@ _SIGS_R = {
     "orc_normalizer": (None, [_vp, _sz, _vp]),
     "orc_refine_sampson": (ctypes.c_int, [_vp, _vp, _vp, _sz, _sz, _vp, _vp, _vp]),
     "orc_homography_support": (_u64, [_vp, _vp, _sz, _d]),
+    "orc_homography_dlt": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "orc_homography_4pt": (ctypes.c_int, [_vp, _vp, _vp]),
     "orc_select_two_view_model": (ctypes.c_int, [_vp, _vp, _sz, _vp, _d, _vp]),
     "orc_ransac_fundamental": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "orc_ransac_last_stats": (None, [_vp, _vp, _vp, _vp, _vp]),
@@ -238,6 +240,21 @@ def normalizer(p):
     T = np.zeros(9)
     lib().orc_normalizer(P(p), p.shape[0], P(T))
     return T.reshape(3, 3)
+
+
+def homography_dlt(x1, x2):
+    x1, x2 = _xy(x1), _xy(x2)
+    H = np.zeros(9)
+    ok = lib().orc_homography_dlt(P(x1), P(x2), x1.shape[0], P(H))
+    return H.reshape(3, 3) if ok else None
+
+
+def homography_4pt(x1, x2):
+    x1, x2 = _xy(x1), _xy(x2)
+    assert x1.shape[0] == 4 and x2.shape[0] == 4
+    H = np.zeros(9)
+    ok = lib().orc_homography_4pt(P(x1), P(x2), P(H))
+    return H.reshape(3, 3) if ok else None
 
 
 def homography_support(x1, x2, th):

@@ -71,6 +71,40 @@ def test_eigen_min_against_lapack():
             assert min(np.linalg.norm(v - V[:, 0]), np.linalg.norm(v + V[:, 0])) < 1e-6
 
 
+def test_homography_4pt_closed_form():
+    # four correspondences fix H: the closed form returns the generating H (up to
+    # sign, unit norm), maps every sample point onto its partner, equals the exact
+    # null vector of the 8x9 DLT system (numpy SVD of the unnormalised system, to
+    # its own accuracy), agrees with the A^T A
+    # eigenvector route within that route's own error (it squares the condition
+    # number), and rejects collinear triples
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        x1 = rng.uniform(0, 1000, (4, 2))
+        Ht = np.eye(3) + rng.normal(0, 0.2, (3, 3))
+        Ht[2, :2] *= 1e-3
+        q = np.c_[x1, np.ones(4)] @ Ht.T
+        x2 = q[:, :2] / q[:, 2:]
+        H = ob.homography_4pt(x1, x2)
+        assert H is not None
+        assert abs(np.linalg.norm(H) - 1.0) < 1e-12
+        T = Ht / np.linalg.norm(Ht)
+        assert min(np.linalg.norm(H - T), np.linalg.norm(H + T)) < 1e-8
+        p = np.c_[x1, np.ones(4)] @ H.T
+        assert np.max(np.abs(p[:, :2] / p[:, 2:] - x2)) < 1e-8
+        G = ob.homography_dlt(x1, x2)
+        assert min(np.linalg.norm(H - G), np.linalg.norm(H + G)) < 1e-3
+        A = []
+        for (x, y), (u, v) in zip(x1, x2):
+            A.append([-x, -y, -1, 0, 0, 0, u * x, u * y, u])
+            A.append([0, 0, 0, -x, -y, -1, v * x, v * y, v])
+        h0 = np.linalg.svd(np.array(A))[2][-1]
+        h0 = h0 / np.linalg.norm(h0)
+        assert min(np.linalg.norm(H.ravel() - h0), np.linalg.norm(H.ravel() + h0)) < 1e-6
+    col = np.array([[0.0, 0.0], [1.0, 1.0], [2.0, 2.0], [5.0, 1.0]])
+    assert ob.homography_4pt(col, col + 3.0) is None
+
+
 def test_eigen_min_null_vectors():
     # rank-deficient A = X^T X (8 rows): the vector spans the null space of X,
     # also for a tridiagonal form whose eliminations pivot (graded rows) and
