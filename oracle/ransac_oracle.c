@@ -12,6 +12,9 @@
  *   - symmetric 9x9 eigen: Householder tridiagonalisation and implicit QL
  *     (the scheme of nalgebra's SymmetricEigen: reduction to tridiagonal
  *     form, then shifted implicit iterations);
+ *   - homography of a four-point sample (two_view.rs:607-642 on 4 points):
+ *     the exact null vector of its DLT system in closed form (projective
+ *     basis construction) instead of the A^T A eigenvector;
  *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958),
  *     inside the Sampson LM started from the current F's singular basis;
  *   - Cholesky: right-looking column factorisation, forward/back solves;
