@@ -9,15 +9,17 @@
  * 577), Cholesky (:714), try_inverse (:263, :340, :359, :675). nalgebra is not
  * vendored and cannot be built here, so the oracle uses its own solvers with
  * the same mathematical contract:
- *   - symmetric 9x9 eigen: Householder tridiagonalisation and implicit QL
- *     (the scheme of nalgebra's SymmetricEigen: reduction to tridiagonal
- *     form, then shifted implicit iterations);
+ *   - symmetric 9x9 eigen: Householder tridiagonalisation, the eigenvalues
+ *     by implicit QL (nalgebra's SymmetricEigen also reduces to tridiagonal
+ *     form and iterates with shifts), the smallest one's eigenvector by
+ *     inverse iteration on the tridiagonal form;
  *   - homography of a four-point sample (two_view.rs:607-642 on 4 points):
  *     the exact null vector of its DLT system in closed form (projective
  *     basis construction) instead of the A^T A eigenvector;
  *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958),
  *     inside the Sampson LM started from the current F's singular basis;
- *   - Cholesky: right-looking column factorisation, forward/back solves;
+ *   - Cholesky: right-looking column factorisation, forward/back solves
+ *     (divisions by the diagonal through its reciprocal);
  *   - 3x3 inverse: cofactor expansion.
  * The GPU performs the same operations in the same order and produces the
  * same bits. The F bits therefore pin the HIP path to this oracle, not to
