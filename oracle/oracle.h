@@ -115,6 +115,11 @@ void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_call
  * two_view.rs:638-639): the nine eigenvalues and a unit eigenvector of the
  * first smallest one. */
 void orc_eigen_min9(const double* A, double* evals, double* evec);
+/* The 8-point solver's eigen step (matching.rs:539-546): 1 if the normal
+ * matrix is degenerate (second smallest eigenvalue at most 1e-10 of the
+ * largest, Gershgorin-bounded), else 0 and a unit eigenvector of the smallest
+ * eigenvalue by inverse iteration. */
+int orc_eigen_min_inv9(const double* A, double* evec);
 
 /* ---- test-geometry generators (matching.rs:1165-1220, camera.rs:41-52,123-159) ---- */
 /* two_cameras(): K from_hfov(960,540,70deg), poses as in matching.rs:1165-1178.

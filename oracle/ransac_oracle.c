@@ -308,6 +308,119 @@ static void tinv9(const double* d, const double* e, double lam, double* x) {
     }
 }
 
+/* The eigenvector of the smallest eigenvalue of the symmetric positive
+ * semidefinite 9x9 A (an 8-point normal matrix) without the full eigen-solve.
+ * Returns 1 if the system is degenerate (fundamental_8pt's test,
+ * matching.rs:541-545: the second smallest eigenvalue at most 1e-10 of the
+ * largest), else 0 with a unit eigenvector in vec.
+ *   T = Q^T A Q (tridiag9). Degeneracy: the eigenvalues of T below
+ *   tau = 1e-10 G, G = max_i |d_i| + |e_{i-1}| + |e_i| >= lambda_max
+ *   (Gershgorin), counted by the Sturm sequence q_0 = d_0 - tau,
+ *   q_i = (d_i - tau) - e_{i-1}^2 / q_{i-1} (|q_i| < 2^-1000 taken as
+ *   -2^-1000): two or more negative q_i. Against lambda_max itself this also
+ *   rejects the systems with lambda_2 / lambda_max in (1e-10, 1e-10 G /
+ *   lambda_max], where G / lambda_max <= 3.
+ *   Eigenvector: inverse iteration on T with shift 0 (the smallest eigenvalue
+ *   of a semidefinite matrix is the one nearest 0): T = P L U as in tinv9,
+ *   x_1 = unit(U^-1 (1, ..., 1)), x_{k+1} = unit((P L U)^-1 x_k) until two
+ *   successive iterates agree to 2^-44 in every component (up to sign), at
+ *   most 30 steps; vec = Q x. The error shrinks by lambda_1 / lambda_2 per
+ *   step: one or two steps for an exact 8-point sample, a few for a
+ *   least-squares refit. */
+#define INV_MAX_STEPS 30
+static int eigen_min_inv9(const double* A, double* vec) {
+    double a[81], q[81], d[9], e[9];
+    memcpy(a, A, sizeof a);
+    tridiag9(a, q, d, e);
+    double g = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        double rs = fabs(d[i]);
+        if (i < 8) rs = rs + fabs(e[i]);
+        if (i > 0) rs = rs + fabs(e[i - 1]);
+        g = fmax(g, rs);
+    }
+    const double tau = 1e-10 * g;
+    int neg = 0;
+    double qs = d[0] - tau;
+    for (int i = 0;; ++i) {
+        if (fabs(qs) < 0x1p-1000) qs = -0x1p-1000;
+        if (qs < 0.0) ++neg;
+        if (i == 8) break;
+        qs = (d[i + 1] - tau) - e[i] * e[i] / qs;
+    }
+    if (neg >= 2) return 1;
+    /* T = P L U with partial pivoting (tinv9's elimination, shift 0) */
+    const double eps3 = g * 0x1p-52;
+    double dd[9], du[8], du2[7], dl[8], ui[9];
+    int swp[8];
+    for (int i = 0; i < 9; ++i) dd[i] = d[i];
+    for (int i = 0; i < 8; ++i) du[i] = dl[i] = e[i];
+    for (int i = 0; i < 8; ++i) {
+        if (fabs(dd[i]) >= fabs(dl[i])) {
+            swp[i] = 0;
+            const double fact = dd[i] != 0.0 ? dl[i] / dd[i] : 0.0;
+            dl[i] = fact;
+            dd[i + 1] = dd[i + 1] - fact * du[i];
+            if (i < 7) du2[i] = 0.0;
+        } else {
+            swp[i] = 1;
+            const double fact = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = fact;
+            const double t = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = t - fact * dd[i + 1];
+            if (i < 7) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -fact * du[i + 1];
+            }
+        }
+    }
+    for (int i = 0; i < 9; ++i) {
+        double u = dd[i];
+        if (fabs(u) < eps3) u = u >= 0.0 ? eps3 : -eps3;
+        ui[i] = 1.0 / u;
+    }
+    double b[9], x[9];
+    for (int i = 0; i < 9; ++i) b[i] = 1.0;
+    for (int it = 0; it < INV_MAX_STEPS; ++it) {
+        if (it > 0)
+            for (int i = 0; i < 8; ++i) {
+                if (swp[i]) {
+                    const double t = b[i];
+                    b[i] = b[i + 1];
+                    b[i + 1] = t - dl[i] * b[i];
+                } else {
+                    b[i + 1] = b[i + 1] - dl[i] * b[i];
+                }
+            }
+        x[8] = b[8] * ui[8];
+        x[7] = (b[7] - du[7] * x[8]) * ui[7];
+        for (int i = 6; i >= 0; --i) x[i] = (b[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) * ui[i];
+        unit9(x);
+        int conv = 0;
+        if (it > 0) {
+            double dot = 0.0;
+            for (int i = 0; i < 9; ++i) dot = dot + x[i] * vec[i];
+            const double sg = dot >= 0.0 ? 1.0 : -1.0;
+            double dm = 0.0;
+            for (int i = 0; i < 9; ++i) dm = fmax(dm, fabs(x[i] - sg * vec[i]));
+            conv = dm <= 0x1p-44;
+        }
+        memcpy(vec, x, sizeof x); /* vec: the previous unit iterate */
+        memcpy(b, x, sizeof b);
+        if (conv) break;
+    }
+    double y[9];
+    for (int i = 0; i < 9; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 9; ++j) s = s + q[9 * i + j] * vec[j];
+        y[i] = s;
+    }
+    memcpy(vec, y, sizeof y);
+    return 0;
+}
+
 /* The eigenvalues of the symmetric 9x9 A (ev, in the order the QL sweep
  * leaves them) and a unit eigenvector of the first smallest one (vec):
  * tridiagonalisation T = Q^T A Q, eigenvalues of T by QL, the eigenvector
@@ -330,6 +443,7 @@ static void eigen_min9(const double* A, double* ev, double* vec) {
 }
 
 void orc_eigen_min9(const double* A, double* evals, double* evec) { eigen_min9(A, evals, evec); }
+int orc_eigen_min_inv9(const double* A, double* evec) { return eigen_min_inv9(A, evec); }
 
 static int imin9(const double* e) {
     int k = 0;
@@ -486,12 +600,8 @@ int orc_fundamental_8pt(const double* x1, const double* x2, size_t n, double* F)
         for (int r = 0; r < 9; ++r)
             for (int c = 0; c < 9; ++c) ata[9 * r + c] = ata[9 * r + c] + row[r] * row[c];
     }
-    double ev[9], vec[9];
-    eigen_min9(ata, ev, vec);
-    double sorted[9];
-    memcpy(sorted, ev, sizeof sorted);
-    qsort(sorted, 9, sizeof(double), cmp_double);
-    if (sorted[1] <= 1e-10 * sorted[8]) return 0;
+    double vec[9];
+    if (eigen_min_inv9(ata, vec)) return 0;
     double fn[9];
     for (int i = 0; i < 9; ++i) fn[i] = vec[i];
     double r2[9];

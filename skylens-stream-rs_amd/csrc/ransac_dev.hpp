@@ -726,26 +726,141 @@ __device__ __forceinline__ int eigen_min_group(double* a, double* d, double* vk)
     return k;
 }
 
+// The eigenvector of the smallest eigenvalue of a semidefinite 8-point normal
+// matrix `a` (upper triangle, 45, overwritten) by a solver group, without the
+// full eigen-solve (the host's eigen_min_inv9 step for step). Returns true if
+// the system is degenerate (matching.rs:541-545), else false with the unit
+// eigenvector in vk in every lane of the group:
+//   T = Q^T A Q (each lane accumulating three rows of Q); degenerate when the
+//   Sturm sequence of T - tau I, tau = 1e-10 G (G: Gershgorin bound of T),
+//   has two or more negative terms; the eigenvector by inverse iteration on T
+//   with shift 0 (partial-pivoting LU, pivots below 2^-52 G raised to it,
+//   x_1 = unit(U^-1 1), x_{k+1} = unit((P L U)^-1 x_k)) until two iterates
+//   agree to 2^-44 per component up to sign (at most 30 steps); vk = Q x.
+// All three lanes of the group must be active with the same `a`.
+__device__ __forceinline__ bool eigen_min_inv_group(double* a, double* vk) {
+    const int part = solver_part();
+    const int base = 3 * solver_group();
+    double q[3][9], d[9], e[9];
+    tridiag9_rows<3>(a, 3 * part, q, d, e);
+    double g = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double rs = fabs(d[i]);
+        if (i < 8) rs = rs + fabs(e[i]);
+        if (i > 0) rs = rs + fabs(e[i - 1]);
+        g = fmax(g, rs);
+    }
+    const double tau = 1e-10 * g;
+    int neg = 0;
+    double qs = d[0] - tau;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (fabs(qs) < 0x1p-1000) qs = -0x1p-1000;
+        if (qs < 0.0) ++neg;
+        if (i < 8) qs = (d[i + 1] - tau) - e[i] * e[i] / qs;
+    }
+    const bool degenerate = neg >= 2;
+    double x[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!degenerate) {
+        const double eps3 = g * 0x1p-52;
+        double dd[9], du[8], du2[7], dl[8], ui[9];
+        bool swp[8];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dd[i] = d[i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) du[i] = dl[i] = e[i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bool keep = fabs(dd[i]) >= fabs(dl[i]);
+            swp[i] = !keep;
+            if (keep) {
+                const double fact = dd[i] != 0.0 ? dl[i] / dd[i] : 0.0;
+                dl[i] = fact;
+                dd[i + 1] = dd[i + 1] - fact * du[i];
+                if (i < 7) du2[i] = 0.0;
+            } else {
+                const double fact = dd[i] / dl[i];
+                dd[i] = dl[i];
+                dl[i] = fact;
+                const double t = du[i];
+                du[i] = dd[i + 1];
+                dd[i + 1] = t - fact * dd[i + 1];
+                if (i < 7) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -fact * du[i + 1];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            double u = dd[i];
+            if (fabs(u) < eps3) u = u >= 0.0 ? eps3 : -eps3;
+            ui[i] = 1.0 / u;
+        }
+        double b[9], prev[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) b[i] = 1.0;
+        for (int it = 0; it < 30; ++it) {
+            if (it > 0) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (swp[i]) {
+                        const double t = b[i];
+                        b[i] = b[i + 1];
+                        b[i + 1] = t - dl[i] * b[i];
+                    } else {
+                        b[i + 1] = b[i + 1] - dl[i] * b[i];
+                    }
+                }
+            }
+            x[8] = b[8] * ui[8];
+            x[7] = (b[7] - du[7] * x[8]) * ui[7];
+#pragma unroll
+            for (int i = 6; i >= 0; --i)
+                x[i] = (b[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) * ui[i];
+            unit9(x);
+            bool conv = false;
+            if (it > 0) {
+                double dot = 0.0;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) dot = dot + x[i] * prev[i];
+                const double sg = dot >= 0.0 ? 1.0 : -1.0;
+                double dm = 0.0;
+#pragma unroll
+                for (int i = 0; i < 9; ++i) dm = fmax(dm, fabs(x[i] - sg * prev[i]));
+                conv = dm <= 0x1p-44;
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                prev[i] = x[i];
+                b[i] = x[i];
+            }
+            if (conv) break;
+        }
+    }
+    // this lane's three rows of Q x
+    double mine[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc = acc + q[i][j] * x[j];
+        mine[i] = acc;
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vk[3 * t + i] = __shfl(mine[i], base + t);
+    return degenerate;
+}
+
 // matching.rs:539-556 computed by a solver group (see eigen_min_group):
 // the eigen-solve, the degeneracy test, the rank-2 projection and the
 // denormalisation. ata is overwritten.
 __device__ bool solve_8pt_group(double* ata, const double* t1, const double* t2, double* F) {
-    double fn[9], ev[9];
-    eigen_min_group(ata, ev, fn);
-    // second smallest and largest of the nine eigenvalues
-    double e1 = INFINITY, e0 = INFINITY, e8 = -INFINITY;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        double x = ev[i];
-        if (x < e0) {
-            e1 = e0;
-            e0 = x;
-        } else if (x < e1) {
-            e1 = x;
-        }
-        if (x > e8) e8 = x;
-    }
-    if (e1 <= 1e-10 * e8) return false;
+    double fn[9];
+    if (eigen_min_inv_group(ata, fn)) return false;
     double r2[9];
     rank2_recompose(fn, r2);
     double t2t[9], tmp[9], f[9];

@@ -174,6 +174,7 @@ _SIGS_R = {
     "orc_ransac_fundamental": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "orc_ransac_last_stats": (None, [_vp, _vp, _vp, _vp, _vp]),
     "orc_eigen_min9": (None, [_vp, _vp, _vp]),
+    "orc_eigen_min_inv9": (ctypes.c_int, [_vp, _vp]),
     "orc_two_cameras": (None, [_vp, _vp]),
     "orc_project": (ctypes.c_int, [_vp, _vp, _vp]),
     "orc_fundamental_from_cameras": (None, [_vp, _vp, _vp]),
@@ -294,6 +295,14 @@ def ransac_last_stats():
     lib().orc_ransac_last_stats(*[P(x) for x in v])
     return dict(zip(["iters", "lo_runs", "lm_calls", "lm_converged", "lm_stalled_at_start"],
                     [int(x[0]) for x in v]))
+
+
+def eigen_min_inv9(A):
+    """(degenerate, unit eigenvector of the smallest eigenvalue or None)."""
+    A = np.ascontiguousarray(A, np.float64).reshape(81)
+    vec = np.zeros(9)
+    deg = lib().orc_eigen_min_inv9(P(A), P(vec))
+    return bool(deg), (None if deg else vec)
 
 
 def eigen_min9(A):

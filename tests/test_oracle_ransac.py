@@ -105,6 +105,34 @@ def test_homography_4pt_closed_form():
     assert ob.homography_4pt(col, col + 3.0) is None
 
 
+def test_eigen_min_inverse_iteration():
+    # the 8-point solver's eigen step (inverse iteration, error ~ (l1/l2)^steps):
+    # LAPACK's smallest eigenvector (up to sign) when the gap is wide, a small
+    # residual when l1 < l2 / 2, a Rayleigh quotient below l2 always; degenerate
+    # exactly when two eigenvalues sit below 1e-10 of the largest
+    rng = np.random.default_rng(4)
+    for trial in range(100):
+        X = rng.standard_normal((12, 9)) * rng.uniform(0.1, 10, 9)
+        A = X.T @ X
+        deg, v = ob.eigen_min_inv9(A)
+        assert not deg
+        w, V = np.linalg.eigh(A)
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-13
+        assert v @ A @ v < w[1]
+        if w[0] < 0.5 * w[1]:
+            assert np.linalg.norm(A @ v - w[0] * v) < 1e-9 * w[-1]
+        if w[0] < 0.1 * w[1]:
+            assert min(np.linalg.norm(v - V[:, 0]), np.linalg.norm(v + V[:, 0])) < 1e-10
+    Q, _ = np.linalg.qr(rng.standard_normal((9, 9)))
+    for lam2, want in ((1e-11, True), (1e-9, False), (0.0, True)):
+        lam = np.array([0.0, lam2, 0.3, 0.5, 0.7, 0.8, 0.9, 0.95, 1.0])
+        deg, v = ob.eigen_min_inv9((Q * lam) @ Q.T)
+        assert deg == want, lam2
+    lam = np.array([0.0, 1e-3, 0.3, 0.5, 0.7, 0.8, 0.9, 0.95, 1.0])
+    deg, v = ob.eigen_min_inv9((Q * lam) @ Q.T)
+    assert not deg and min(np.linalg.norm(v - Q[:, 0]), np.linalg.norm(v + Q[:, 0])) < 1e-9
+
+
 def test_eigen_min_null_vectors():
     # rank-deficient A = X^T X (8 rows): the vector spans the null space of X,
     # also for a tridiagonal form whose eliminations pivot (graded rows) and
