@@ -9,10 +9,13 @@
  * 577), Cholesky (:714), try_inverse (:263, :340, :359, :675). nalgebra is not
  * vendored and cannot be built here, so the oracle uses its own solvers with
  * the same mathematical contract:
- *   - symmetric 9x9 eigen: Householder tridiagonalisation, the eigenvalues
- *     by implicit QL (nalgebra's SymmetricEigen also reduces to tridiagonal
- *     form and iterates with shifts), the smallest one's eigenvector by
- *     inverse iteration on the tridiagonal form;
+ *   - 8-point solve (matching.rs:539-546): Householder tridiagonalisation,
+ *     the degeneracy test by a Sturm count, the smallest eigenvector by
+ *     inverse iteration with shift 0 (no eigen-solve);
+ *   - symmetric 9x9 eigen for the DLT refits: Householder
+ *     tridiagonalisation, the eigenvalues by implicit QL (nalgebra's
+ *     SymmetricEigen also reduces to tridiagonal form and iterates with
+ *     shifts), the smallest one's eigenvector by inverse iteration;
  *   - homography of a four-point sample (two_view.rs:607-642 on 4 points):
  *     the exact null vector of its DLT system in closed form (projective
  *     basis construction) instead of the A^T A eigenvector;
