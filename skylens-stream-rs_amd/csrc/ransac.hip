@@ -55,6 +55,27 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Orders this wave's LDS accesses before the following ones (a staged
+// chunk's LDS writes before the lanes read each other's entries) without
+// waiting for its outstanding device-memory loads: LDS operations of one wave
+// are performed in program order, so this only waits for the wave's own LDS
+// operations to finish and keeps the compiler from moving LDS accesses across.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt not waited on
+    __builtin_amdgcn_wave_barrier();
+}
+// Loads from device (global) memory through the global address space: the
+// compiler then issues global loads, which only the vector-memory counter
+// tracks, instead of flat loads, which the LDS counter tracks too and which
+// force full waits before every LDS access that follows them.
+__device__ __forceinline__ double2 ldg2(const double2* p, uint32_t i) {
+    const __attribute__((address_space(1))) double* g =
+        (const __attribute__((address_space(1))) double*)p;
+    return make_double2(g[2 * (size_t)i], g[2 * (size_t)i + 1]);
+}
+__device__ __forceinline__ uint32_t ldg(const uint32_t* p, uint32_t i) {
+    return ((const __attribute__((address_space(1))) uint32_t*)p)[i];
+}
 __device__ __forceinline__ void load9(const double* src, double* dst) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) dst[i] = src[i];
@@ -382,7 +403,7 @@ enum CacheMode { kNoCache = 0, kFill = 1, kUse = 2 };
 // for each lane holding a point of the chunk (slot = lane), then, once the
 // wave has staged, reduce(cnt) over the chunk's cnt points. Without the cache,
 // the next chunk's points and the indices two chunks ahead are loaded while a
-// chunk is processed. kFill / kUse require len <= kCacheN.
+// chunk is summed. kFill / kUse require len <= kCacheN.
 template <class Stage, class Reduce>
 __device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
                                           const uint32_t* list, uint32_t len, Stage&& stage,
@@ -393,28 +414,22 @@ __device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
         for (uint32_t base = 0; base < len; base += NL) {
             const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
             if (lane < cnt) stage(lane, pc.p[base + lane], pc.q[base + lane]);
-            wave_sync();
+            lds_sync();
             reduce(cnt);
-            wave_sync();
+            lds_sync();
         }
         return;
     }
-    double2 p = make_double2(0.0, 0.0), q = p;
-    if (lane < len) {
-        const uint32_t i0 = list[lane];
-        p = x1[i0];
-        q = x2[i0];
-    }
-    uint32_t inext = lane + NL < len ? list[lane + NL] : 0u;
+    if (len == 0) return;
+    // every load is issued unconditionally at an index clamped into the list
+    // (lanes past the end reload its last point, unused), so no lane-masked
+    // branch separates a load from the wait for it
+    const uint32_t last = len - 1;
+    uint32_t inext = ldg(list, lane < last ? lane : last);
+    double2 p = ldg2(x1, inext), q = ldg2(x2, inext);
+    inext = ldg(list, lane + NL < last ? lane + NL : last);
     for (uint32_t base = 0; base < len; base += NL) {
         const uint32_t cnt = len - base < (uint32_t)NL ? len - base : (uint32_t)NL;
-        double2 pn = make_double2(0.0, 0.0), qn = pn;
-        if (base + NL + lane < len) {
-            pn = x1[inext];
-            qn = x2[inext];
-        }
-        const uint32_t j2 = base + 2 * NL + lane;
-        const uint32_t inext2 = j2 < len ? list[j2] : 0u;
         if (lane < cnt) {
             if (mode == kFill) {
                 pc.p[base + lane] = p;
@@ -422,12 +437,15 @@ __device__ __forceinline__ void list_pass(const double2* x1, const double2* x2,
             }
             stage(lane, p, q);
         }
-        wave_sync();
+        // the next chunk's points and the indices two chunks ahead load while
+        // this chunk is summed
+        p = ldg2(x1, inext);
+        q = ldg2(x2, inext);
+        const uint32_t j2 = base + 2 * NL + lane;
+        inext = ldg(list, j2 < last ? j2 : last);
+        lds_sync();
         reduce(cnt);
-        wave_sync();
-        p = pn;
-        q = qn;
-        inext = inext2;
+        lds_sync();
     }
 }
 
@@ -454,15 +472,15 @@ __device__ uint32_t coop_masks2(const double* F, const double2* x1, const double
     // the next chunk's points are loaded while this chunk's errors are computed
     double2 p = make_double2(0.0, 0.0), q = p;
     if ((uint32_t)lane < n) {
-        p = x1[lane];
-        q = x2[lane];
+        p = ldg2(x1, lane);
+        q = ldg2(x2, lane);
     }
     for (uint32_t base = 0; base < n; base += NL) {
         const uint32_t i = base + lane, inext = i + NL;
         double2 pn = make_double2(0.0, 0.0), qn = pn;
         if (inext < n) {
-            pn = x1[inext];
-            qn = x2[inext];
+            pn = ldg2(x1, inext);
+            qn = ldg2(x2, inext);
         }
         bool a = false, b = false;
         if (i < n) {
