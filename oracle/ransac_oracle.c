@@ -21,8 +21,8 @@
  *     basis construction) instead of the A^T A eigenvector;
  *   - 3x3 SVD for the rank-2 projection: one-sided Jacobi (Hestenes 1958),
  *     inside the Sampson LM started from the current F's singular basis;
- *   - Cholesky: right-looking column factorisation, forward/back solves
- *     (divisions by the diagonal through its reciprocal);
+ *   - Cholesky solve of the LM step: as LDL^T (no square roots),
+ *     right-looking column factorisation, forward/back solves;
  *   - 3x3 inverse: cofactor expansion.
  * The GPU performs the same operations in the same order and produces the
  * same bits. The F bits therefore pin the HIP path to this oracle, not to
@@ -678,35 +678,35 @@ static double sampson_cost(const double* g, double s1, double s2, const double* 
     return cost;
 }
 
-/* 9x9 Cholesky (nalgebra's column-by-column, right-looking update) and solve
- * m x = rhs; every division by a diagonal entry L_jj is a multiplication by
- * its reciprocal (one division per column). Returns 0 if a pivot is not
- * positive. */
+/* Solve m x = rhs for the symmetric 9x9 m of an LM step (matching.rs:714,
+ * nalgebra's Cholesky there) by its LDL^T factorisation, column by column
+ * with right-looking updates: L unit lower, D diagonal (the Cholesky factor
+ * is L D^1/2, so no square root is taken); then L y = rhs, y := D^-1 y
+ * (through the reciprocals of D), L^T x = y. Returns 0 if a pivot D_j is
+ * not positive: m is not positive definite, where the reference's Cholesky
+ * fails. */
 static int cholesky_solve9(const double* m_in, const double* rhs, double* x) {
-    double L[81], rd[9];
-    memcpy(L, m_in, sizeof L);
+    double A[81], D[9], rd[9];
+    memcpy(A, m_in, sizeof A);
     for (int j = 0; j < 9; ++j) {
         for (int k = 0; k < j; ++k) {
-            double factor = -L[9 * j + k];
-            for (int i = j; i < 9; ++i) L[9 * i + j] = L[9 * i + j] + factor * L[9 * i + k];
+            const double w = A[9 * j + k] * D[k];
+            for (int i = j; i < 9; ++i) A[9 * i + j] = A[9 * i + j] - w * A[9 * i + k];
         }
-        double diag = L[9 * j + j];
-        if (!(diag > 0.0)) return 0;
-        double den = sqrt(diag);
-        L[9 * j + j] = den;
-        rd[j] = 1.0 / den;
-        for (int i = j + 1; i < 9; ++i) L[9 * i + j] = L[9 * i + j] * rd[j];
+        D[j] = A[9 * j + j];
+        if (!(D[j] > 0.0)) return 0;
+        rd[j] = 1.0 / D[j];
+        for (int i = j + 1; i < 9; ++i) A[9 * i + j] = A[9 * i + j] * rd[j];
     }
     double y[9];
     memcpy(y, rhs, sizeof y);
-    for (int i = 0; i < 9; ++i) {
-        y[i] = y[i] * rd[i];
-        for (int k = i + 1; k < 9; ++k) y[k] = y[k] - L[9 * k + i] * y[i];
-    }
+    for (int i = 0; i < 9; ++i)
+        for (int k = i + 1; k < 9; ++k) y[k] = y[k] - A[9 * k + i] * y[i];
+    for (int i = 0; i < 9; ++i) y[i] = y[i] * rd[i];
     for (int i = 8; i >= 0; --i) {
         double dot = 0.0;
-        for (int k = i + 1; k < 9; ++k) dot = dot + L[9 * k + i] * y[k];
-        y[i] = (y[i] - dot) * rd[i];
+        for (int k = i + 1; k < 9; ++k) dot = dot + A[9 * k + i] * y[k];
+        y[i] = y[i] - dot;
     }
     memcpy(x, y, sizeof y);
     return 1;

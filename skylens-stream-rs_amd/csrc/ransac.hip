@@ -97,30 +97,28 @@ struct Pts4 {
 
 // ---------------------------------------------------------------- Cholesky
 
-// Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs
-// (right-looking Cholesky, then forward and back substitution; divisions by
-// a diagonal entry are multiplications by its reciprocal). Fully
-// expanded at compile time: every index is a constant, so L stays in
-// registers. A non-positive pivot makes it return false (the remaining
-// columns are computed but unused).
+// Lower triangle of a symmetric 9x9 in ut() slots; solves m x = rhs by its
+// LDL^T factorisation (the host's cholesky_solve9 step for step: right-
+// looking column updates, unit lower L, diagonal D and its reciprocals, then
+// L y = rhs, y *= D^-1, L^T x = y). Fully expanded at compile time: every
+// index is a constant, so L stays in registers. A non-positive pivot makes it
+// return false (the remaining columns are computed but unused).
 __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
     bool ok = true;
-    double rd[9];  // reciprocals of the diagonal
+    double D[9], rd[9];
     sfor<0, 9>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         sfor<0, j>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            const double factor = -L[ut(j, k)];
+            const double w = L[ut(j, k)] * D[k];
             sfor<j, 9>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
-                L[ut(i, j)] = L[ut(i, j)] + factor * L[ut(i, k)];
+                L[ut(i, j)] = L[ut(i, j)] - w * L[ut(i, k)];
             });
         });
-        const double diag = L[ut(j, j)];
-        ok = ok && diag > 0.0;
-        const double den = sqrt(diag);
-        L[ut(j, j)] = den;
-        rd[j] = 1.0 / den;
+        D[j] = L[ut(j, j)];
+        ok = ok && D[j] > 0.0;
+        rd[j] = 1.0 / D[j];
         sfor<j + 1, 9>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
             L[ut(i, j)] = L[ut(i, j)] * rd[j];
@@ -132,12 +130,13 @@ __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
     for (int i = 0; i < 9; ++i) y[i] = rhs[i];
     sfor<0, 9>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        y[i] = y[i] * rd[i];
         sfor<i + 1, 9>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             y[k] = y[k] - L[ut(k, i)] * y[i];
         });
     });
+#pragma unroll
+    for (int i = 0; i < 9; ++i) y[i] = y[i] * rd[i];
     sfor<0, 9>([&](auto ic) {
         constexpr int i = 8 - decltype(ic)::value;
         double dot = 0.0;
@@ -145,7 +144,7 @@ __device__ bool cholesky_solve9(double* L, const double* rhs, double* x) {
             constexpr int k = decltype(kc)::value;
             dot = dot + L[ut(k, i)] * y[k];
         });
-        y[i] = (y[i] - dot) * rd[i];
+        y[i] = y[i] - dot;
     });
 #pragma unroll
     for (int i = 0; i < 9; ++i) x[i] = y[i];
