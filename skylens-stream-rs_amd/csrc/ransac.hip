@@ -1409,10 +1409,10 @@ __device__ __noinline__ void local_opt_batch(const RansacTask& t, HypMem& hm, ui
                     }
                     swapped ^= 1ull << h;
                 }
-                // From round 2 on the selection thresholds of this and every later round are
-                // equal (kNM[3..5] = 1): an unchanged selection makes the next round solve
-                // this round's system again, giving the same F, a count that cannot beat the
-                // current one and the same selection, so every remaining round is a no-op.
+                // From LO stage 2 on the selection thresholds of this and every later stage
+                // are equal (kNM[3..5] = 1): an unchanged selection makes the next stage solve
+                // this stage's system again, giving the same F, a count that cannot beat the
+                // current one and the same selection, so every remaining stage is a no-op.
                 if (r >= 2 && !changed) active &= ~(1ull << h);
                 wave_sync();
             }
