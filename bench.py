@@ -12,11 +12,11 @@ equal the exact path's; the plain FP32 GPU path is timed beside it, and the
 CPU oracle (the restated reference algorithm) is timed on a bounded sample of
 the same pairs on the host cores.
 
-Steps are issued as a stream of batches would be: two plans over the same
-inputs on two HIP streams, consecutive steps alternating between them, so
-that one step's matching fills the CUs the previous step's RANSAC frees as
-its last pairs finish. Every step is a complete run over the whole batch;
-the latency of one isolated step is reported beside the throughput, and both
+Steps are issued as a stream of batches would be: four plans over the same
+inputs on four HIP streams, consecutive steps rotating over them, so that
+later steps' matching and RANSAC fill the CUs an earlier step's RANSAC frees
+as its last pairs finish. Every step is a complete run over the whole batch;
+the latency of one isolated step is reported beside the throughput, and all
 plans' results are checked to be identical.
 
 Prints one JSON line (see the field notes in the repository documentation).
@@ -104,9 +104,9 @@ def main():
     ap.add_argument("--workload", default="full", choices=["full", "uneven", "fixed"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=2)
-    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+    ap.add_argument("--streams", type=int, default=4, choices=[1, 2, 3, 4],
                     help="plans (each on its own context and stream) that consecutive steps "
-                         "alternate between; 1 for profiler runs that need isolated kernels")
+                         "rotate over; 1 for profiler runs that need isolated kernels")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true",
@@ -147,10 +147,10 @@ def main():
     log(f"workload {args.workload}: {len(sc.views)} views, {P} pairs, "
         f"{sc.offsets[-1]} features, built in {time.time() - t0:.1f}s")
 
-    # Two contexts (two streams) each hold one plan over the same views and pairs; consecutive
-    # steps alternate between them, so that step k + 1's matching runs on the CUs that step k's
-    # RANSAC frees while its last pairs finish. Each step is a complete run of the path over the
-    # whole batch; a plan's own runs stay ordered on its stream.
+    # Each context (stream) holds one plan over the same views and pairs; consecutive steps
+    # rotate over them, so that later steps' matching and RANSAC run on the CUs that an earlier
+    # step's RANSAC frees while its last pairs finish. Each step is a complete run of the path
+    # over the whole batch; a plan's own runs stay ordered on its stream.
     ctxs = [skylens_hip.Context(0) for _ in range(args.streams)]
     d_desc = torch.from_numpy(sc.desc).cuda()
     d_kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
@@ -204,7 +204,7 @@ def main():
         return
     el, kern_ms, match_ms, ransac_ms, iso_ms = timed(plans, args.steps, args.warmup)
     res = plans[0].results()
-    same_plans = same_results(res, plans[-1].results(), P)
+    same_plans = all(same_results(res, q.results(), P) for q in plans[1:])
     ms_step = el / args.steps * 1e3
     value = P * args.steps / el
     n_found = int(np.sum(res.found))
@@ -213,8 +213,8 @@ def main():
         f"distance kernel {kern_ms:.2f}; isolated step {iso_ms:.2f} ms), {value:.0f} pairs/s, "
         f"{n_matches} matches, {n_found}/{P} verified, plans agree {same_plans}")
 
-    extra = {"steps_issue": ("consecutive steps alternate between two plans on two streams"
-                             if len(ctxs) == 2 else "one plan, one stream"),
+    extra = {"steps_issue": (f"consecutive steps rotate over {len(ctxs)} plans on {len(ctxs)} streams"
+                             if len(ctxs) > 1 else "one plan, one stream"),
              "isolated_step_ms": round(iso_ms, 3), "plans_agree": bool(same_plans)}
     if args.precision != "fp32" and args.fp32_steps > 0:
         plans32 = make_plans("fp32", sc.offsets, d_desc, d_kp, sc.pairs)
