@@ -112,8 +112,8 @@ def test_batch_empty_and_tiny_views(ctx):
         plan.close()
 
 
-def test_two_plans_interleaved(ctx):
-    """Two plans on two contexts (two streams), runs issued alternately with no
+def test_plans_interleaved(ctx):
+    """Four plans on four contexts (four streams), runs issued in rotation with no
     synchronisation in between (the way bench.py issues steps): every run's
     results equal an isolated run's, byte for byte."""
     import torch
@@ -123,11 +123,11 @@ def test_two_plans_interleaved(ctx):
     (ref,) = _run_plan(ctx, sc, prec, steps=1)
     desc = torch.from_numpy(sc.desc).cuda()
     kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
-    ctx2 = skylens_hip.Context(0)
+    extra = [skylens_hip.Context(0) for _ in range(3)]
     plans = [skylens_hip.Plan(c, sc.offsets, desc.data_ptr(), kp.data_ptr(), sc.pairs, 0.8, True,
-                              prec) for c in (ctx, ctx2)]
-    for k in range(6):
-        plans[k % 2].run()
+                              prec) for c in [ctx] + extra]
+    for k in range(10):
+        plans[k % len(plans)].run()
     P = len(sc.pairs)
     for plan in plans:
         r = plan.results()
@@ -139,4 +139,5 @@ def test_two_plans_interleaved(ctx):
                 assert np.array_equal(r.F[p].view(np.uint64), ref.F[p].view(np.uint64)), f"pair {p}"
                 assert np.array_equal(r.inlier_mask(p), ref.inlier_mask(p)), f"pair {p}"
         plan.close()
-    ctx2.close()
+    for c in extra:
+        c.close()
