@@ -99,7 +99,7 @@ def same_results(a, b, P):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="full", choices=["full", "uneven", "fixed"])
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
