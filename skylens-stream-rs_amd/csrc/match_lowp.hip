@@ -255,43 +255,93 @@ __global__ __launch_bounds__(256, 1) void mfma_rank_kernel(const LowpTask* __res
         store(0);
     }
     __syncthreads();
+    // b fragments of column tile cb of LDS buffer buf
+    auto load_b = [&](bf16x8 (&bf)[9], int buf, int cb) {
+        const uint16_t* brow = &Bs[buf][(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) bf[s] = load_frag(brow + 16 * s);
+        bf[8] = tail_frag(Bn[buf][cb * 32 + l32], false, lo_half);
+    };
+    // epilogue of accumulator registers r0..r1-1 of row tile rb: key = -(acc with
+    // the tile id in its low bits) ~ D/2, one v_and_or_b32 (keep_v is a VGPR so
+    // the SGPR tile id fits gfx9's constant bus), then row and column top-2.
+    auto epilogue = [&](const f32x16& acc, int rb, int r0, int r1, uint32_t tile, float& c1,
+                        float& c2) {
+#pragma unroll
+        for (int r = r0; r < r1; ++r) {
+            const float p = __uint_as_float((__float_as_uint(acc[r]) & keep_v) | tile);
+            top2_neg(m1[rb][r], m2[rb][r], p);
+            top2_neg(c1, c2, p);
+        }
+    };
+    // Column partial of tile cb: merge the two half-waves (same column, other
+    // 16*RB rows), one (c1, c2) per wave and column into Cs.
+    auto column_out = [&](float c1, float c2, int buf, int cb) {
+        const float o1 = __shfl_xor(c1, 32);
+        const float o2 = __shfl_xor(c2, 32);
+        const float n2 = fminf(fmaxf(c1, o1), fminf(c2, o2));
+        c1 = fminf(c1, o1);
+        c2 = n2;
+        if (lo_half) Cs[buf][wave][cb * 32 + l32] = make_float2(c1, c2);
+    };
+
+    // Software pipeline over the CB*RB tiles of a chunk (tile ti = (cb, rb) =
+    // (ti / RB, ti % RB)): the nine MFMAs of tile ti + 1 are issued one at a
+    // time, each followed by a slice of tile ti's epilogue, so the VALU work
+    // runs while the matrix core is busy. The scheduling barriers keep that
+    // interleave; the top-2 updates per register happen in the same order as a
+    // plain tile-by-tile sweep, so the results do not depend on it. With two
+    // tiles per chunk (128 x 64) the second accumulator costs the kernel its
+    // third wave per SIMD, which is worth more: that shape runs the plain sweep.
+    constexpr bool kPipe = CB * RB > 2;
     for (uint32_t ch = 0; ch < nchunks; ++ch) {
         const int buf = ch & 1;
         if (ch + 1 < nchunks) fetch((ch + 1) * BN);
         if (ch > 0) merge_cols(ch - 1);
 
+        bf16x8 bfrag[9];
+        load_b(bfrag, buf, 0);
+        f32x16 acc[2];
+        if (kPipe) {
+            acc[0] = f32x16{};
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-            const uint16_t* brow = &Bs[buf][(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
-            bf16x8 bfrag[9];
+            for (int s = 0; s < 9; ++s)
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[0][s], bfrag[s], acc[0], 0,
+                                                                 0, 0);
+        }
+        float c1 = INFINITY, c2 = INFINITY;
 #pragma unroll
-            for (int s = 0; s < 8; ++s) bfrag[s] = load_frag(brow + 16 * s);
-            bfrag[8] = tail_frag(Bn[buf][cb * 32 + l32], false, lo_half);
+        for (int ti = 0; ti < CB * RB; ++ti) {
+            const int cb = ti / RB, rb = ti % RB, cur = ti & 1;
             const uint32_t tile = (ch * CB + cb) & tmask;
-            float c1 = INFINITY, c2 = INFINITY;
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                f32x16 acc = {};
+            if (!kPipe) {
+                if (rb == 0 && ti > 0) load_b(bfrag, buf, cb);
+                acc[cur] = f32x16{};
 #pragma unroll
                 for (int s = 0; s < 9; ++s)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[rb][s], bfrag[s], acc, 0,
-                                                                  0, 0);
+                    acc[cur] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[rb][s], bfrag[s],
+                                                                       acc[cur], 0, 0, 0);
+                epilogue(acc[cur], rb, 0, 16, tile, c1, c2);
+            } else if (ti + 1 < CB * RB) {
+                const int nrb = (ti + 1) % RB;
+                if (nrb == 0) load_b(bfrag, buf, (ti + 1) / RB);
+                acc[cur ^ 1] = f32x16{};
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    // key = -(acc with the tile id in its low bits) ~ D/2: one v_and_or_b32
-                    // (keep_v is a VGPR so the SGPR tile id fits gfx9's constant bus).
-                    const float p = __uint_as_float((__float_as_uint(acc[r]) & keep_v) | tile);
-                    top2_neg(m1[rb][r], m2[rb][r], p);
-                    top2_neg(c1, c2, p);
+                for (int s = 0; s < 9; ++s) {
+                    acc[cur ^ 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        afrag[nrb][s], bfrag[s], acc[cur ^ 1], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    epilogue(acc[cur], rb, (16 * s) / 9, (16 * (s + 1)) / 9, tile, c1, c2);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
+            } else {
+                epilogue(acc[cur], rb, 0, 16, tile, c1, c2);
             }
-            // Merge the two half-waves (same column, other 16*RB rows).
-            const float o1 = __shfl_xor(c1, 32);
-            const float o2 = __shfl_xor(c2, 32);
-            const float n2 = fminf(fmaxf(c1, o1), fminf(c2, o2));
-            c1 = fminf(c1, o1);
-            c2 = n2;
-            if (lo_half) Cs[buf][wave][cb * 32 + l32] = make_float2(c1, c2);
+            if (rb == RB - 1) {
+                column_out(c1, c2, buf, cb);
+                c1 = INFINITY;
+                c2 = INFINITY;
+            }
         }
         if (ch + 1 < nchunks) store(buf ^ 1);
         __syncthreads();
