@@ -170,5 +170,11 @@ def test_every_tile_shape(ctx, prec, tile):
         from workloads import scene_pair_descriptors
         a, b = scene_pair_descriptors(seed=11, na=1537, nb=4097)
         assert ctx.ratio_match(a, b, 0.8, True, prec) == ob.ratio_match(a, b, 0.8, True)
+        # one partly filled chunk, a chunk whose second column tile is all
+        # padding, and a last chunk of one column
+        for seed, na, nb in [(12, 17, 5), (13, 33, 95), (14, 300, 31), (15, 129, 193)]:
+            a, b = scene_pair_descriptors(seed=seed, na=na, nb=nb, pool=48)
+            assert ctx.ratio_match(a, b, 0.8, True, prec) == ob.ratio_match(a, b, 0.8, True)
+            assert ctx.ratio_match(b, a, 0.8, False, prec) == ob.ratio_match(b, a, 0.8, False)
     finally:
         ctx.set_tile(-1)
