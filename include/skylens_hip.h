@@ -14,7 +14,8 @@
  *   candidate_pairs   :18-40   -> skh_candidate_pairs
  *   ratio_match       :167-215 -> skh_ratio_match / skh_ratio_match_dev
  *   ransac_fundamental:941-1104 -> skh_ransac_fundamental
- *   per-pair driver   benches/pipeline.rs:219-305 -> skh_match_pairs_batched
+ *   per-pair driver   benches/pipeline.rs:219-305 -> skh_plan_create / skh_plan_run /
+ *                                                      skh_plan_results
  */
 #ifndef SKYLENS_HIP_H
 #define SKYLENS_HIP_H
@@ -53,7 +54,8 @@ typedef struct skh_ctx skh_ctx;
 int skh_ctx_create(int device, skh_ctx** out);
 int skh_ctx_destroy(skh_ctx* ctx);
 /* MFMA ranking tile (rows per workgroup x columns per chunk). SKH_TILE_AUTO
- * lets the context pick (the autotuned table when one has been loaded). */
+ * lets the context pick: 256x64, or 128x64 for a single pair with at most
+ * 128 rows. */
 #define SKH_TILE_AUTO -1
 #define SKH_TILE_128x64 0
 #define SKH_TILE_128x128 1
@@ -177,7 +179,9 @@ int skh_plan_match_base(skh_plan* plan, uint64_t* match_base);
 /* Copies results to host buffers (any may be NULL): match_len (n_pairs),
  * match_ij (2 * match_base[n_pairs] uint32), inliers (match_base[n_pairs]
  * bytes, aligned with match_ij), F (9 * n_pairs, row-major), found (n_pairs:
- * 1 = Some, 0 = None), sel (n_pairs). Synchronises first. */
+ * 1 = Some, 0 = None), sel (n_pairs). Synchronises first. Returns SKH_EHIP
+ * (after copying) if any pair's RANSAC found its replay inconsistent, an
+ * internal error that the single-pair call reports the same way. */
 int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* match_ij, uint8_t* inliers,
                      double* F, int32_t* found, skh_model_sel* sel);
 /* Device time (ms) of the named stage in the plan's last run, measured with

@@ -124,7 +124,7 @@ int Plan::build() {
     if (lowp && max_view_rows > 131072) lowp = false;  // tile ids would erode the key precision
     tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile256x64;
     const uint32_t bm = lowp ? lowp_block_rows(tile) : 64u;
-    tbits = tile_bits(max_view_rows);
+    tbits = tile_bits(max_view_rows, tile_bn(tile));
 
     // per-pair bases
     std::vector<PairTask> pt(P);
@@ -443,6 +443,15 @@ extern "C" int skh_plan_results(skh_plan* plan, uint32_t* match_len, uint32_t* m
         (rc = get(F, plan->d_F, 72ull * P, "F")) || (rc = get(found, plan->d_found, 4ull * P, "found")) ||
         (rc = get(sel, plan->d_sel, sizeof(skh_model_sel) * P, "sel")))
         return rc;
+    // A pair whose replay found its batch inconsistent reports found = -1: the
+    // whole result is then an error, whether or not the caller asked for found.
+    if (plan->d_found.ptr && P) {
+        std::vector<int32_t> f(P);
+        if ((rc = get(f.data(), plan->d_found, 4ull * P, "found"))) return rc;
+        for (uint32_t p = 0; p < P; ++p)
+            if (f[p] < 0)
+                return ctx->fail(SKH_EHIP, "ransac: internal replay inconsistency in pair %u", p);
+    }
     return SKH_OK;
 }
 

@@ -222,9 +222,9 @@ __global__ void exact_decide_kernel(const PairTask* __restrict__ tasks, const ui
                                     const float* row_d1, const float* row_d2,
                                     const unsigned long long* col_key, float ratio, int mutual,
                                     uint32_t* row_match) {
-    const PairTask t = tasks[blockIdx.y];
+    const PairTask t = tasks[blockIdx.x];  // pairs on x: gridDim.y is capped at 65535
     const float r2 = ratio * ratio;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < t.na; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.y * blockDim.x + threadIdx.x; i < t.na; i += gridDim.y * blockDim.x) {
         const uint32_t g = t.row_base + i;
         const uint32_t j = row_j1[g];
         const float d1 = row_d1[g], d2 = row_d2[g];
@@ -296,7 +296,7 @@ void launch_exact_decide(const PairTask* d_tasks, uint32_t n_pairs, const uint32
                          const unsigned long long* col_key, float ratio, int mutual,
                          uint32_t* row_match, hipStream_t s) {
     if (n_pairs == 0) return;
-    hipLaunchKernelGGL(exact_decide_kernel, dim3(8, n_pairs), dim3(256), 0, s, d_tasks, row_j1,
+    hipLaunchKernelGGL(exact_decide_kernel, dim3(n_pairs, 8), dim3(256), 0, s, d_tasks, row_j1,
                        row_d1, row_d2, col_key, ratio, mutual, row_match);
 }
 

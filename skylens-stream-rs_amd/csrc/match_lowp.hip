@@ -57,6 +57,10 @@ __device__ __forceinline__ void atomic_max_pos(float* p, float v) {
     atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
 }
 
+// Largest |a|^2 ranked by MFMA: real keys stay below 2 kMaxSq = 2e28, far
+// under the padding key and inside the f32 accumulator's range.
+constexpr float kMaxSq = 1e28f;
+
 __device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* __restrict__ q,
                              uint32_t* __restrict__ nrm, ViewStats* __restrict__ st);
 
@@ -67,12 +71,13 @@ __global__ void quantize_kernel(const float* __restrict__ x, uint32_t n, uint16_
     if (i < n) quantize_row(x, i, q, nrm, st);
 }
 
-// All views of a scene: blockIdx.y = view, rows [off[v], off[v+1]).
+// All views of a scene: blockIdx.x = view, rows [off[v], off[v+1]) in blocks of
+// blockDim.x along y.
 __global__ void quantize_views_kernel(const float* __restrict__ x, const uint64_t* __restrict__ off,
                                       uint16_t* __restrict__ q, uint32_t* __restrict__ nrm,
                                       ViewStats* __restrict__ st) {
-    const uint32_t v = blockIdx.y;
-    const uint64_t r = off[v] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = blockIdx.x;
+    const uint64_t r = off[v] + (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
     if (r < off[v + 1]) quantize_row(x, (uint32_t)r, q, nrm, st + v);
 }
 
@@ -99,6 +104,11 @@ __device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* 
         qo[c / 2] = h[0] | (h[1] << 16);
         qo[c / 2 + 1] = h[2] | (h[3] << 16);
     }
+    // Padding rows and columns carry the norm kPadNorm (~1e30), so their keys
+    // (~5e29) must stay above every real key, which is at most
+    // (|a| + |b|)^2 / 2 <= 2 max |a|^2. Rows with |a|^2 above kMaxSq are
+    // scored exactly instead.
+    bad |= !(s <= kMaxSq);
     uint32_t hi = bf16_rne(s);
     float lo = s - bf16_to_f(hi);
     uint32_t lo16 = bf16_rne(lo);
@@ -388,10 +398,10 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
 
 // Per column of every pair: merge the workgroup partials.
 __global__ void colmerge_kernel(const LowpTask* __restrict__ tasks, LowpOut out, uint32_t bm) {
-    const LowpTask t = tasks[blockIdx.y];
+    const LowpTask t = tasks[blockIdx.x];  // pairs on x: gridDim.y is capped at 65535
     const uint32_t nparts = (t.na + bm - 1) / bm;
-    for (uint32_t col = blockIdx.x * blockDim.x + threadIdx.x; col < t.nb;
-         col += gridDim.x * blockDim.x) {
+    for (uint32_t col = blockIdx.y * blockDim.x + threadIdx.x; col < t.nb;
+         col += gridDim.y * blockDim.x) {
         float a1 = INFINITY, a2 = INFINITY;
         for (uint32_t p = 0; p < nparts; ++p) {
             const float2 v = out.col_part[(size_t)t.part_base + (size_t)p * t.nb + col];
@@ -479,7 +489,7 @@ __global__ __launch_bounds__(256) void decide_kernel(const LowpTask* __restrict_
                                                      uint32_t tbits) {
     __shared__ uint32_t wave_tot[4];
     __shared__ uint32_t run_base;
-    const LowpTask t = tasks[blockIdx.y];
+    const LowpTask t = tasks[blockIdx.x];  // one workgroup per pair
     const bool bad = (t.sa->bad | t.sb->bad) != 0;
     const double eps = pair_eps(t, tbits);
     const float r2 = ratio * ratio;
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(256) void decide_kernel(const LowpTask* __restrict_
         uint32_t verdict, j1;
         bool queue, need_row;
         decide_row(t, out, bad, eps, r2, mutual, i, verdict, queue, need_row, j1);
-        if (queue) out.queue[slot++] = make_uint4(blockIdx.y, i, need_row ? kNone : j1, 0u);
+        if (queue) out.queue[slot++] = make_uint4(blockIdx.x, i, need_row ? kNone : j1, 0u);
     }
 }
 
@@ -795,7 +805,7 @@ void launch_quantize_views(const float* x, const uint64_t* d_off, uint32_t n_vie
                            uint32_t max_rows, uint16_t* q, uint32_t* nrm, ViewStats* st,
                            hipStream_t s) {
     if (!n_views || !max_rows) return;
-    hipLaunchKernelGGL(quantize_views_kernel, dim3((max_rows + 255) / 256, n_views), dim3(256), 0,
+    hipLaunchKernelGGL(quantize_views_kernel, dim3(n_views, (max_rows + 255) / 256), dim3(256), 0,
                        s, x, d_off, q, nrm, st);
 }
 
@@ -827,13 +837,13 @@ void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, u
 void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
                      hipStream_t s) {
     if (!n_pairs) return;
-    hipLaunchKernelGGL(colmerge_kernel, dim3(8, n_pairs), dim3(256), 0, s, tasks, out, bm);
+    hipLaunchKernelGGL(colmerge_kernel, dim3(n_pairs, 8), dim3(256), 0, s, tasks, out, bm);
 }
 
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
                    int mutual, uint32_t tbits, hipStream_t s) {
     if (!n_pairs) return;
-    hipLaunchKernelGGL(decide_kernel, dim3(1, n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
+    hipLaunchKernelGGL(decide_kernel, dim3(n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
                        tbits);
 }
 

@@ -28,7 +28,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     const int tile = tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile256x64);
     const uint32_t bm = lowp_block_rows(tile);
     const uint32_t nblk = (uint32_t)((na + bm - 1) / bm);
-    const uint32_t tbits = tile_bits(nb);
+    const uint32_t tbits = tile_bits(nb, tile_bn(tile));
     if ((rc = ensure(q_a, 2 * kDescLen * na, "q_a")) || (rc = ensure(q_b, 2 * kDescLen * nb, "q_b")) ||
         (rc = ensure(nrm_a, 4 * na, "nrm_a")) || (rc = ensure(nrm_b, 4 * nb, "nrm_b")) ||
         (rc = ensure(stats, 2 * sizeof(ViewStats), "stats")) ||

@@ -65,9 +65,11 @@ void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, 
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
                     uint32_t tbits, uint32_t grid, hipStream_t s);
 
-// Bits needed for a column-tile id of a pair with nb columns (tiles of 32).
-inline uint32_t tile_bits(uint64_t nb_max) {
-    uint64_t tiles = (nb_max + 31) / 32;
+// Bits needed for a column-tile id of a pair with up to nb_max columns swept in
+// chunks of bn columns (tiles of 32): the last chunk's padding tiles count too,
+// so no tile id wraps onto a real column.
+inline uint32_t tile_bits(uint64_t nb_max, uint32_t bn) {
+    uint64_t tiles = (nb_max + bn - 1) / bn * (bn / 32);
     uint32_t b = 1;
     while ((1ull << b) < tiles) ++b;
     return b;

@@ -1,4 +1,5 @@
-"""Seeded synthetic workloads for the matching path (numpy only).
+"""Seeded synthetic workloads for the matching path (numpy; scene pair lists
+come from the library's candidate_pairs).
 
 Descriptors are unit 128-d f32 rows; keypoints are f32 pixel positions, one
 per feature, in the bench's raw pixel convention (benches/pipeline.rs:276-283
@@ -6,9 +7,10 @@ gathers `kp.x as f64` without +0.5).
 
 * ``single_pair()`` -- the single CPU pair: A 800 unit Gaussian rows; B 480
   noisy copies (sigma 0.05, renormalised) of distinct random A rows plus 320
-  fresh rows, permuted; ratio 0.8, mutual. Its RANSAC input is 600 projected
-  points (two pinholes fx=fy=300, 320x180, 5 m baseline, 15 deg yaw, sigma
-  0.5 px) plus 400 uniform outliers (40 % of the 1000 correspondences).
+  fresh rows, permuted; ratio 0.8, mutual. Keypoints are tied to the rows:
+  copies see their source's 3D point (two pinholes fx=fy=300, 320x180, 5 m
+  baseline, 15 deg yaw, sigma 0.5 px), 40 % of them are uniform outliers, so
+  the matches handed to ransac_fundamental carry the geometry.
 * ``scene(...)`` -- a multi-view scene: 3 cameras x P positions (view index
   3p + c, matching synth.rs:284-305), per-view feature counts drawn by the
   caller's rule, 60 % noisy copies (sigma 0.05) of a shared pool of 20k unit
@@ -99,22 +101,36 @@ class Camera:
 
 @dataclass
 class SinglePair:
-    a: np.ndarray
-    b: np.ndarray
+    a: np.ndarray         # (800, 128) f32
+    b: np.ndarray         # (800, 128) f32
     b_source: np.ndarray  # A row each B row copies, -1 for fresh
-    x1: np.ndarray        # (1000, 2) f64 RANSAC input
-    x2: np.ndarray
-    truth: np.ndarray     # True = projected inlier
+    kp_a: np.ndarray      # (800, 2) f32 keypoints of A's features (image 1)
+    kp_b: np.ndarray      # (800, 2) f32 keypoints of B's features (image 2)
+    b_inlier: np.ndarray  # B row's keypoint is its source's 3D point seen in image 2
+
+    def handoff(self, matches, px_offset: float = 0.0):
+        """x1, x2 (n, 2) f64 of a match list (benches/pipeline.rs:276-283)."""
+        m = np.asarray(matches, np.int64).reshape(-1, 2)
+        x1 = self.kp_a[m[:, 0]].astype(np.float64) + px_offset
+        x2 = self.kp_b[m[:, 1]].astype(np.float64) + px_offset
+        return x1, x2
 
 
 def single_pair(seed: int = 0) -> SinglePair:
+    """A 800 unit Gaussian rows; B 480 noisy copies (sigma 0.05) of distinct A rows
+    plus 320 fresh rows, permuted. Keypoints are tied to the descriptors: 600 3D
+    points (uniform in a 100 x 100 x 20 m box, visible in both 320x180 pinholes,
+    fx = fy = 300, 5 m baseline, 15 deg yaw) belong to the 480 copied A rows and
+    120 more; a copy's keypoint in image 2 is its point's projection (sigma 0.5
+    px) unless it is one of the 40 % outliers, drawn uniformly over the image,
+    like the fresh rows and the A rows without a point. So the correspondences
+    that ratio_match hands to ransac_fundamental are ~60 % inliers."""
     rng = np.random.default_rng(seed)
-    a = unit_rows(rng, 800)
-    src = rng.choice(800, 480, replace=False)
+    na = 800
+    a = unit_rows(rng, na)
+    src = rng.choice(na, 480, replace=False)
     b = np.concatenate([noisy_copies(rng, a[src], 0.05), unit_rows(rng, 320)])
     b_source = np.concatenate([src, -np.ones(320, np.int64)])
-    perm = rng.permutation(800)
-    b, b_source = b[perm], b_source[perm]
 
     w, h, f = 320, 180, 300.0
     c1 = Camera(np.eye(3), np.zeros(3), f, 160.0, 90.0, w, h)
@@ -128,15 +144,27 @@ def single_pair(seed: int = 0) -> SinglePair:
         ok = v1 & v2
         pts.append(np.concatenate([p1[ok], p2[ok]], 1))
     P = np.concatenate(pts)[:600]
-    x1 = P[:, :2] + 0.5 * rng.standard_normal((600, 2))
-    x2 = P[:, 2:] + 0.5 * rng.standard_normal((600, 2))
-    o1 = np.stack([rng.uniform(0, w, 400), rng.uniform(0, h, 400)], 1)
-    o2 = np.stack([rng.uniform(0, w, 400), rng.uniform(0, h, 400)], 1)
-    x1 = np.concatenate([x1, o1])
-    x2 = np.concatenate([x2, o2])
-    truth = np.concatenate([np.ones(600, bool), np.zeros(400, bool)])
-    perm = rng.permutation(1000)
-    return SinglePair(a, b, b_source, x1[perm].copy(), x2[perm].copy(), truth[perm].copy())
+    others = np.setdiff1d(np.arange(na), src)
+    owners = np.concatenate([src, rng.choice(others, 120, replace=False)])
+    point_of = -np.ones(na, np.int64)
+    point_of[owners] = np.arange(600)
+
+    def uniform(n):
+        return np.stack([rng.uniform(0, w, n), rng.uniform(0, h, n)], 1)
+
+    kp_a = uniform(na)
+    kp_a[owners] = P[:, :2] + 0.5 * rng.standard_normal((600, 2))
+    kp_b = uniform(800)
+    kp_b[:480] = P[point_of[src], 2:] + 0.5 * rng.standard_normal((480, 2))
+    inl = np.zeros(800, bool)
+    inl[:480] = rng.random(480) >= 0.4
+    kp_b[:480][~inl[:480]] = uniform(int((~inl[:480]).sum()))
+    perm = rng.permutation(800)
+
+    def img(k):
+        return np.clip(k, 0.0, [w - 1e-3, h - 1e-3]).astype(np.float32)
+
+    return SinglePair(a, b[perm], b_source[perm], img(kp_a), img(kp_b[perm]), inl[perm])
 
 
 # --------------------------------------------------------------------------
@@ -163,8 +191,18 @@ class Scene:
         return self.kp[self.offsets[v]:self.offsets[v + 1]]
 
 
+def candidate_pair_array(views, temporal=5, cross=4, pow2_max=16) -> np.ndarray:
+    """The scene's pair list from the library's candidate_pairs (matching.rs:18-40,
+    host code behind the C-ABI), as an (n, 2) int64 array."""
+    import skylens_hip
+
+    pairs = skylens_hip.candidate_pairs(views, temporal, cross, pow2_max)
+    return np.asarray(pairs, np.int64).reshape(-1, 2)
+
+
 def _candidate_pairs(views, temporal=5, cross=4, pow2_max=16) -> np.ndarray:
-    """matching.rs:18-40, vectorised (i ascending, then j)."""
+    """matching.rs:18-40, vectorised (i ascending, then j): an independent
+    restatement the tests compare the library's list with."""
     v = np.asarray(views, np.int64)
     n = len(v)
     out = []
@@ -223,7 +261,7 @@ def scene(seed: int, positions: int, counts: np.ndarray, outlier_lo: float, outl
         kp[s] = np.clip(key[perm], 0.0, [W - 1e-3, H - 1e-3]).astype(np.float32)
         pool_id[s] = pid[perm]
         outl[s] = rep[perm]
-    pairs = _candidate_pairs(views)
+    pairs = candidate_pair_array(views)
     if max_pairs is not None:
         pairs = pairs[:max_pairs]
     return Scene(views, np.asarray(counts, np.int64), offsets, desc, kp, pool_id, outl, pairs,

@@ -67,3 +67,47 @@ def test_null_context_is_rejected():
     n = ctypes.c_size_t(0)
     assert lib.skh_ratio_match(None, None, 0, None, 0, ctypes.c_float(0.8), 1, 0, None, 0,
                                ctypes.byref(n)) == skylens_hip.SKH_EINVAL
+
+
+def _c_layout():
+    """sizeof / offsetof of the header's structs, as a C compiler lays them out."""
+    import subprocess
+    import tempfile
+
+    fields = {
+        "skh_ransac_cfg": ["threshold_px", "min_inlier_ratio", "max_iters", "confidence", "seed",
+                           "min_inliers", "max_chance_prob"],
+        "skh_model_sel": ["has_selection", "model", "gric_f", "gric_h", "parallax",
+                          "parallax_needed", "sigma_px"],
+        "skh_batch_desc": ["n_views", "view_offset", "d_desc", "d_kp", "n_pairs", "pairs", "ratio",
+                           "mutual", "precision", "run_ransac", "px_offset", "ransac"],
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "skylens_hip.h"',
+             "int main(void) {"]
+    for st, fs in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        with open(src, "w") as f:
+            f.write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), src, "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return {k: int(v) for k, v in (l.split() for l in out.splitlines())}, fields
+
+
+def test_struct_layout_matches_c_header():
+    """The ctypes mirrors (and so the Rust #[repr(C)] binding written from the same
+    header) lay out every field where the C compiler does."""
+    got, fields = _c_layout()
+    mirrors = {"skh_ransac_cfg": skylens_hip.RansacConfig,
+               "skh_model_sel": skylens_hip.ModelSelectionC,
+               "skh_batch_desc": skylens_hip.BatchDesc}
+    for st, cls in mirrors.items():
+        assert ctypes.sizeof(cls) == got[st], st
+        for f in fields[st]:
+            assert getattr(cls, f).offset == got[f"{st}.{f}"], f"{st}.{f}"

@@ -123,12 +123,17 @@ def test_planar_and_buildings(ctx):
 
 
 def test_single_pair_config(ctx):
+    """The single pair's correspondences (the oracle's matches handed off to pixel
+    coordinates): same fit as the oracle, and it keeps the true correspondences."""
     from workloads import single_pair
 
     sp = single_pair(0)
-    got = _check(ctx, sp.x1, sp.x2, "single pair 1000 @ 40% outliers")
-    tp = np.sum(got.inliers & sp.truth)
-    assert tp / sp.truth.sum() >= 0.95
+    m = ob.ratio_match(sp.a, sp.b, 0.8, True)
+    x1, x2 = sp.handoff(m)
+    got = _check(ctx, x1, x2, f"single pair, {len(m)} matches")
+    mm = np.asarray(m)
+    truth = (sp.b_source[mm[:, 1]] == mm[:, 0]) & sp.b_inlier[mm[:, 1]]
+    assert np.sum(got.inliers & truth) / truth.sum() >= 0.95
 
 
 def test_config_variants(ctx):

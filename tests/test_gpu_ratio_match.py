@@ -178,3 +178,27 @@ def test_every_tile_shape(ctx, prec, tile):
             assert ctx.ratio_match(b, a, 0.8, False, prec) == ob.ratio_match(b, a, 0.8, False)
     finally:
         ctx.set_tile(-1)
+
+
+@pytest.mark.parametrize("prec", LOWP)
+@pytest.mark.parametrize("tile", ["128x64", "128x128", "256x64", "256x128"])
+def test_large_finite_descriptors(ctx, prec, tile):
+    """Large but finite components: |a|^2 far above 1 (near and past the range the
+    MFMA ranking accepts), nb = 63 so every chunk has padding columns, na = 300 so
+    the last row block has padding rows. The padding keys must never enter a
+    top-2, in mutual and non-mutual mode."""
+    _say(f"large finite {prec} {tile}")
+    ctx.set_tile(skylens_hip.TILES[tile])
+    try:
+        for scale in (2e14, 5e12, 1e6):
+            rng = np.random.default_rng(int(np.log10(scale)))
+            a = (rng.standard_normal((300, 128)) * scale).astype(np.float32)
+            b = np.concatenate([a[:40] + np.float32(0.05 * scale) *
+                                rng.standard_normal((40, 128)).astype(np.float32),
+                                (rng.standard_normal((23, 128)) * scale).astype(np.float32)])
+            for mutual in (True, False):
+                for x, y in ((a, b), (b, a)):
+                    got = ctx.ratio_match(x, y, 0.8, mutual, prec)
+                    assert got == ob.ratio_match(x, y, 0.8, mutual), (scale, mutual, x.shape)
+    finally:
+        ctx.set_tile(-1)
