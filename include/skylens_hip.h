@@ -193,6 +193,18 @@ double skh_plan_stage_ms(skh_plan* plan, int stage);
  * on the context stream, on the inputs of the plan's last run (call after a
  * run). Synchronises. Returns -1 on error. */
 double skh_plan_time_distance(skh_plan* plan, int reps);
+/* Approximate ranking state of the last run of a BF16 or FP8 plan, for error
+ * checks and diagnostics (SKH_EINVAL for an FP32 plan). The rows of pair p
+ * occupy [R_p, R_p + na_p), R_p = the sum of na over the pairs before p (the
+ * columns likewise with nb). row_k1 / row_k2: the row's two smallest keys
+ * (key ~ d^2 / 2 with the column-tile id in its lowest mantissa bits);
+ * row_j1: the column of the smallest; col_top2: (c1, c2) per column, the
+ * column's two smallest keys; eps[p]: the pair's error bound, |2 key - d^2| <=
+ * eps for every element of the pair, d^2 the exact squared distance (infinity
+ * when the pair is scored exactly); *queued: rows the run sent to the exact
+ * FP32 re-score. Any pointer may be NULL. Synchronises. */
+int skh_plan_lowp_state(skh_plan* plan, float* row_k1, uint32_t* row_j1, float* row_k2,
+                        float* col_top2, double* eps, uint64_t* queued);
 /* 1 = record stage events in each run (costs nothing measurable), 0 = off. */
 int skh_plan_set_timing(skh_plan* plan, int on);
 /* RANSAC diagnostics of the last run (any pointer may be NULL): iters[p] =

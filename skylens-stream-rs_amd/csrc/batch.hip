@@ -55,11 +55,12 @@ struct Plan {
     int tile = kTile256x64;
     uint32_t tbits = 1;
     bool lowp = false;
+    bool fp8 = false;
     uint32_t n_blocks = 0;
     uint64_t total_rows = 0, total_cols = 0, total_parts = 0, cap_total = 0;
     uint32_t max_view_rows = 0;
 
-    DevBuf d_view_off, d_tasks_exact, d_tasks_lowp, d_blocks, d_stats, d_q, d_nrm;
+    DevBuf d_view_off, d_tasks_exact, d_tasks_lowp, d_blocks, d_stats, d_q, d_nrm, d_q8, d_meta;
     DevBuf d_row_k1, d_row_j1, d_row_k2, d_row_d1, d_row_d2, d_col_key, d_col_part, d_col_top2;
     DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
         d_kp_b;
@@ -80,7 +81,8 @@ struct Plan {
                 &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len,
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
-                &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue};
+                &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue,
+                &d_q8,       &d_meta};
     }
 
     ~Plan() {
@@ -119,9 +121,10 @@ struct Plan {
 
 int Plan::build() {
     const uint32_t P = desc.n_pairs, V = desc.n_views;
-    lowp = desc.precision == SKH_PREC_BF16;
+    fp8 = desc.precision == SKH_PREC_FP8;
+    lowp = desc.precision == SKH_PREC_BF16 || fp8;
     for (uint32_t v = 0; v < V; ++v) max_view_rows = std::max(max_view_rows, rows(v));
-    if (lowp && max_view_rows > 131072) lowp = false;  // tile ids would erode the key precision
+    if (lowp && max_view_rows > 131072) lowp = fp8 = false;  // tile ids would erode the key precision
     tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile256x64;
     const uint32_t bm = lowp ? lowp_block_rows(tile) : 64u;
     tbits = tile_bits(max_view_rows, tile_bn(tile));
@@ -165,17 +168,31 @@ int Plan::build() {
     // quantised views (lowp)
     if ((rc = upload(d_view_off, view_off.data(), 8ull * (V + 1), "view_off"))) return rc;
     const uint64_t total_feat = view_off[V];
-    if (lowp) {
-        if ((rc = alloc(d_q, 2ull * kDescLen * total_feat, "q")) ||
-            (rc = alloc(d_nrm, 4ull * total_feat, "nrm")) ||
-            (rc = alloc(d_stats, sizeof(ViewStats) * V, "stats")))
+    if (fp8) {
+        if ((rc = alloc(d_q8, kDescLen * total_feat, "q8")) ||
+            (rc = alloc(d_meta, sizeof(Fp8Meta) * total_feat, "meta")))
             return rc;
+    } else if (lowp) {
+        if ((rc = alloc(d_q, 2ull * kDescLen * total_feat, "q")) ||
+            (rc = alloc(d_nrm, 4ull * total_feat, "nrm")))
+            return rc;
+    }
+    if (lowp) {
+        if ((rc = alloc(d_stats, sizeof(ViewStats) * V, "stats"))) return rc;
         for (uint32_t p = 0; p < P; ++p) {
             const uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
-            lt[p].qa = d_q.as<uint16_t>() + view_off[i] * kDescLen;
-            lt[p].qb = d_q.as<uint16_t>() + view_off[j] * kDescLen;
-            lt[p].nrm_a = d_nrm.as<uint32_t>() + view_off[i];
-            lt[p].nrm_b = d_nrm.as<uint32_t>() + view_off[j];
+            if (fp8) {
+                lt[p].q8a = d_q8.as<uint8_t>() + view_off[i] * kDescLen;
+                lt[p].q8b = d_q8.as<uint8_t>() + view_off[j] * kDescLen;
+                lt[p].meta_a = d_meta.as<Fp8Meta>() + view_off[i];
+                lt[p].meta_b = d_meta.as<Fp8Meta>() + view_off[j];
+                lt[p].fp8 = 1;
+            } else {
+                lt[p].qa = d_q.as<uint16_t>() + view_off[i] * kDescLen;
+                lt[p].qb = d_q.as<uint16_t>() + view_off[j] * kDescLen;
+                lt[p].nrm_a = d_nrm.as<uint32_t>() + view_off[i];
+                lt[p].nrm_b = d_nrm.as<uint32_t>() + view_off[j];
+            }
             lt[p].sa = d_stats.as<ViewStats>() + i;
             lt[p].sb = d_stats.as<ViewStats>() + j;
         }
@@ -297,8 +314,14 @@ void Plan::enqueue_match(hipStream_t s) {
         o.queue_len = d_queue_len.as<uint32_t>();
         launch_stats_reset(d_stats.as<ViewStats>(), desc.n_views, s);
         launch_fill_u32(o.queue_len, 1, 0u, s);
-        launch_quantize_views(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views, max_view_rows,
-                              d_q.as<uint16_t>(), d_nrm.as<uint32_t>(), d_stats.as<ViewStats>(), s);
+        if (fp8)
+            launch_quantize_views_fp8(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
+                                      max_view_rows, d_q8.as<uint8_t>(), d_meta.as<Fp8Meta>(),
+                                      d_stats.as<ViewStats>(), s);
+        else
+            launch_quantize_views(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
+                                  max_view_rows, d_q.as<uint16_t>(), d_nrm.as<uint32_t>(),
+                                  d_stats.as<ViewStats>(), s);
         const LowpTask* lt = d_tasks_lowp.as<LowpTask>();
         launch_distance(s);
         launch_colmerge(lt, P, o, lowp_block_rows(tile), s);
@@ -329,8 +352,8 @@ void Plan::launch_distance(hipStream_t s) {
         o.row_j1 = d_row_j1.as<uint32_t>();
         o.row_k2 = d_row_k2.as<float>();
         o.col_part = d_col_part.as<float2>();
-        launch_mfma_rank(tile, d_tasks_lowp.as<LowpTask>(), d_blocks.as<RowBlock>(), n_blocks, o,
-                         tbits, s);
+        launch_mfma_rank(tile, fp8, d_tasks_lowp.as<LowpTask>(), d_blocks.as<RowBlock>(), n_blocks,
+                         o, tbits, s);
     } else {
         launch_exact_state(d_tasks_exact.as<PairTask>(), d_blocks.as<RowBlock>(), n_blocks,
                            d_row_j1.as<uint32_t>(), d_row_d1.as<float>(), d_row_d2.as<float>(),
@@ -363,8 +386,6 @@ extern "C" int skh_plan_create(skh_ctx* ctx, const skh_batch_desc* desc, skh_pla
     if (!desc->view_offset || (!desc->pairs && desc->n_pairs) || !desc->d_desc ||
         (desc->run_ransac && !desc->d_kp) || !skh_precision_supported(desc->precision))
         return ctx->fail(SKH_EINVAL, "skh_plan_create: invalid argument");
-    if (desc->precision == SKH_PREC_FP8)
-        return ctx->fail(SKH_EINVAL, "skh_plan_create: FP8 batches are not built");
     for (uint32_t v = 0; v < desc->n_views; ++v)
         if (desc->view_offset[v + 1] < desc->view_offset[v])
             return ctx->fail(SKH_EINVAL, "skh_plan_create: view offsets must not decrease");
@@ -475,6 +496,44 @@ extern "C" int skh_plan_ransac_profile(skh_plan* plan, uint64_t* iters, uint64_t
                            "clock rate")))
             return rc;
         *ticks_per_ms = (double)khz;
+    }
+    return SKH_OK;
+}
+
+extern "C" int skh_plan_lowp_state(skh_plan* plan, float* row_k1, uint32_t* row_j1,
+                                   float* row_k2, float* col_top2, double* eps,
+                                   uint64_t* queued) {
+    if (!plan) return SKH_EINVAL;
+    Ctx* ctx = plan->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!plan->lowp) return ctx->fail(SKH_EINVAL, "skh_plan_lowp_state: not a BF16/FP8 plan");
+    int rc;
+    hipStream_t s = ctx->stream;
+    if ((rc = ctx->hip(hipStreamSynchronize(s), "sync"))) return rc;
+    const uint32_t P = plan->desc.n_pairs;
+    auto get = [&](void* dst, const void* src, size_t bytes, const char* what) -> int {
+        if (!dst || !bytes) return SKH_OK;
+        return ctx->hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), what);
+    };
+    if ((rc = get(row_k1, plan->d_row_k1.ptr, 4ull * plan->total_rows, "row_k1")) ||
+        (rc = get(row_j1, plan->d_row_j1.ptr, 4ull * plan->total_rows, "row_j1")) ||
+        (rc = get(row_k2, plan->d_row_k2.ptr, 4ull * plan->total_rows, "row_k2")) ||
+        (rc = get(col_top2, plan->d_col_top2.ptr, 8ull * plan->total_cols, "col_top2")))
+        return rc;
+    if (eps && P) {
+        double* d_eps = nullptr;
+        if ((rc = ctx->hip(hipMalloc(&d_eps, 8ull * P), "eps"))) return rc;
+        launch_pair_eps(plan->d_tasks_lowp.as<LowpTask>(), P, plan->tbits, d_eps, s);
+        rc = ctx->hip(hipGetLastError(), "eps kernel");
+        if (!rc) rc = ctx->hip(hipMemcpyAsync(eps, d_eps, 8ull * P, hipMemcpyDeviceToHost, s), "eps");
+        if (!rc) rc = ctx->hip(hipStreamSynchronize(s), "sync");
+        (void)hipFree(d_eps);
+        if (rc) return rc;
+    }
+    if (queued) {
+        uint32_t q = 0;
+        if ((rc = get(&q, plan->d_queue_len.ptr, 4, "queue_len"))) return rc;
+        *queued = q;
     }
     return SKH_OK;
 }

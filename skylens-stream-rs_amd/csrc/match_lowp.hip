@@ -1,5 +1,6 @@
-// Low-precision ratio_match: BF16 MFMA ranking with a rigorous error bound,
-// then the reference's FP32 formula for every decision the bound leaves open.
+// Low-precision ratio_match: BF16 or FP8 (e4m3) MFMA ranking with a rigorous
+// error bound, then the reference's FP32 formula for every decision the bound
+// leaves open.
 //
 // Reference semantics: crates/core/src/matching.rs:157-215 (ratio_match),
 // reproduced exactly; this file only changes how candidates are found.
@@ -17,6 +18,14 @@
 //    row top-2 (v_min, v_med3) and column top-2 (v_min, v_med3). The
 //    distance matrix never reaches HBM: rows keep a running top-2 per lane,
 //    columns leave one (c1, c2) partial per workgroup.
+//    FP8 form: per row, q = e4m3(a / 2^e) with 2^e the smallest power of two
+//    >= max|a| / 448 (the row's scale; e4m3 precision is the same at every
+//    binade, so rounding the scale up to a power of two costs nothing and lets
+//    the matrix core apply it: v_mfma_scale_f32_32x32x64_f8f6f4 with E8M0
+//    scales). Two K = 64 steps take the 128 components; a third folds the
+//    norms in, |a|^2 as three e4m3 terms (hi + mid + lo, 2^-11 relative) times
+//    their own E8M0 scale against -1/2 terms, so the accumulator is again
+//    -D/2 and the epilogue is the BF16 one.
 // 3. colmerge: partials -> per-column approximate top-2 values.
 // 4. decide: with eps bounding |2*key - D_ref| for every element of the pair,
 //    each row is a certain accept, a certain reject, or queued.
@@ -31,7 +40,38 @@ namespace skh {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// e4m3 (OCP "fn" form: bias 7, largest 448, subnormals in steps of 2^-9) of
+// x with |x| <= 448, rounded to nearest even.
+__device__ __forceinline__ uint32_t e4m3_rne(float x) {
+    const uint32_t sign = (__float_as_uint(x) >> 24) & 0x80u;
+    const float a = fabsf(x);
+    uint32_t code;
+    if (a < 0x1p-6f) {
+        code = (uint32_t)__builtin_rintf(a * 512.0f);  // 0..8 (8 = 2^-6, code 0x08)
+    } else {
+        uint32_t u = __float_as_uint(a);
+        u += 0x7ffffu + ((u >> 20) & 1u);  // round the mantissa to 3 bits
+        code = (((u >> 23) - 120u) << 3) | ((u >> 20) & 7u);
+    }
+    return sign | code;
+}
+__device__ __forceinline__ float e4m3_to_f(uint32_t c) {
+    const uint32_t e = (c >> 3) & 15u, m = c & 7u;
+    const float v = e ? __uint_as_float(((e + 120u) << 23) | (m << 20)) : (float)m * 0x1p-9f;
+    return (c & 0x80u) ? -v : v;
+}
+// Smallest e with m <= 448 * 2^e, for finite m > 0 (448 = 1.75 * 2^8, so the
+// first guess is off by at most one step; only f32-subnormal m, whose bound
+// underflows, take a few more). The loops are bounded all the same.
+__device__ __forceinline__ int fp8_scale_exp(float m) {
+    int e = ilogbf(m) - 8;
+    for (int k = 0; k < 64 && ldexpf(448.0f, e) < m; ++k) ++e;
+    for (int k = 0; k < 64 && ldexpf(448.0f, e - 1) >= m; ++k) --e;
+    return e;
+}
 
 __device__ __forceinline__ uint32_t bf16_rne(float x) {
     uint32_t u = __float_as_uint(x);
@@ -129,6 +169,89 @@ __device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* 
     atomic_max_pos(&st->max_sq, up(s * 1.0001f));
 }
 
+// FP8 form of quantize_row: e4m3 bytes of a / 2^e, the row's E8M0 scale code
+// and the three-term fold of |a|^2, plus the same bound statistics in real
+// units (q(a) = e4m3 value * 2^e).
+__device__ void quantize_row_fp8(const float* __restrict__ x, uint32_t i, uint8_t* __restrict__ q,
+                                 Fp8Meta* __restrict__ meta, ViewStats* __restrict__ st) {
+    const float* r = x + (size_t)i * kDescLen;
+    float s = 0.f, mx = 0.f;
+    bool bad = false;
+    for (int c = 0; c < kDescLen; c += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(r + c);
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            bad |= !(fabsf(vv[e]) <= 1e15f);
+            s = s + vv[e] * vv[e];  // the same f32 sum as the BF16 quantiser
+            mx = fmaxf(mx, fabsf(vv[e]));
+        }
+    }
+    bad |= !(s <= kMaxSq);
+    // Scale exponents only of finite, bounded rows (|v| <= 1e15, s <= 1e28):
+    // there fp8_scale_exp's first guess is within a step or two of the answer.
+    int e = 0, f = 0;
+    if (!bad) {
+        e = mx > 0.f ? fp8_scale_exp(mx) : 0;
+        f = s > 0.f ? fp8_scale_exp(s) : 0;
+        bad = e < -127 || e > 127 || f < -127 || f > 127;  // outside E8M0
+    }
+    uint32_t* qo = reinterpret_cast<uint32_t*>(q + (size_t)i * kDescLen);
+    if (bad) {
+        for (int c = 0; c < kDescLen / 4; ++c) qo[c] = 0u;
+        meta[i] = Fp8Meta{127u | (127u << 8), 0u};
+        atomicOr(&st->bad, 1u);
+        return;
+    }
+    float res = 0.f, qq = 0.f;
+    for (int c = 0; c < kDescLen; c += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(r + c);
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t b = e4m3_rne(ldexpf(vv[k], -e));
+            const float qf = ldexpf(e4m3_to_f(b), e);
+            const float d = vv[k] - qf;  // exact: a rounding residual is representable
+            res = res + d * d;
+            qq = qq + qf * qf;
+            word |= b << (8 * k);
+        }
+        qo[c / 4] = word;
+    }
+    uint32_t fold = 0u;
+    if (s > 0.f) {
+        const float xs = ldexpf(s, -f);  // in (224, 448]
+        const uint32_t hi = e4m3_rne(xs);
+        const float r1 = xs - e4m3_to_f(hi);  // exact (Sterbenz)
+        const uint32_t mid = e4m3_rne(r1);
+        const float r2 = r1 - e4m3_to_f(mid);
+        const uint32_t lo = e4m3_rne(r2);
+        fold = hi | (mid << 8) | (lo << 16);
+    }
+    meta[i] = Fp8Meta{(uint32_t)(e + 127) | ((uint32_t)(f + 127) << 8), fold};
+    atomic_max_pos(&st->max_norm, up(sqrtf(s) * 1.0001f));
+    atomic_max_pos(&st->max_res, up(sqrtf(res) * 1.0001f));
+    atomic_max_pos(&st->max_qnorm, up(sqrtf(qq) * 1.0001f));
+    atomic_max_pos(&st->max_sq, up(s * 1.0001f));
+}
+
+__global__ void quantize_fp8_kernel(const float* __restrict__ x, uint32_t n,
+                                    uint8_t* __restrict__ q, Fp8Meta* __restrict__ meta,
+                                    ViewStats* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) quantize_row_fp8(x, i, q, meta, st);
+}
+
+__global__ void quantize_views_fp8_kernel(const float* __restrict__ x,
+                                          const uint64_t* __restrict__ off,
+                                          uint8_t* __restrict__ q, Fp8Meta* __restrict__ meta,
+                                          ViewStats* __restrict__ st) {
+    const uint32_t v = blockIdx.x;
+    const uint64_t r = off[v] + (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (r < off[v + 1]) quantize_row_fp8(x, (uint32_t)r, q, meta, st + v);
+}
+
 __global__ void stats_reset_kernel(ViewStats* st, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) st[i] = ViewStats{0.f, 0.f, 0.f, 0.f, 0u, 0u};
@@ -163,23 +286,168 @@ __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
 
 // Huge norm used for padding rows/columns: their keys never win a top-2.
 constexpr uint32_t kPadNorm = 0x00007149u;  // bf16 hi = 0x7149 (~1e30), lo = 0
+// FP8 padding: zero components, fold 448 * 2^92 (~2.2e30).
+constexpr uint32_t kPadCodes = 127u | ((127u + 92u) << 8);
+constexpr uint32_t kPadFold = 0x7eu;
+constexpr uint32_t kNegHalf3 = 0x00b0b0b0u;  // three e4m3 -1/2 bytes
+
+// Operand handling of the two ranking precisions. Per 32 x 32 tile the matrix
+// core runs kSteps MFMAs; a row of b takes kRowBytes in global memory and
+// kStride bytes in LDS (16 bytes of padding keep the 16-byte reads of 8
+// consecutive lanes on distinct banks).
+template <bool F8>
+struct RankOps;
+
+template <>
+struct RankOps<false> {
+    static constexpr int kSteps = 9;  // K = 144: 128 components + the norm fold
+    static constexpr int kRowBytes = 2 * kDescLen;
+    static constexpr int kStride = 272;
+    using Meta = uint32_t;  // packed bf16 norm
+    struct AFrag {
+        bf16x8 f[9];
+    };
+    struct BFrag {
+        bf16x8 f[9];
+    };
+    static __device__ __forceinline__ Meta pad_meta() { return kPadNorm; }
+    static __device__ __forceinline__ Meta meta_a(const LowpTask& t, uint32_t row) {
+        return t.nrm_a[row];
+    }
+    static __device__ __forceinline__ Meta meta_b(const LowpTask& t, uint32_t col) {
+        return t.nrm_b[col];
+    }
+    static __device__ __forceinline__ const uint8_t* rows_a(const LowpTask& t) {
+        return reinterpret_cast<const uint8_t*>(t.qa);
+    }
+    static __device__ __forceinline__ const uint8_t* rows_b(const LowpTask& t) {
+        return reinterpret_cast<const uint8_t*>(t.qb);
+    }
+    // lane (row l32, half h) holds components 16 s + 8 h .. + 7 of step s
+    static __device__ __forceinline__ void load_a(AFrag& a, const uint8_t* row, bool valid,
+                                                  Meta m, bool lo_half) {
+        const uint16_t* src = reinterpret_cast<const uint16_t*>(row) + 8 * (lo_half ? 0 : 1);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            a.f[s] = load_frag(src + 16 * s);
+            if (!valid) a.f[s] = bf16x8{};
+        }
+        a.f[8] = tail_frag(m, true, lo_half);
+    }
+    static __device__ __forceinline__ void load_b(BFrag& b, const uint8_t* lds_row, Meta m,
+                                                  bool lo_half) {
+        const uint16_t* src = reinterpret_cast<const uint16_t*>(lds_row) + 8 * (lo_half ? 0 : 1);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) b.f[s] = load_frag(src + 16 * s);
+        b.f[8] = tail_frag(m, false, lo_half);
+    }
+    static __device__ __forceinline__ f32x16 mfma(const AFrag& a, const BFrag& b, int s,
+                                                  f32x16 acc) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.f[s], b.f[s], acc, 0, 0, 0);
+    }
+};
+
+// e4m3 operands of v_mfma_scale_f32_32x32x64_f8f6f4. Lane (row l32, half h)
+// holds 32 bytes per step; bytes 0-15 of every lane are K block 0 and bytes
+// 16-31 block 1, and block b's E8M0 scale of row/column l32 is the scale
+// operand of lane l32 + 32 b. Steps 0 and 1 take components 64 s + 32 h ..
+// + 31 (A and B alike, so any such assignment gives the dot product), with the
+// row's (column's) scale in both halves. Step 2 is the norm fold: the a row's
+// [hi, mid, lo] at bytes 0-2 (block 0, the fold scale) against the b column's
+// three -1/2 (block 0, scale 1), and three -1/2 at bytes 16-18 (block 1, scale
+// 1) against the b column's [hi, mid, lo] (block 1, its fold scale):
+// -(|a|^2 + |b|^2)/2 to 2^-11 relative.
+template <>
+struct RankOps<true> {
+    static constexpr int kSteps = 3;
+    static constexpr int kRowBytes = kDescLen;
+    static constexpr int kStride = 144;
+    using Meta = Fp8Meta;
+    struct AFrag {
+        i32x8 f[3];
+        int scale, fold_scale;
+    };
+    struct BFrag {
+        i32x8 f[3];
+        int scale, fold_scale;
+    };
+    static __device__ __forceinline__ Meta pad_meta() { return Fp8Meta{kPadCodes, kPadFold}; }
+    static __device__ __forceinline__ Meta meta_a(const LowpTask& t, uint32_t row) {
+        return t.meta_a[row];
+    }
+    static __device__ __forceinline__ Meta meta_b(const LowpTask& t, uint32_t col) {
+        return t.meta_b[col];
+    }
+    static __device__ __forceinline__ const uint8_t* rows_a(const LowpTask& t) { return t.q8a; }
+    static __device__ __forceinline__ const uint8_t* rows_b(const LowpTask& t) { return t.q8b; }
+    static __device__ __forceinline__ i32x8 load32(const uint8_t* p) {
+        union {
+            uint4 u[2];
+            i32x8 v;
+        } f;
+        f.u[0] = *reinterpret_cast<const uint4*>(p);
+        f.u[1] = *reinterpret_cast<const uint4*>(p + 16);
+        return f.v;
+    }
+    static __device__ __forceinline__ i32x8 fold_frag(uint32_t w0, uint32_t w4, bool lo_half) {
+        i32x8 v = i32x8{};
+        if (lo_half) {
+            v[0] = (int)w0;
+            v[4] = (int)w4;
+        }
+        return v;
+    }
+    static __device__ __forceinline__ void load_a(AFrag& a, const uint8_t* row, bool valid, Meta m,
+                                                  bool lo_half) {
+        const uint8_t* src = row + 32 * (lo_half ? 0 : 1);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            a.f[s] = load32(src + 64 * s);
+            if (!valid) a.f[s] = i32x8{};
+        }
+        a.f[2] = fold_frag(m.fold, kNegHalf3, lo_half);
+        a.scale = (int)(m.codes & 0xffu);
+        a.fold_scale = lo_half ? (int)((m.codes >> 8) & 0xffu) : 127;
+    }
+    static __device__ __forceinline__ void load_b(BFrag& b, const uint8_t* lds_row, Meta m,
+                                                  bool lo_half) {
+        const uint8_t* src = lds_row + 32 * (lo_half ? 0 : 1);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) b.f[s] = load32(src + 64 * s);
+        b.f[2] = fold_frag(kNegHalf3, m.fold, lo_half);
+        b.scale = (int)(m.codes & 0xffu);
+        b.fold_scale = lo_half ? 127 : (int)((m.codes >> 8) & 0xffu);
+    }
+    static __device__ __forceinline__ f32x16 mfma(const AFrag& a, const BFrag& b, int s,
+                                                  f32x16 acc) {
+        if (s < 2)
+            return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.f[s], b.f[s], acc, 0, 0, 0,
+                                                                   a.scale, 0, b.scale);
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.f[2], b.f[2], acc, 0, 0, 0,
+                                                               a.fold_scale, 0, b.fold_scale);
+    }
+};
 
 // The 256 x 128 shape is held to 256 VGPRs (two waves per SIMD, a few spills
 // outside the MFMA chains): 12.9 -> 11.7 ms on a 3663-pair scene.
-template <int RB, int CB>
-__global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_kernel(const LowpTask* __restrict__ tasks,
-                                                           const RowBlock* __restrict__ blocks,
-                                                           LowpOut out, uint32_t tbits) {
+template <bool F8, int RB, int CB>
+__global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_kernel(
+    const LowpTask* __restrict__ tasks, const RowBlock* __restrict__ blocks, LowpOut out,
+    uint32_t tbits) {
+    using Ops = RankOps<F8>;
+    using Meta = typename Ops::Meta;
     constexpr int WR = 32 * RB;         // rows per wave
     constexpr int BM = 4 * WR;          // rows per workgroup
     constexpr int BN = 32 * CB;         // columns per chunk
-    constexpr int LDS_STRIDE = 136;     // bf16 per LDS row: 128 + 8 pad (272 B)
-    constexpr int PF = BN * 16 / 256;   // 16-byte pieces of a chunk per thread
+    constexpr int STRIDE = Ops::kStride;
+    constexpr int PIECES = Ops::kRowBytes / 16;  // 16-byte pieces of one row of b
+    constexpr int PF = BN * PIECES / 256;        // pieces of a chunk per thread
+    constexpr int KS = Ops::kSteps;
     // Two b chunks in LDS: chunk ch + 1 is fetched from global memory into
     // registers while chunk ch is multiplied, then stored into the other
     // buffer; one barrier per chunk. The column partials alternate the same way.
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
-    __shared__ uint32_t Bn[2][BN];
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[2][BN * STRIDE];
+    __shared__ Meta Bn[2][BN];
     __shared__ float2 Cs[2][4][BN];
 
     const RowBlock blk = blocks[blockIdx.x];
@@ -195,18 +463,13 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
     asm volatile("v_mov_b32 %0, %1" : "=v"(keep_v) : "s"(~tmask));
 
     // a fragments held in registers for the whole sweep.
-    bf16x8 afrag[RB][9];
+    typename Ops::AFrag afrag[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
         const uint32_t row = blk.row0 + wave * WR + rb * 32 + l32;
         const bool valid = row < t.na;
-        const uint16_t* src = t.qa + (size_t)(valid ? row : 0) * kDescLen + 8 * (lane >> 5);
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            afrag[rb][s] = load_frag(src + 16 * s);
-            if (!valid) afrag[rb][s] = bf16x8{};
-        }
-        afrag[rb][8] = tail_frag(valid ? t.nrm_a[row] : kPadNorm, true, lo_half);
+        const uint8_t* src = Ops::rows_a(t) + (size_t)(valid ? row : 0) * Ops::kRowBytes;
+        Ops::load_a(afrag[rb], src, valid, valid ? Ops::meta_a(t, row) : Ops::pad_meta(), lo_half);
     }
 
     // Running row top-2 per accumulator register: m1 = packed key, m2 = value.
@@ -219,25 +482,28 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
             m2[rb][r] = INFINITY;
         }
 
-    // chunk loader: thread tid fetches pieces e = tid + 256 k (column e / 16,
-    // 16-byte segment e % 16) and, for tid < BN, the column's packed norm
+    // chunk loader: thread tid fetches pieces e = tid + 256 k (column
+    // e / PIECES, 16-byte segment e % PIECES) and, for tid < BN, the column's
+    // metadata
     uint4 pf[PF];
-    uint32_t pfn = kPadNorm;
+    Meta pfn = Ops::pad_meta();
+    const uint8_t* qb = Ops::rows_b(t);
     auto fetch = [&](uint32_t col0) {
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
-            const int e = tid + 256 * k, c = e >> 4, seg = e & 15;
+            const int e = tid + 256 * k, c = e / PIECES, seg = e % PIECES;
             const uint32_t col = col0 + c;
-            pf[k] = col < t.nb ? *reinterpret_cast<const uint4*>(t.qb + (size_t)col * kDescLen + 8 * seg)
+            pf[k] = col < t.nb ? *reinterpret_cast<const uint4*>(qb + (size_t)col * Ops::kRowBytes +
+                                                                  16 * seg)
                                : make_uint4(0, 0, 0, 0);
         }
-        if (tid < BN) pfn = (col0 + tid < t.nb) ? t.nrm_b[col0 + tid] : kPadNorm;
+        if (tid < BN) pfn = (col0 + tid < t.nb) ? Ops::meta_b(t, col0 + tid) : Ops::pad_meta();
     };
     auto store = [&](int buf) {
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
-            const int e = tid + 256 * k, c = e >> 4, seg = e & 15;
-            *reinterpret_cast<uint4*>(&Bs[buf][c * LDS_STRIDE + 8 * seg]) = pf[k];
+            const int e = tid + 256 * k, c = e / PIECES, seg = e % PIECES;
+            *reinterpret_cast<uint4*>(&Bs[buf][c * STRIDE + 16 * seg]) = pf[k];
         }
         if (tid < BN) Bn[buf][tid] = pfn;
     };
@@ -268,11 +534,8 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
     }
     __syncthreads();
     // b fragments of column tile cb of LDS buffer buf
-    auto load_b = [&](bf16x8 (&bf)[9], int buf, int cb) {
-        const uint16_t* brow = &Bs[buf][(cb * 32 + l32) * LDS_STRIDE + 8 * (lane >> 5)];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) bf[s] = load_frag(brow + 16 * s);
-        bf[8] = tail_frag(Bn[buf][cb * 32 + l32], false, lo_half);
+    auto load_b = [&](typename Ops::BFrag& bf, int buf, int cb) {
+        Ops::load_b(bf, &Bs[buf][(cb * 32 + l32) * STRIDE], Bn[buf][cb * 32 + l32], lo_half);
     };
     // epilogue of accumulator registers r0..r1-1 of row tile rb: key = -(acc with
     // the tile id in its low bits) ~ D/2, one v_and_or_b32 (keep_v is a VGPR so
@@ -298,9 +561,9 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
     };
 
     // Software pipeline over the CB*RB tiles of a chunk (tile ti = (cb, rb) =
-    // (ti / RB, ti % RB)): the nine MFMAs of tile ti + 1 are issued one at a
-    // time, each followed by a slice of tile ti's epilogue, so the VALU work
-    // runs while the matrix core is busy. The scheduling barriers keep that
+    // (ti / RB, ti % RB)): the MFMAs of tile ti + 1 are issued one at a time,
+    // each followed by a slice of tile ti's epilogue, so the VALU work runs
+    // while the matrix core is busy. The scheduling barriers keep that
     // interleave; the top-2 updates per register happen in the same order as a
     // plain tile-by-tile sweep, so the results do not depend on it. With two
     // tiles per chunk (128 x 64) the second accumulator costs the kernel its
@@ -311,15 +574,13 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
         if (ch + 1 < nchunks) fetch((ch + 1) * BN);
         if (ch > 0) merge_cols(ch - 1);
 
-        bf16x8 bfrag[9];
+        typename Ops::BFrag bfrag;
         load_b(bfrag, buf, 0);
         f32x16 acc[2];
         if (kPipe) {
             acc[0] = f32x16{};
 #pragma unroll
-            for (int s = 0; s < 9; ++s)
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[0][s], bfrag[s], acc[0], 0,
-                                                                 0, 0);
+            for (int s = 0; s < KS; ++s) acc[0] = Ops::mfma(afrag[0], bfrag, s, acc[0]);
         }
         float c1 = INFINITY, c2 = INFINITY;
 #pragma unroll
@@ -330,20 +591,17 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
                 if (rb == 0 && ti > 0) load_b(bfrag, buf, cb);
                 acc[cur] = f32x16{};
 #pragma unroll
-                for (int s = 0; s < 9; ++s)
-                    acc[cur] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag[rb][s], bfrag[s],
-                                                                       acc[cur], 0, 0, 0);
+                for (int s = 0; s < KS; ++s) acc[cur] = Ops::mfma(afrag[rb], bfrag, s, acc[cur]);
                 epilogue(acc[cur], rb, 0, 16, tile, c1, c2);
             } else if (ti + 1 < CB * RB) {
                 const int nrb = (ti + 1) % RB;
                 if (nrb == 0) load_b(bfrag, buf, (ti + 1) / RB);
                 acc[cur ^ 1] = f32x16{};
 #pragma unroll
-                for (int s = 0; s < 9; ++s) {
-                    acc[cur ^ 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                        afrag[nrb][s], bfrag[s], acc[cur ^ 1], 0, 0, 0);
+                for (int s = 0; s < KS; ++s) {
+                    acc[cur ^ 1] = Ops::mfma(afrag[nrb], bfrag, s, acc[cur ^ 1]);
                     __builtin_amdgcn_sched_barrier(0);
-                    epilogue(acc[cur], rb, (16 * s) / 9, (16 * (s + 1)) / 9, tile, c1, c2);
+                    epilogue(acc[cur], rb, (16 * s) / KS, (16 * (s + 1)) / KS, tile, c1, c2);
                     __builtin_amdgcn_sched_barrier(0);
                 }
             } else {
@@ -420,6 +678,18 @@ __device__ __forceinline__ double pair_eps(const LowpTask& t, uint32_t tbits) {
     const double dot_q = (double)A.max_norm * B.max_res + (double)A.max_res * B.max_qnorm;
     const double qq = (double)A.max_qnorm * B.max_qnorm;
     const double nn = (double)A.max_sq + B.max_sq;
+    if (t.fp8) {
+        // |acc| bound: |q_a . q_b| + (n_a + n_b)/2 (the e4m3 hi+mid+lo folds are
+        // within 2^-11 of n)
+        const double acc = qq + 0.5 * nn * (1.0 + 0x1p-11);
+        const double e = 2.0 * dot_q                  // e4m3 rounding of the components
+                         + 2.0 * 2.0 * 200.0 * u * acc  // accumulation over K = 192 (2x margin)
+                         + 2.0 * ldexp(1.0, (int)tbits - 22) * acc  // tile id in the low bits
+                         + nn * 0x1p-11               // hi + mid + lo fold of the norms
+                         + 64.0 * u * (nn + 2.0 * A.max_norm * (double)B.max_norm)
+                         + 128.0 * u * nn;
+        return e * 1.25 + 1e-6;
+    }
     // |acc| bound: |q_a . q_b| + (n_a + n_b)/2 (the bf16 hi+lo norms are within 2^-15 of n)
     const double acc = qq + 0.5 * nn * (1.0 + 0x1p-15);
     double e = 2.0 * dot_q                      // bf16 rounding of the components
@@ -429,6 +699,15 @@ __device__ __forceinline__ double pair_eps(const LowpTask& t, uint32_t tbits) {
                + 64.0 * u * (nn + 2.0 * A.max_norm * (double)B.max_norm)  // reference f32 rounding
                + 128.0 * u * nn;                // f32 norms of the quantizer
     return e * 1.25 + 1e-6;
+}
+
+// eps[p] = the error bound of pair p (infinity when a view is scored exactly).
+__global__ void pair_eps_kernel(const LowpTask* __restrict__ tasks, uint32_t n, uint32_t tbits,
+                                double* __restrict__ eps) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const LowpTask t = tasks[p];
+    eps[p] = (t.sa->bad | t.sb->bad) ? (double)INFINITY : pair_eps(t, tbits);
 }
 
 // Certain decisions; uncertain rows go to the exact queue.
@@ -809,29 +1088,60 @@ void launch_quantize_views(const float* x, const uint64_t* d_off, uint32_t n_vie
                        s, x, d_off, q, nrm, st);
 }
 
+void launch_quantize_fp8(const float* x, uint32_t n, uint8_t* q, Fp8Meta* meta, ViewStats* st,
+                        hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(quantize_fp8_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, n, q, meta,
+                       st);
+}
+
+void launch_quantize_views_fp8(const float* x, const uint64_t* d_off, uint32_t n_views,
+                               uint32_t max_rows, uint8_t* q, Fp8Meta* meta, ViewStats* st,
+                               hipStream_t s) {
+    if (!n_views || !max_rows) return;
+    hipLaunchKernelGGL(quantize_views_fp8_kernel, dim3(n_views, (max_rows + 255) / 256), dim3(256),
+                       0, s, x, d_off, q, meta, st);
+}
+
 uint32_t lowp_block_rows(int tile) { return tile_rb(tile) * 128u; }
 
-void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
-                      const LowpOut& out, uint32_t tbits, hipStream_t s) {
-    if (!n_blocks) return;
+template <bool F8>
+static void launch_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
+                        const LowpOut& out, uint32_t tbits, hipStream_t s) {
     switch (tile) {
         case kTile128x64:
-            hipLaunchKernelGGL((mfma_rank_kernel<1, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
+            hipLaunchKernelGGL((mfma_rank_kernel<F8, 1, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
                                blocks, out, tbits);
             break;
         case kTile128x128:
-            hipLaunchKernelGGL((mfma_rank_kernel<1, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
+            hipLaunchKernelGGL((mfma_rank_kernel<F8, 1, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
                                blocks, out, tbits);
             break;
         case kTile256x64:
-            hipLaunchKernelGGL((mfma_rank_kernel<2, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
+            hipLaunchKernelGGL((mfma_rank_kernel<F8, 2, 2>), dim3(n_blocks), dim3(256), 0, s, tasks,
                                blocks, out, tbits);
             break;
         default:
-            hipLaunchKernelGGL((mfma_rank_kernel<2, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
+            hipLaunchKernelGGL((mfma_rank_kernel<F8, 2, 4>), dim3(n_blocks), dim3(256), 0, s, tasks,
                                blocks, out, tbits);
             break;
     }
+}
+
+void launch_mfma_rank(int tile, bool fp8, const LowpTask* tasks, const RowBlock* blocks,
+                      uint32_t n_blocks, const LowpOut& out, uint32_t tbits, hipStream_t s) {
+    if (!n_blocks) return;
+    if (fp8)
+        launch_rank<true>(tile, tasks, blocks, n_blocks, out, tbits, s);
+    else
+        launch_rank<false>(tile, tasks, blocks, n_blocks, out, tbits, s);
+}
+
+void launch_pair_eps(const LowpTask* tasks, uint32_t n_pairs, uint32_t tbits, double* eps,
+                     hipStream_t s) {
+    if (!n_pairs) return;
+    hipLaunchKernelGGL(pair_eps_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0, s, tasks,
+                       n_pairs, tbits, eps);
 }
 
 void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,

@@ -1472,7 +1472,10 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
     }
     valid = __all(valid);  // every wave scans every point: the same verdict
     if (!valid) {
-        if (lead && lane == 0) *t.found = 0;
+        if (lead && lane == 0) {
+            *t.found = 0;
+            if (t.iters_out) *t.iters_out = 0;  // no hypothesis drawn
+        }
         return;
     }
     const double th2 = cfg.threshold_px * cfg.threshold_px;

@@ -12,7 +12,7 @@ constexpr int kDescLen = 128;
 
 // Which low-precision ranking kernels this build contains.
 constexpr bool kHaveBf16 = true;
-constexpr bool kHaveFp8 = false;
+constexpr bool kHaveFp8 = true;
 
 // One ratio_match problem: a (na rows) against b (nb rows), both row-major
 // [n][128] f32 in device memory. row_base/col_base index the per-row and

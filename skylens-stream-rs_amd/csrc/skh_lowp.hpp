@@ -6,11 +6,20 @@
 namespace skh {
 
 // Per-view inputs of the error bound (all upper bounds, f32): max |a|,
-// max |a - bf16(a)|, max |bf16(a)|, max |a|^2; bad != 0 if any component is
-// non-finite or too large for the bound.
+// max |a - q(a)|, max |q(a)|, max |a|^2, where q(a) is the quantised row in
+// real units (bf16, or e4m3 times its scale); bad != 0 if any component is
+// non-finite or too large for the bound (such views are scored exactly).
 struct ViewStats {
     float max_norm, max_res, max_qnorm, max_sq;
     uint32_t bad, pad;
+};
+
+// FP8 row metadata: x = E8M0 code of the row's scale 2^e (e + 127) | E8M0
+// code of the norm's fold scale << 8; y = e4m3 bytes hi | mid << 8 | lo << 16
+// of |a|^2 / 2^f (hi + mid + lo carry |a|^2 to 2^-11 relative).
+struct Fp8Meta {
+    uint32_t codes;
+    uint32_t fold;
 };
 
 struct LowpTask {
@@ -20,12 +29,18 @@ struct LowpTask {
     const uint16_t* qb;
     const uint32_t* nrm_a;     // packed bf16 (hi | lo << 16) of |row|^2
     const uint32_t* nrm_b;
+    const uint8_t* q8a;        // e4m3 rows [n][128] (FP8 ranking)
+    const uint8_t* q8b;
+    const Fp8Meta* meta_a;
+    const Fp8Meta* meta_b;
     const ViewStats* sa;
     const ViewStats* sb;
     uint32_t na, nb;
     uint32_t row_base;         // per-row arrays
     uint32_t col_base;         // per-column arrays (col_top2)
     uint64_t part_base;        // column partials: ceil(na / BM) * nb float2
+    uint32_t fp8;              // 1: e4m3 ranking (error bound of the FP8 form)
+    uint32_t pad;
 };
 
 struct LowpOut {
@@ -56,8 +71,17 @@ void launch_quantize(const float* x, uint32_t n, uint16_t* q, uint32_t* nrm, Vie
 void launch_quantize_views(const float* x, const uint64_t* d_off, uint32_t n_views,
                            uint32_t max_rows, uint16_t* q, uint32_t* nrm, ViewStats* st,
                            hipStream_t s);
-void launch_mfma_rank(int tile, const LowpTask* tasks, const RowBlock* blocks, uint32_t n_blocks,
-                      const LowpOut& out, uint32_t tbits, hipStream_t s);
+void launch_quantize_fp8(const float* x, uint32_t n, uint8_t* q, Fp8Meta* meta, ViewStats* st,
+                        hipStream_t s);
+void launch_quantize_views_fp8(const float* x, const uint64_t* d_off, uint32_t n_views,
+                               uint32_t max_rows, uint8_t* q, Fp8Meta* meta, ViewStats* st,
+                               hipStream_t s);
+// MFMA ranking of every row block: BF16 (fp8 = false) or FP8 e4m3 operands.
+void launch_mfma_rank(int tile, bool fp8, const LowpTask* tasks, const RowBlock* blocks,
+                      uint32_t n_blocks, const LowpOut& out, uint32_t tbits, hipStream_t s);
+// Per-pair error bound eps (>= |2 key - D_ref| over every element of the pair).
+void launch_pair_eps(const LowpTask* tasks, uint32_t n_pairs, uint32_t tbits, double* eps,
+                     hipStream_t s);
 void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
                      hipStream_t s);
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,

@@ -18,7 +18,9 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 import threading
+import weakref
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -77,6 +79,7 @@ EXPORTS = [
     ("skh_plan_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("skh_plan_time_distance", ctypes.c_double, [_vp, ctypes.c_int]),
     ("skh_plan_ransac_profile", ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    ("skh_plan_lowp_state", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 RANSAC_PHASES = ["sampling", "hypotheses", "local_opt", "replay", "pool_lm", "final_gates",
@@ -179,13 +182,22 @@ class Context:
             raise SkhError(f"skh_ctx_create({device}) failed: status {rc} "
                            "(a gfx950 device is required)")
         self.handle = h
+        self._plans = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context, after every plan still open on it (a plan's
+        resources belong to its context, so it must go first)."""
         if self.handle:
+            for plan in list(self._plans):
+                plan.close()
             self.lib.skh_ctx_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be gone; the
+        # process exit releases everything.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -345,6 +357,7 @@ class Plan:
         rc = self.lib.skh_plan_create(ctx.handle, ctypes.byref(d), ctypes.byref(h))
         ctx._check(rc, "skh_plan_create")
         self.handle = h
+        ctx._plans.add(self)
         self.match_base = np.zeros(self.n_pairs + 1, np.uint64)
         ctx._check(self.lib.skh_plan_match_base(self.handle, _ptr(self.match_base)),
                    "skh_plan_match_base")
@@ -369,6 +382,32 @@ class Plan:
         self.ctx._check(self.lib.skh_plan_ransac_profile(self.handle, _ptr(it), _ptr(ph),
                                                          ctypes.byref(tpm)), "ransac_profile")
         return it[: self.n_pairs], ph[: self.n_pairs], tpm.value
+
+    def lowp_state(self) -> dict:
+        """The BF16/FP8 ranking state of the last run (skh_plan_lowp_state):
+        per-row keys k1, k2 (~ d^2 / 2) and j1, per-column (c1, c2), per-pair
+        error bound eps, and the number of rows the exact re-score took."""
+        R = int(np.sum(self._rows()))
+        C = int(np.sum(self._cols()))
+        k1 = np.zeros(max(R, 1), np.float32)
+        k2 = np.zeros(max(R, 1), np.float32)
+        j1 = np.zeros(max(R, 1), np.uint32)
+        ct = np.zeros((max(C, 1), 2), np.float32)
+        eps = np.zeros(max(self.n_pairs, 1), np.float64)
+        q = ctypes.c_uint64(0)
+        self.ctx._check(self.lib.skh_plan_lowp_state(self.handle, _ptr(k1), _ptr(j1), _ptr(k2),
+                                                     _ptr(ct), _ptr(eps), ctypes.byref(q)),
+                        "skh_plan_lowp_state")
+        rb = np.concatenate([[0], np.cumsum(self._rows())])
+        cb = np.concatenate([[0], np.cumsum(self._cols())])
+        return {"k1": k1[:R], "k2": k2[:R], "j1": j1[:R], "col": ct[:C], "eps": eps[:self.n_pairs],
+                "queued": int(q.value), "row_base": rb, "col_base": cb}
+
+    def _rows(self):
+        return np.array([self._off[i + 1] - self._off[i] for i, _ in self._pairs], np.int64)
+
+    def _cols(self):
+        return np.array([self._off[j + 1] - self._off[j] for _, j in self._pairs], np.int64)
 
     def time_distance(self, reps: int = 3) -> float:
         """Average ms of the distance kernel alone (HIP events, plan stream)."""
@@ -395,10 +434,13 @@ class Plan:
 
     def close(self):
         if self.handle:
-            self.lib.skh_plan_destroy(self.handle)
+            if self.ctx.handle:
+                self.lib.skh_plan_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
