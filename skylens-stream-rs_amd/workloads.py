@@ -167,6 +167,34 @@ def single_pair(seed: int = 0) -> SinglePair:
     return SinglePair(a, b[perm], b_source[perm], img(kp_a), img(kp_b[perm]), inl[perm])
 
 
+def two_view_correspondences(n: int, sigma: float, outlier_frac: float, seed: int):
+    """n pixel correspondences (f64) between two 960x540 pinholes (fx = fy = 750,
+    5 m baseline, 15 deg yaw): (1 - outlier_frac) n projections of 3D points
+    visible in both (sigma px noise), the rest uniform over the images, shuffled
+    -- the shape of the reference's 4000-correspondence timing case
+    (crates/core/src/matching.rs:2133-2149)."""
+    rng = np.random.default_rng(seed)
+    W, H, F = 960, 540, 750.0
+    c1 = Camera(np.eye(3), np.zeros(3), F, W / 2, H / 2, W, H)
+    c2 = Camera(rot_y(15.0), np.array([5.0, 0.0, 0.0]), F, W / 2, H / 2, W, H)
+    k = int(round((1.0 - outlier_frac) * n))
+    pts = []
+    while sum(len(p) for p in pts) < k:
+        X = np.stack([rng.uniform(-40, 40, 4 * n), rng.uniform(-25, 25, 4 * n),
+                      rng.uniform(50, 90, 4 * n)], 1)
+        p1, v1 = c1.project(X)
+        p2, v2 = c2.project(X)
+        ok = v1 & v2
+        pts.append(np.concatenate([p1[ok], p2[ok]], 1))
+    P = np.concatenate(pts)[:k]
+    x1 = np.concatenate([P[:, :2] + sigma * rng.standard_normal((k, 2)),
+                         np.stack([rng.uniform(0, W, n - k), rng.uniform(0, H, n - k)], 1)])
+    x2 = np.concatenate([P[:, 2:] + sigma * rng.standard_normal((k, 2)),
+                         np.stack([rng.uniform(0, W, n - k), rng.uniform(0, H, n - k)], 1)])
+    perm = rng.permutation(n)
+    return np.ascontiguousarray(x1[perm]), np.ascontiguousarray(x2[perm])
+
+
 # --------------------------------------------------------------------------
 # multi-view scenes
 # --------------------------------------------------------------------------
