@@ -1,12 +1,115 @@
-// Host orchestration of the low-precision (BF16 or FP8) single-pair ratio_match.
+// Host orchestration of the single-pair ratio_match passes: the exact FP32
+// pass and the low-precision (BF16 or FP8) pass, on a set of device buffers.
 #include <algorithm>
-#include <vector>
 
 #include "skh_ctx.hpp"
 #include "skh_lowp.hpp"
+#include "skh_single.hpp"
 #include "skylens_hip.h"
 
 namespace skh {
+namespace {
+
+// Writes the single pair's task records and its row-block list on the device,
+// so a call needs no host-to-device copy (and no host buffer to keep alive).
+__global__ void single_setup_kernel(LowpTask lt, PairTask pt, uint32_t nblk, uint32_t bm,
+                                    LowpTask* __restrict__ d_lt, PairTask* __restrict__ d_pt,
+                                    RowBlock* __restrict__ d_blocks) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        if (d_lt) *d_lt = lt;
+        *d_pt = pt;
+    }
+    if (k < nblk) d_blocks[k] = RowBlock{0u, k * bm};
+}
+
+}  // namespace
+
+void launch_single_setup(const LowpTask* lt, const PairTask& pt, uint32_t nblk, uint32_t bm,
+                         LowpTask* d_lt, PairTask* d_pt, RowBlock* d_blocks, hipStream_t s) {
+    const uint32_t n = std::max(nblk, 1u);
+    hipLaunchKernelGGL(single_setup_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                       lt ? *lt : LowpTask{}, pt, nblk, bm, lt ? d_lt : nullptr, d_pt, d_blocks);
+}
+
+int lowp_tile(int tile_override, size_t na) {
+    return tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile256x64);
+}
+
+// The whole low-precision single-pair pass on the buffers of `B`: set-up,
+// quantisation, MFMA ranking, column merge, certain decisions, exact
+// re-score, compaction into d_out_ij / d_out_len. Asynchronous on `s`; no
+// allocation and no host synchronisation. Needs na >= 1 and nb >= 2.
+void enqueue_lowp_single(const SingleBufs& B, const float* d_a, uint32_t na, const float* d_b,
+                         uint32_t nb, float ratio, int mutual, bool fp8, int tile, int num_cu,
+                         uint32_t* d_out_ij, uint32_t* d_out_len, hipStream_t s) {
+    const uint32_t bm = lowp_block_rows(tile);
+    const uint32_t nblk = (na + bm - 1) / bm;
+    const uint32_t tbits = tile_bits(nb, tile_bn(tile));
+    LowpTask lt{};
+    lt.a = d_a;
+    lt.b = d_b;
+    if (fp8) {
+        lt.q8a = B.q8_a;
+        lt.q8b = B.q8_b;
+        lt.meta_a = B.meta_a;
+        lt.meta_b = B.meta_b;
+        lt.fp8 = 1;
+    } else {
+        lt.qa = B.q_a;
+        lt.qb = B.q_b;
+        lt.nrm_a = B.nrm_a;
+        lt.nrm_b = B.nrm_b;
+    }
+    lt.sa = B.stats;
+    lt.sb = B.stats + 1;
+    lt.na = na;
+    lt.nb = nb;
+    const PairTask pt{d_a, d_b, na, nb, 0u, 0u};
+    launch_single_setup(&lt, pt, nblk, bm, B.ltask, B.task, B.blocks, s);
+
+    LowpOut o{};
+    o.row_k1 = B.row_k1;
+    o.row_j1 = B.row_j1;
+    o.row_k2 = B.row_k2;
+    o.col_part = B.col_part;
+    o.col_top2 = B.col_top2;
+    o.row_match = B.row_match;
+    o.queue = B.queue;
+    o.queue_len = B.queue_len;
+
+    launch_stats_reset(B.stats, 2, s);
+    launch_fill_u32(o.queue_len, 1, 0u, s);
+    launch_fill_u64(reinterpret_cast<unsigned long long*>(B.out_base), 1, 0ull, s);
+    if (fp8) {
+        launch_quantize_fp8(d_a, na, B.q8_a, B.meta_a, B.stats, s);
+        launch_quantize_fp8(d_b, nb, B.q8_b, B.meta_b, B.stats + 1, s);
+    } else {
+        launch_quantize(d_a, na, B.q_a, B.nrm_a, B.stats, s);
+        launch_quantize(d_b, nb, B.q_b, B.nrm_b, B.stats + 1, s);
+    }
+    launch_mfma_rank(tile, fp8, B.ltask, B.blocks, nblk, o, tbits, s);
+    launch_colmerge(B.ltask, 1, o, bm, s);
+    launch_decide(B.ltask, 1, o, ratio, mutual, tbits, s);
+    launch_rescore(B.ltask, o, ratio, mutual, tbits, (uint32_t)std::max(64, 4 * num_cu), s);
+    launch_compact(B.task, 1, o.row_match, B.out_base, d_out_ij, d_out_len, s);
+}
+
+// The exact FP32 pass on the buffers of `B` (row state, column keys, decision,
+// compaction). Asynchronous; no allocation, no host synchronisation.
+void enqueue_exact_single(const SingleBufs& B, const float* d_a, uint32_t na, const float* d_b,
+                          uint32_t nb, float ratio, int mutual, uint32_t* d_out_ij,
+                          uint32_t* d_out_len, hipStream_t s) {
+    const uint32_t nblk = (na + 63) / 64;
+    const PairTask pt{d_a, d_b, na, nb, 0u, 0u};
+    launch_single_setup(nullptr, pt, nblk, 64u, nullptr, B.task, B.blocks, s);
+    launch_fill_u64(B.col_key, nb, kColEmpty, s);
+    launch_exact_state(B.task, B.blocks, nblk, B.row_j1, B.row_d1, B.row_d2, B.col_key, s);
+    launch_fill_u64(reinterpret_cast<unsigned long long*>(B.out_base), 1, 0ull, s);
+    launch_exact_decide(B.task, 1, B.row_j1, B.row_d1, B.row_d2, B.col_key, ratio, mutual,
+                        B.row_match, s);
+    launch_compact(B.task, 1, B.row_match, B.out_base, d_out_ij, d_out_len, s);
+}
 
 // Largest b the packed tile id supports without eroding the key precision
 // below 2^-11 relative.
@@ -20,17 +123,15 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     const bool fp8 = precision == SKH_PREC_FP8;
     if (na == 0 || nb < 2) {
         // No row can pass (matching.rs:206 needs a finite second neighbour).
-        if ((rc = ensure(out_base, sizeof(uint32_t), "out_base"))) return rc;
         launch_fill_u32(d_out_len, 1, 0u, stream);
         return hip(hipGetLastError(), "fill");
     }
     if (nb > kLowpMaxCols)
         return ratio_match_dev(d_a, na, d_b, nb, ratio, mutual, SKH_PREC_FP32, d_out_ij, d_out_len);
 
-    const int tile = tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile256x64);
+    const int tile = lowp_tile(tile_override, na);
     const uint32_t bm = lowp_block_rows(tile);
     const uint32_t nblk = (uint32_t)((na + bm - 1) / bm);
-    const uint32_t tbits = tile_bits(nb, tile_bn(tile));
     if (fp8) {
         if ((rc = ensure(q8_a, kDescLen * na, "q8_a")) || (rc = ensure(q8_b, kDescLen * nb, "q8_b")) ||
             (rc = ensure(meta_a, sizeof(Fp8Meta) * na, "meta_a")) ||
@@ -54,69 +155,30 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
         (rc = ensure(tasks, sizeof(PairTask), "tasks")))
         return rc;
 
-    ViewStats* st = stats.as<ViewStats>();
-    LowpTask lt{};
-    lt.a = d_a;
-    lt.b = d_b;
-    if (fp8) {
-        lt.q8a = q8_a.as<uint8_t>();
-        lt.q8b = q8_b.as<uint8_t>();
-        lt.meta_a = meta_a.as<Fp8Meta>();
-        lt.meta_b = meta_b.as<Fp8Meta>();
-        lt.fp8 = 1;
-    } else {
-        lt.qa = q_a.as<uint16_t>();
-        lt.qb = q_b.as<uint16_t>();
-        lt.nrm_a = nrm_a.as<uint32_t>();
-        lt.nrm_b = nrm_b.as<uint32_t>();
-    }
-    lt.sa = st;
-    lt.sb = st + 1;
-    lt.na = (uint32_t)na;
-    lt.nb = (uint32_t)nb;
-    PairTask pt{d_a, d_b, (uint32_t)na, (uint32_t)nb, 0u, 0u};
-    h_blocks.resize(nblk);
-    for (uint32_t k = 0; k < nblk; ++k) h_blocks[k] = RowBlock{0u, k * bm};
-    if ((rc = hip(hipMemcpyAsync(ltasks.ptr, &lt, sizeof lt, hipMemcpyHostToDevice, stream),
-                  "copy ltask")) ||
-        (rc = hip(hipMemcpyAsync(tasks.ptr, &pt, sizeof pt, hipMemcpyHostToDevice, stream),
-                  "copy task")) ||
-        (rc = hip(hipMemcpyAsync(blocks.ptr, h_blocks.data(), sizeof(RowBlock) * nblk,
-                                 hipMemcpyHostToDevice, stream),
-                  "copy blocks")))
-        return rc;
-    // The host arrays above must outlive the async copies.
-    if ((rc = hip(hipStreamSynchronize(stream), "sync"))) return rc;
-
-    LowpOut o{};
-    o.row_k1 = row_k1.as<float>();
-    o.row_j1 = row_j1.as<uint32_t>();
-    o.row_k2 = row_k2.as<float>();
-    o.col_part = col_part.as<float2>();
-    o.col_top2 = col_top2.as<float2>();
-    o.row_match = row_match.as<uint32_t>();
-    o.queue = queue.as<uint4>();
-    o.queue_len = queue_len.as<uint32_t>();
-
-    launch_stats_reset(st, 2, stream);
-    launch_fill_u32(o.queue_len, 1, 0u, stream);
-    launch_fill_u64(reinterpret_cast<unsigned long long*>(out_base.ptr), 1, 0ull, stream);
-    if (fp8) {
-        launch_quantize_fp8(d_a, (uint32_t)na, q8_a.as<uint8_t>(), meta_a.as<Fp8Meta>(), st, stream);
-        launch_quantize_fp8(d_b, (uint32_t)nb, q8_b.as<uint8_t>(), meta_b.as<Fp8Meta>(), st + 1,
-                            stream);
-    } else {
-        launch_quantize(d_a, (uint32_t)na, q_a.as<uint16_t>(), nrm_a.as<uint32_t>(), st, stream);
-        launch_quantize(d_b, (uint32_t)nb, q_b.as<uint16_t>(), nrm_b.as<uint32_t>(), st + 1, stream);
-    }
-    launch_mfma_rank(tile, fp8, ltasks.as<LowpTask>(), blocks.as<RowBlock>(), nblk, o, tbits,
-                     stream);
-    launch_colmerge(ltasks.as<LowpTask>(), 1, o, bm, stream);
-    launch_decide(ltasks.as<LowpTask>(), 1, o, ratio, mutual, tbits, stream);
-    launch_rescore(ltasks.as<LowpTask>(), o, ratio, mutual, tbits, (uint32_t)std::max(64, 4 * num_cu),
-                   stream);
-    launch_compact(tasks.as<PairTask>(), 1, o.row_match, out_base.as<uint64_t>(), d_out_ij,
-                   d_out_len, stream);
+    SingleBufs B{};
+    B.q_a = q_a.as<uint16_t>();
+    B.q_b = q_b.as<uint16_t>();
+    B.nrm_a = nrm_a.as<uint32_t>();
+    B.nrm_b = nrm_b.as<uint32_t>();
+    B.q8_a = q8_a.as<uint8_t>();
+    B.q8_b = q8_b.as<uint8_t>();
+    B.meta_a = meta_a.as<Fp8Meta>();
+    B.meta_b = meta_b.as<Fp8Meta>();
+    B.stats = stats.as<ViewStats>();
+    B.ltask = ltasks.as<LowpTask>();
+    B.task = tasks.as<PairTask>();
+    B.blocks = blocks.as<RowBlock>();
+    B.row_k1 = row_k1.as<float>();
+    B.row_j1 = row_j1.as<uint32_t>();
+    B.row_k2 = row_k2.as<float>();
+    B.col_part = col_part.as<float2>();
+    B.col_top2 = col_top2.as<float2>();
+    B.row_match = row_match.as<uint32_t>();
+    B.queue = queue.as<uint4>();
+    B.queue_len = queue_len.as<uint32_t>();
+    B.out_base = out_base.as<uint64_t>();
+    enqueue_lowp_single(B, d_a, (uint32_t)na, d_b, (uint32_t)nb, ratio, mutual, fp8, tile, num_cu,
+                        d_out_ij, d_out_len, stream);
     return hip(hipGetLastError(), "launch lowp");
 }
 
