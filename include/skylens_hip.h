@@ -90,6 +90,19 @@ int skh_ratio_match_dev(skh_ctx* ctx, const float* d_a, size_t na, const float* 
                         float ratio, int mutual, int precision, uint32_t* d_out_ij,
                         uint32_t* d_out_len);
 
+/* skh_ratio_match_dev without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and asynchrony as skh_ratio_match_dev; its working
+ * buffers are one fixed device array of about 8.6 MB per GPU, held until the
+ * process ends, so na and nb are limited to 8192 rows each (the reference's
+ * default feature cap, features.rs:285); larger inputs return SKH_EINVAL.
+ * Every call on a GPU shares that array and leaves its contents behind:
+ * calls that overlap on one GPU (several contexts or streams at once) race. */
+int skh_ratio_match_dev_noalloc(skh_ctx* ctx, const float* d_a, size_t na, const float* d_b,
+                                size_t nb, float ratio, int mutual, int precision,
+                                uint32_t* d_out_ij, uint32_t* d_out_len);
+
 /* Exact per-row state of ratio_match for diagnostics and tests: row_j1
  * (UINT32_MAX = none), row_d1, row_d2 (na each) and the column winner
  * col_i (UINT32_MAX = none), col_d (nb each), all host buffers, computed

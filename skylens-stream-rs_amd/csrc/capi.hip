@@ -135,6 +135,9 @@ extern "C" int skh_ctx_create(int device, skh_ctx** out) {
         delete c;
         return SKH_EHIP;
     }
+    // The allocation-free entry point's scratch: its device address on this
+    // device, looked up once per device with the device current.
+    if (noalloc_scratch_address(device, &c->noalloc_scratch) != SKH_OK) c->noalloc_scratch = nullptr;
     *out = c;
     return SKH_OK;
 }

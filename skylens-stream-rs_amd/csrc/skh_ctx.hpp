@@ -27,6 +27,7 @@ struct Ctx {
     int num_cu = 0;
     int tile_override = -1;  // SKH_TILE_AUTO
     uint64_t last_iters = 0;  // hypotheses drawn by the last single-pair RANSAC
+    void* noalloc_scratch = nullptr;  // device address of the allocation-free entry's scratch
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string last_error;
@@ -63,6 +64,9 @@ struct Ctx {
     int ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t nb, float ratio,
                          int mutual, int precision, uint32_t* d_out_ij, uint32_t* d_out_len);
 };
+
+int noalloc_scratch_address(int device, void** out);
+size_t noalloc_max_rows();
 
 }  // namespace skh
 

@@ -63,6 +63,9 @@ EXPORTS = [
     ("skh_ratio_match_dev", ctypes.c_int,
      [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_int,
       ctypes.c_int, _vp, _vp]),
+    ("skh_ratio_match_dev_noalloc", ctypes.c_int,
+     [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_int,
+      ctypes.c_int, _vp, _vp]),
     ("skh_ratio_match_state", ctypes.c_int,
      [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
     ("skh_ransac_cfg_default", None, [_vp]),
@@ -293,6 +296,16 @@ class Context:
                                           int(bool(mutual)), PRECISIONS[precision], out_ij_ptr,
                                           out_len_ptr)
         self._check(rc, "skh_ratio_match_dev")
+
+
+    def ratio_match_dev_noalloc(self, a_ptr: int, na: int, b_ptr: int, nb: int, ratio: float,
+                                mutual: bool, precision: str, out_ij_ptr: int, out_len_ptr: int):
+        """ratio_match_dev without allocating (fixed per-GPU scratch; at most 8192
+        rows per side; calls overlapping on one GPU race)."""
+        rc = self.lib.skh_ratio_match_dev_noalloc(self.handle, a_ptr, na, b_ptr, nb,
+                                                  ctypes.c_float(ratio), int(bool(mutual)),
+                                                  PRECISIONS[precision], out_ij_ptr, out_len_ptr)
+        self._check(rc, "skh_ratio_match_dev_noalloc")
 
 
 class BatchDesc(ctypes.Structure):
