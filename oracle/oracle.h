@@ -110,6 +110,32 @@ void orc_lm_work(uint64_t* iters, uint64_t* evals);
 void orc_ransac_last_stats(uint64_t* iters, uint64_t* lo_runs, uint64_t* lm_calls,
                            uint64_t* lm_converged, uint64_t* lm_stalled_at_start);
 
+/* Trace of the hypothesis loop (matching.rs:967-1036), for the tests that
+ * pin the loop's semantics against independent restatements. One record per
+ * iteration: the sample, whether fundamental_8pt gave Some, the minimal
+ * sample's F and inlier count, the best count before the iteration, whether
+ * local optimisation ran and its six stages, the LO result, the iteration
+ * limit after the iteration, and the pool (count and origin iteration of each
+ * entry) after it. lo_masks (cap x n bytes) receives each LO result's mask. */
+typedef struct {
+    uint64_t it;
+    uint64_t idx[8];
+    int32_t ok, lo;
+    uint64_t raw_cnt, lo_cnt, best_before, iters_after;
+    double F[9], F_lo[9];
+    int32_t npool, pad;
+    uint64_t pool_cnt[5], pool_origin[5];
+    uint64_t stage_ns[6];
+    int32_t stage_ok[6];
+    uint64_t stage_cnt[6];
+    double stage_F[6][9];
+} orc_trace_rec;
+/* Starts recording into recs (at most cap records) for the next
+ * orc_ransac_fundamental call on this thread; n = its correspondence count. */
+void orc_ransac_trace_begin(orc_trace_rec* recs, size_t cap, uint8_t* lo_masks, size_t n);
+/* Stops recording; returns the number of records written. */
+size_t orc_ransac_trace_end(void);
+
 /* Eigen-solve of the symmetric 9x9 A used by the 8-point and DLT solvers
  * (stands in for nalgebra SymmetricEigen, matching.rs:539-541,
  * two_view.rs:638-639): the nine eigenvalues and a unit eigenvector of the

@@ -1220,7 +1220,38 @@ typedef struct {
     double F[9];
     uint8_t* mask;
     size_t cnt;
+    uint64_t origin; /* iteration whose local optimisation produced it (trace only) */
 } hyp_t;
+
+/* Hypothesis-loop trace (tests only; see oracle.h). */
+static __thread orc_trace_rec* g_tr = NULL;
+static __thread size_t g_tr_cap = 0, g_tr_len = 0, g_tr_n = 0;
+static __thread uint8_t* g_tr_masks = NULL;
+
+void orc_ransac_trace_begin(orc_trace_rec* recs, size_t cap, uint8_t* lo_masks, size_t n) {
+    g_tr = recs;
+    g_tr_cap = cap;
+    g_tr_len = 0;
+    g_tr_n = n;
+    g_tr_masks = lo_masks;
+}
+
+size_t orc_ransac_trace_end(void) {
+    g_tr = NULL;
+    return g_tr_len;
+}
+
+/* Closes the current record: the limit and pool after the iteration. */
+static void trace_close(orc_trace_rec* r, uint64_t iters, const hyp_t* pool, int npool) {
+    if (!r) return;
+    r->iters_after = iters;
+    r->npool = npool;
+    for (int k = 0; k < npool && k < 5; ++k) {
+        r->pool_cnt[k] = pool[k].cnt;
+        r->pool_origin[k] = pool[k].origin;
+    }
+    g_tr_len++;
+}
 
 void orc_lm_work(uint64_t* iters, uint64_t* evals) {
     *iters = g_lm_iters;
@@ -1259,6 +1290,8 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
     uint64_t iters = cfg->max_iters, it = 0;
     while (it < iters) {
         it++;
+        orc_trace_rec* tr = (g_tr && g_tr_len < g_tr_cap && g_tr_n == n) ? g_tr + g_tr_len : NULL;
+        if (tr) memset(tr, 0, sizeof *tr);
         uint64_t idx[8];
         int k = 0;
         while (k < 8) {
@@ -1272,10 +1305,23 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
             a8[2 * e] = x1[2 * idx[e]]; a8[2 * e + 1] = x1[2 * idx[e] + 1];
             b8[2 * e] = x2[2 * idx[e]]; b8[2 * e + 1] = x2[2 * idx[e] + 1];
         }
-        double f[9];
-        if (!orc_fundamental_8pt(a8, b8, 8, f)) continue;
-        size_t cnt = inliers_of(f, x1, x2, n, th2, inl);
         size_t best_cnt = have_best ? best_cnt_v : 0;
+        if (tr) {
+            tr->it = it;
+            memcpy(tr->idx, idx, sizeof idx);
+            tr->best_before = best_cnt;
+        }
+        double f[9];
+        if (!orc_fundamental_8pt(a8, b8, 8, f)) {
+            trace_close(tr, iters, pool, npool);
+            continue;
+        }
+        size_t cnt = inliers_of(f, x1, x2, n, th2, inl);
+        if (tr) {
+            tr->ok = 1;
+            memcpy(tr->F, f, sizeof f);
+            tr->raw_cnt = tr->lo_cnt = cnt;
+        }
         if (cnt >= 8 && 4 * cnt >= best_cnt) {
             g_lo_runs++;
             double f_lo[9], cur[9];
@@ -1285,14 +1331,26 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
                 double t2 = th2 * NM[r] * NM[r];
                 size_t ns = select_pts(cur, x1, x2, n, t2, s1, s2);
                 double g[9];
+                if (tr) tr->stage_ns[r] = ns;
                 if (!orc_fundamental_8pt(s1, s2, ns, g)) break;
                 memcpy(cur, g, sizeof cur);
                 size_t gc = inliers_of(g, x1, x2, n, th2, gi);
+                if (tr) {
+                    tr->stage_ok[r] = 1;
+                    tr->stage_cnt[r] = gc;
+                    memcpy(tr->stage_F[r], g, sizeof g);
+                }
                 if (gc > cnt) {
                     memcpy(f_lo, g, sizeof f_lo);
                     memcpy(inl, gi, n);
                     cnt = gc;
                 }
+            }
+            if (tr) {
+                tr->lo = 1;
+                tr->lo_cnt = cnt;
+                memcpy(tr->F_lo, f_lo, sizeof f_lo);
+                if (g_tr_masks) memcpy(g_tr_masks + g_tr_len * n, inl, n);
             }
             /* matching.rs:1017-1028: Jaccard > 0.7 valley test, stable sort. */
             int same = -1;
@@ -1309,11 +1367,13 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
                     memcpy(pool[same].F, f_lo, sizeof f_lo);
                     memcpy(pool[same].mask, inl, n);
                     pool[same].cnt = cnt;
+                    pool[same].origin = it;
                 }
             } else {
                 memcpy(pool[npool].F, f_lo, sizeof f_lo);
                 memcpy(pool[npool].mask, inl, n);
                 pool[npool].cnt = cnt;
+                pool[npool].origin = it;
                 npool++;
             }
             /* stable insertion sort by count, descending */
@@ -1327,7 +1387,10 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
                 pool[b + 1] = x;
             }
             if (npool > 5) npool = 5; /* the dropped mask buffer stays at pool[5] */
-            if (cnt <= best_cnt) continue;
+            if (cnt <= best_cnt) {
+                trace_close(tr, iters, pool, npool);
+                continue;
+            }
             double w = (double)cnt / (double)n;
             iters = orc_adaptive_iterations(w, 8, cfg->confidence, cfg->max_iters);
             memcpy(bestF, f_lo, sizeof bestF);
@@ -1335,6 +1398,7 @@ int orc_ransac_fundamental(const double* x1, const double* x2, size_t n, const o
             best_cnt_v = cnt;
             have_best = 1;
         }
+        trace_close(tr, iters, pool, npool);
     }
     g_iters = it;
     int result = 0;

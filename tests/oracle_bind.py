@@ -173,6 +173,8 @@ _SIGS_R = {
     "orc_select_two_view_model": (ctypes.c_int, [_vp, _vp, _sz, _vp, _d, _vp]),
     "orc_ransac_fundamental": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp]),
     "orc_ransac_last_stats": (None, [_vp, _vp, _vp, _vp, _vp]),
+    "orc_ransac_trace_begin": (None, [_vp, _sz, _vp, _sz]),
+    "orc_ransac_trace_end": (_sz, []),
     "orc_eigen_min9": (None, [_vp, _vp, _vp]),
     "orc_eigen_min_inv9": (ctypes.c_int, [_vp, _vp]),
     "orc_two_cameras": (None, [_vp, _vp]),
@@ -364,3 +366,27 @@ def line_plus_general(k, sigma, seed):
     h2 = np.zeros((100, 2))
     lib().orc_line_plus_general(k, sigma, seed, P(x1), P(x2), P(h1), P(h2))
     return x1, x2, h1, h2
+
+
+class TraceRec(ctypes.Structure):
+    """orc_trace_rec: one iteration of the hypothesis loop (matching.rs:967-1036)."""
+    _fields_ = [("it", _u64), ("idx", _u64 * 8), ("ok", ctypes.c_int32), ("lo", ctypes.c_int32),
+                ("raw_cnt", _u64), ("lo_cnt", _u64), ("best_before", _u64),
+                ("iters_after", _u64), ("F", _d * 9), ("F_lo", _d * 9),
+                ("npool", ctypes.c_int32), ("pad", ctypes.c_int32), ("pool_cnt", _u64 * 5),
+                ("pool_origin", _u64 * 5), ("stage_ns", _u64 * 6), ("stage_ok", ctypes.c_int32 * 6),
+                ("stage_cnt", _u64 * 6), ("stage_F", (_d * 9) * 6)]
+
+
+def ransac_traced(x1, x2, cfg=None, cap=4096):
+    """(result as ransac_fundamental, trace records, LO masks (records x n bool))."""
+    x1, x2 = _xy(x1), _xy(x2)
+    n = x1.shape[0]
+    recs = (TraceRec * cap)()
+    masks = np.zeros((cap, max(n, 1)), np.uint8)
+    lib().orc_ransac_trace_begin(ctypes.addressof(recs), cap, P(masks), n)
+    try:
+        res = ransac_fundamental(x1, x2, cfg)
+    finally:
+        k = lib().orc_ransac_trace_end()
+    return res, recs[:k], masks[:k, :n].astype(bool)
