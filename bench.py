@@ -189,6 +189,32 @@ def ransac_roofline(batch, plan, ransac_ms):
             "stage_ms": round(ransac_ms, 3)}
 
 
+def tile_comparison(batch, ctx, prec):
+    """Distance-kernel ms of one launch over the batch with each fixed tile and
+    with the context's tuned table (pairs grouped by tile, the groups' launches
+    overlapped)."""
+    import skylens_hip
+
+    out = {}
+    for t, name in enumerate(skylens_hip.TILE_NAMES):
+        ctx.set_tile(t)
+        p = batch.plan(ctx, prec, run_ransac=False)
+        p.run()
+        p.sync()
+        out[name] = round(p.time_distance(3), 4)
+        p.close()
+    ctx.set_tile(-1)
+    p = batch.plan(ctx, prec, run_ransac=False)
+    p.run()
+    p.sync()
+    out["tuned"] = round(p.time_distance(3), 4)
+    out["tuned_tiles"] = p.tiles()
+    p.close()
+    best = min(out[n] for n in skylens_hip.TILE_NAMES)
+    out["tuned_vs_best_fixed"] = round(best / out["tuned"], 4)
+    return out
+
+
 def e2e_with_h2d(batch, plan, steps=3):
     """Pairs/s when each step copies the inputs from pinned host memory, runs the
     plan and copies every result back (the PCIe-inclusive rate)."""
@@ -228,6 +254,7 @@ def secondary(name, precs, ctx):
              "match_ms": round(m_ms, 3), "ransac_ms": round(r_ms, 3),
              "distance_kernel_ms": round(k_ms, 4)}
         if prec in PEAK:
+            d["tiles"] = plan.tiles()
             ach = b.gemm_flop / (k_ms * 1e-3) / 1e12
             d["distance_roofline"] = {"achieved": round(ach, 2), "peak": PEAK[prec],
                                       "unit": "TFLOP/s", "frac": round(ach / PEAK[prec], 4)}
@@ -369,6 +396,8 @@ def main():
                          "consecutive steps rotate over; 1 for profiler runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the tile autotuner (every pair on the fixed default tile)")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the full-scene and 8192-feature lines and the single-pair latencies")
     ap.add_argument("--kernel-only", action="store_true",
@@ -401,6 +430,16 @@ def main():
     S = args.streams
     desc, seed0 = WORKLOADS[args.workload]
     ctxs = [skylens_hip.Context(0) for _ in range(S)]
+    tables = {}
+    if not args.no_tune:
+        # the tile autotuner: fastest tile per size class, loaded into every context
+        for p in ("bf16", "fp8"):
+            if args.precision == p or not args.no_secondary:
+                t = time.time()
+                tables[p], _ = ctxs[0].autotune(p)
+                for c in ctxs[1:]:
+                    c.set_tile_table(p, tables[p])
+                log(f"autotune {p}: {tables[p].tolist()} ({time.time() - t:.1f} s)")
     batches = [Batch(args.workload, seed0 + s) for s in range(S)]
     torch.cuda.synchronize()
     prec = args.precision
@@ -436,6 +475,10 @@ def main():
     if prec != "fp32":
         extra["fp64_error"] = fp64_error(batches[0], plans[0])
         log(f"fp64 error: {extra['fp64_error']}")
+        if prec in tables:
+            extra["tile_table"] = [skylens_hip.TILE_NAMES[k] for k in tables[prec]]
+            extra["tiles"] = tile_comparison(batches[0], ctxs[0], prec)
+            log(f"tiles: {extra['tiles']}")
     rroof = ransac_roofline(batches[0], plans[0], ransac_ms)
     log(f"ransac roofline: {rroof}")
     extra["e2e_with_h2d"] = e2e_with_h2d(batches[0], plans[0])

@@ -54,14 +54,31 @@ typedef struct skh_ctx skh_ctx;
 int skh_ctx_create(int device, skh_ctx** out);
 int skh_ctx_destroy(skh_ctx* ctx);
 /* MFMA ranking tile (rows per workgroup x columns per chunk). SKH_TILE_AUTO
- * lets the context pick: 256x64, or 128x64 for a single pair with at most
- * 128 rows. */
+ * lets the context pick per pair: from its tuned table (skh_ctx_autotune or
+ * skh_ctx_set_tile_table) by the pair's size class, else 256x64 (128x64 for
+ * at most 128 rows). A fixed tile applies to every pair. */
 #define SKH_TILE_AUTO -1
 #define SKH_TILE_128x64 0
 #define SKH_TILE_128x128 1
 #define SKH_TILE_256x64 2
 #define SKH_TILE_256x128 3
 int skh_ctx_set_tile(skh_ctx* ctx, int tile);
+/* Tile autotuner (the batched driver's tile choice, benches/pipeline.rs:219-305
+ * runs every pair the same way on the CPU). Pairs fall into 25 size classes:
+ * rows and columns each in one of 5 buckets (<= 192, <= 384, <= 1024,
+ * <= 3072, more), class = 5 * bucket(rows) + bucket(columns). For each class
+ * the tuner times the four tiles' ranking kernel on a batch of synthetic
+ * pairs of that class (a few seconds in all), stores the fastest in the
+ * context's table for `precision` (SKH_PREC_BF16 or SKH_PREC_FP8) and copies
+ * it to table (25 entries, SKH_TILE_*) and the times to ms (25 x 4, ms per
+ * launch, class-major); either may be NULL. Results never depend on the tile. */
+int skh_ctx_autotune(skh_ctx* ctx, int precision, int32_t* table, double* ms);
+/* Loads a table (25 SKH_TILE_* entries) for `precision`, e.g. one saved from
+ * an earlier skh_ctx_autotune; NULL clears it. */
+int skh_ctx_set_tile_table(skh_ctx* ctx, int precision, const int32_t* table);
+/* The tile the context picks for each of the 25 classes; *tuned = 1 if a
+ * table is loaded (else the defaults are reported). */
+int skh_ctx_tile_table(skh_ctx* ctx, int precision, int32_t* table, int* tuned);
 /* The HIP stream (hipStream_t) the context launches on. */
 void* skh_ctx_stream(skh_ctx* ctx);
 /* Last error message of this context (static storage owned by ctx). */
@@ -218,6 +235,9 @@ double skh_plan_time_distance(skh_plan* plan, int reps);
  * FP32 re-score. Any pointer may be NULL. Synchronises. */
 int skh_plan_lowp_state(skh_plan* plan, float* row_k1, uint32_t* row_j1, float* row_k2,
                         float* col_top2, double* eps, uint64_t* queued);
+/* Pairs of the plan ranked with each tile (4 entries, SKH_TILE_* order; all
+ * zero for an FP32 plan). */
+int skh_plan_tiles(skh_plan* plan, uint32_t* pairs_per_tile);
 /* 1 = record stage events in each run (costs nothing measurable), 0 = off. */
 int skh_plan_set_timing(skh_plan* plan, int on);
 /* RANSAC diagnostics of the last run (any pointer may be NULL): iters[p] =

@@ -44,6 +44,24 @@ __global__ void handoff_kernel(const uint32_t* __restrict__ match_ij,
     }
 }
 
+// Pseudo-random unit rows for the tile tuner: row r, component k from a hash
+// of (r, k), then scaled to unit length (one 128-thread block per row).
+__global__ void tune_fill_kernel(float* __restrict__ d, uint64_t rows) {
+    __shared__ float part[2];
+    const uint64_t r = blockIdx.x;
+    const uint32_t k = threadIdx.x;
+    uint64_t h = (r * 128 + k + 1) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 27;
+    const float v = (float)((h >> 40) & 0xffffff) * 0x1p-24f - 0.5f;
+    float sq = v * v;
+    for (int off = 32; off >= 1; off >>= 1) sq += __shfl_xor(sq, off);
+    if ((k & 63) == 0) part[k >> 6] = sq;
+    __syncthreads();
+    if (r < rows) d[r * 128 + k] = v / sqrtf(part[0] + part[1]);
+}
+
 }  // namespace
 
 struct Plan {
@@ -52,8 +70,18 @@ struct Plan {
     std::vector<uint64_t> view_off;
     std::vector<uint32_t> pairs;
     std::vector<uint64_t> match_base;  // n_pairs + 1
-    int tile = kTile256x64;
+    int tile = kTile256x64;  // the fixed tile, or the tuned plan's most used one
+    int forced_tile = -1;    // set by the tuner: every pair on this tile
     uint32_t tbits = 1;
+    // Row blocks of the ranking launch, grouped by tile: group g holds the
+    // blocks [grp_off[g], grp_off[g] + grp_n[g]) of d_blocks (the exact plan
+    // uses group 0). Groups after the first run on side streams forked from
+    // the plan's stream, so the groups' launches overlap.
+    uint32_t grp_off[kNumTiles] = {}, grp_n[kNumTiles] = {}, pairs_per_tile[kNumTiles] = {};
+    int grp_tile[kNumTiles] = {};
+    int n_groups = 0;
+    hipStream_t side[kNumTiles - 1] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kNumTiles - 1] = {};
     bool lowp = false;
     bool fp8 = false;
     uint32_t n_blocks = 0;
@@ -92,6 +120,11 @@ struct Plan {
         if (g_ransac) (void)hipGraphDestroy(g_ransac);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (int k = 0; k < kNumTiles - 1; ++k) {
+            if (join_ev[k]) (void)hipEventDestroy(join_ev[k]);
+            if (side[k]) (void)hipStreamDestroy(side[k]);
+        }
         for (DevBuf* b : bufs())
             if (b->ptr) (void)hipFree(b->ptr);
     }
@@ -125,9 +158,20 @@ int Plan::build() {
     lowp = desc.precision == SKH_PREC_BF16 || fp8;
     for (uint32_t v = 0; v < V; ++v) max_view_rows = std::max(max_view_rows, rows(v));
     if (lowp && max_view_rows > 131072) lowp = fp8 = false;  // tile ids would erode the key precision
-    tile = ctx->tile_override >= 0 ? ctx->tile_override : kTile256x64;
-    const uint32_t bm = lowp ? lowp_block_rows(tile) : 64u;
-    tbits = tile_bits(max_view_rows, tile_bn(tile));
+    // Tile of every pair: the tuner's forced tile, the context's fixed tile, the
+    // context's tuned table for this precision (by size class), or 256 x 64.
+    std::vector<int> ptile(P, kTile256x64);
+    for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t na = rows(pairs[2 * p]), nb = rows(pairs[2 * p + 1]);
+        ptile[p] = forced_tile >= 0 ? forced_tile : ctx->pick_tile(desc.precision, na, nb);
+    }
+    tbits = 1;
+    for (uint32_t p = 0; p < P; ++p) {
+        pairs_per_tile[ptile[p]]++;
+        if (lowp) tbits = std::max(tbits, tile_bits(max_view_rows, tile_bn(ptile[p])));
+    }
+    tile = (int)(std::max_element(pairs_per_tile, pairs_per_tile + kNumTiles) - pairs_per_tile);
+    auto block_rows = [&](uint32_t p) { return lowp ? lowp_block_rows(ptile[p]) : 64u; };
 
     // per-pair bases
     std::vector<PairTask> pt(P);
@@ -149,11 +193,12 @@ int Plan::build() {
         l.row_base = (uint32_t)rb;
         l.col_base = (uint32_t)cb;
         l.part_base = pb;
+        l.bm = block_rows(p);
         kp_a[p] = view_off[i];
         kp_b[p] = view_off[j];
         rb += na;
         cb += nb;
-        pb += (uint64_t)((na + bm - 1) / bm) * nb;
+        pb += (uint64_t)((na + l.bm - 1) / l.bm) * nb;
         const uint32_t cap = desc.mutual ? std::min(na, nb) : na;
         match_base[p + 1] = match_base[p] + cap;
     }
@@ -198,27 +243,35 @@ int Plan::build() {
         }
     }
 
-    // XCD-aware, longest-first row-block order
+    // XCD-aware, longest-first row-block order, one list per tile group
     std::vector<uint32_t> order(P);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         return (uint64_t)pt[x].na * pt[x].nb > (uint64_t)pt[y].na * pt[y].nb;
     });
-    std::vector<std::vector<RowBlock>> lists(kXcds);
-    std::vector<uint64_t> load(kXcds, 0);
-    for (uint32_t p : order) {
-        if (pt[p].na == 0) continue;
-        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[x] += (uint64_t)pt[p].na * std::max<uint32_t>(pt[p].nb, 1);
-        for (uint32_t r0 = 0; r0 < pt[p].na; r0 += bm) lists[x].push_back(RowBlock{p, r0});
-    }
-    size_t longest = 0;
-    for (auto& l : lists) longest = std::max(longest, l.size());
     std::vector<RowBlock> blocks;
-    blocks.reserve(longest * kXcds);
-    for (size_t s = 0; s < longest; ++s)
-        for (int x = 0; x < kXcds; ++x)
-            blocks.push_back(s < lists[x].size() ? lists[x][s] : RowBlock{kNone, 0u});
+    int ngroups = 0;
+    for (int tl = 0; tl < kNumTiles; ++tl) {
+        if (lowp ? !pairs_per_tile[tl] : tl > 0) continue;
+        std::vector<std::vector<RowBlock>> lists(kXcds);
+        std::vector<uint64_t> load(kXcds, 0);
+        for (uint32_t p : order) {
+            if (pt[p].na == 0 || (lowp && ptile[p] != tl)) continue;
+            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += (uint64_t)pt[p].na * std::max<uint32_t>(pt[p].nb, 1);
+            for (uint32_t r0 = 0; r0 < pt[p].na; r0 += lt[p].bm) lists[x].push_back(RowBlock{p, r0});
+        }
+        size_t longest = 0;
+        for (auto& l : lists) longest = std::max(longest, l.size());
+        grp_off[ngroups] = (uint32_t)blocks.size();
+        for (size_t r = 0; r < longest; ++r)
+            for (int x = 0; x < kXcds; ++x)
+                blocks.push_back(r < lists[x].size() ? lists[x][r] : RowBlock{kNone, 0u});
+        grp_n[ngroups] = (uint32_t)blocks.size() - grp_off[ngroups];
+        grp_tile[ngroups] = tl;
+        ++ngroups;
+    }
+    n_groups = ngroups;
     n_blocks = (uint32_t)blocks.size();
 
     if ((rc = upload(d_tasks_exact, pt.data(), sizeof(PairTask) * P, "tasks")) ||
@@ -280,6 +333,14 @@ int Plan::build() {
 
     for (hipEvent_t& e : ev)
         if ((rc = ctx->hip(hipEventCreate(&e), "event"))) return rc;
+    if (n_groups > 1) {
+        if ((rc = ctx->hip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "event")))
+            return rc;
+        for (int g = 0; g + 1 < n_groups; ++g)
+            if ((rc = ctx->hip(hipStreamCreateWithFlags(&side[g], hipStreamNonBlocking), "stream")) ||
+                (rc = ctx->hip(hipEventCreateWithFlags(&join_ev[g], hipEventDisableTiming), "event")))
+                return rc;
+    }
     // Capture the two stage graphs.
     hipStream_t s = ctx->stream;
     if ((rc = ctx->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
@@ -324,7 +385,7 @@ void Plan::enqueue_match(hipStream_t s) {
                                   d_stats.as<ViewStats>(), s);
         const LowpTask* lt = d_tasks_lowp.as<LowpTask>();
         launch_distance(s);
-        launch_colmerge(lt, P, o, lowp_block_rows(tile), s);
+        launch_colmerge(lt, P, o, s);
         launch_decide(lt, P, o, desc.ratio, desc.mutual, tbits, s);
         launch_rescore(lt, o, desc.ratio, desc.mutual, tbits, (uint32_t)std::max(64, 4 * ctx->num_cu),
                        s);
@@ -352,8 +413,19 @@ void Plan::launch_distance(hipStream_t s) {
         o.row_j1 = d_row_j1.as<uint32_t>();
         o.row_k2 = d_row_k2.as<float>();
         o.col_part = d_col_part.as<float2>();
-        launch_mfma_rank(tile, fp8, d_tasks_lowp.as<LowpTask>(), d_blocks.as<RowBlock>(), n_blocks,
-                         o, tbits, s);
+        const LowpTask* lt = d_tasks_lowp.as<LowpTask>();
+        const RowBlock* bl = d_blocks.as<RowBlock>();
+        if (n_groups > 1) {  // fork: groups 1.. on side streams, joined back into s
+            (void)hipEventRecord(fork_ev, s);
+            for (int g = 1; g < n_groups; ++g) {
+                (void)hipStreamWaitEvent(side[g - 1], fork_ev, 0);
+                launch_mfma_rank(grp_tile[g], fp8, lt, bl + grp_off[g], grp_n[g], o, tbits,
+                                 side[g - 1]);
+                (void)hipEventRecord(join_ev[g - 1], side[g - 1]);
+            }
+        }
+        if (n_groups > 0) launch_mfma_rank(grp_tile[0], fp8, lt, bl + grp_off[0], grp_n[0], o, tbits, s);
+        for (int g = 1; g < n_groups; ++g) (void)hipStreamWaitEvent(s, join_ev[g - 1], 0);
     } else {
         launch_exact_state(d_tasks_exact.as<PairTask>(), d_blocks.as<RowBlock>(), n_blocks,
                            d_row_j1.as<uint32_t>(), d_row_d1.as<float>(), d_row_d2.as<float>(),
@@ -568,4 +640,117 @@ extern "C" double skh_plan_stage_ms(skh_plan* plan, int stage) {
     if (hipEventSynchronize(b) != hipSuccess) return -1.0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
     return ms;
+}
+
+extern "C" int skh_plan_tiles(skh_plan* plan, uint32_t* pairs_per_tile) {
+    if (!plan || !pairs_per_tile) return SKH_EINVAL;
+    for (int t = 0; t < kNumTiles; ++t) pairs_per_tile[t] = plan->lowp ? plan->pairs_per_tile[t] : 0u;
+    return SKH_OK;
+}
+
+// Representative rows (and columns) of the size buckets (<= 192, <= 384,
+// <= 1024, <= 3072, more).
+static const uint32_t kBucketRep[kSizeBuckets] = {160, 320, 768, 2048, 6000};
+
+extern "C" int skh_ctx_autotune(skh_ctx* ctx, int precision, int32_t* table, double* ms) {
+    if (!ctx || (precision != SKH_PREC_BF16 && precision != SKH_PREC_FP8)) return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return SKH_ENODEV;
+    constexpr uint32_t kViews = 32;  // distinct a views and b views per class
+    const uint64_t max_rows = 2ull * kViews * kBucketRep[kSizeBuckets - 1];
+    float* d_desc = nullptr;
+    int rc = ctx->hip(hipMalloc(&d_desc, 4ull * kDescLen * max_rows), "tuner descriptors");
+    if (rc) return rc;
+    hipLaunchKernelGGL(tune_fill_kernel, dim3((unsigned)max_rows), dim3(128), 0, ctx->stream,
+                       d_desc, max_rows);
+    if ((rc = ctx->hip(hipStreamSynchronize(ctx->stream), "tuner fill"))) {
+        (void)hipFree(d_desc);
+        return rc;
+    }
+    const int saved_override = ctx->tile_override;
+    ctx->tile_override = -1;
+    int32_t tab[kTileClasses];
+    double times[kTileClasses * kNumTiles];
+    for (int c = 0; c < kTileClasses && !rc; ++c) {
+        const uint32_t na = kBucketRep[c / kSizeBuckets], nb = kBucketRep[c % kSizeBuckets];
+        // enough pairs for a few milliseconds of ranking, at least two per XCD
+        const uint64_t want = 1500000000ull / ((uint64_t)na * nb);
+        const uint32_t P = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 64), 4096);
+        std::vector<uint64_t> off(2 * kViews + 1, 0);
+        for (uint32_t v = 0; v < 2 * kViews; ++v) off[v + 1] = off[v] + (v < kViews ? na : nb);
+        std::vector<uint32_t> pl(2 * P);
+        for (uint32_t p = 0; p < P; ++p) {
+            pl[2 * p] = p % kViews;
+            pl[2 * p + 1] = kViews + (p * 7 + p / kViews) % kViews;
+        }
+        double best = 1e30;
+        tab[c] = kTile256x64;
+        for (int tl = 0; tl < kNumTiles && !rc; ++tl) {
+            skh_plan* plan = new skh_plan();
+            plan->ctx = ctx;
+            skh_batch_desc d{};
+            d.n_views = 2 * kViews;
+            d.d_desc = d_desc;
+            d.n_pairs = P;
+            d.ratio = 0.8f;
+            d.mutual = 1;
+            d.precision = precision;
+            d.run_ransac = 0;
+            plan->desc = d;
+            plan->view_off = off;
+            plan->pairs = pl;
+            plan->forced_tile = tl;
+            rc = plan->build();
+            float el = 0.f;
+            if (!rc) rc = ctx->hip(hipGraphLaunch(plan->x_match, ctx->stream), "tuner run");
+            if (!rc) rc = ctx->hip(hipEventRecord(plan->ev[3], ctx->stream), "event");
+            for (int r = 0; r < 3 && !rc; ++r) plan->launch_distance(ctx->stream);
+            if (!rc) rc = ctx->hip(hipEventRecord(plan->ev[4], ctx->stream), "event");
+            if (!rc) rc = ctx->hip(hipEventSynchronize(plan->ev[4]), "sync");
+            if (!rc) rc = ctx->hip(hipEventElapsedTime(&el, plan->ev[3], plan->ev[4]), "elapsed");
+            delete plan;
+            const double t = el / 3.0;
+            times[c * kNumTiles + tl] = t;
+            if (!rc && t < best) {
+                best = t;
+                tab[c] = tl;
+            }
+        }
+    }
+    ctx->tile_override = saved_override;
+    (void)hipFree(d_desc);
+    if (rc) return rc;
+    const int pi = precision == SKH_PREC_FP8 ? 1 : 0;
+    std::memcpy(ctx->tile_table[pi], tab, sizeof tab);
+    ctx->tile_tuned[pi] = true;
+    if (table) std::memcpy(table, tab, sizeof tab);
+    if (ms) std::memcpy(ms, times, sizeof times);
+    return SKH_OK;
+}
+
+extern "C" int skh_ctx_set_tile_table(skh_ctx* ctx, int precision, const int32_t* table) {
+    if (!ctx || (precision != SKH_PREC_BF16 && precision != SKH_PREC_FP8)) return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int pi = precision == SKH_PREC_FP8 ? 1 : 0;
+    if (!table) {
+        ctx->tile_tuned[pi] = false;
+        return SKH_OK;
+    }
+    for (int c = 0; c < kTileClasses; ++c)
+        if (table[c] < 0 || table[c] >= kNumTiles)
+            return ctx->fail(SKH_EINVAL, "skh_ctx_set_tile_table: entry %d names no tile", c);
+    std::memcpy(ctx->tile_table[pi], table, sizeof(int32_t) * kTileClasses);
+    ctx->tile_tuned[pi] = true;
+    return SKH_OK;
+}
+
+extern "C" int skh_ctx_tile_table(skh_ctx* ctx, int precision, int32_t* table, int* tuned) {
+    if (!ctx || !table || (precision != SKH_PREC_BF16 && precision != SKH_PREC_FP8))
+        return SKH_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int pi = precision == SKH_PREC_FP8 ? 1 : 0;
+    for (int c = 0; c < kTileClasses; ++c)
+        table[c] = ctx->pick_tile(precision, kBucketRep[c / kSizeBuckets], kBucketRep[c % kSizeBuckets]);
+    if (tuned) *tuned = ctx->tile_tuned[pi] ? 1 : 0;
+    return SKH_OK;
 }

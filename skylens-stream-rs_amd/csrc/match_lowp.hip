@@ -655,9 +655,9 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
 }
 
 // Per column of every pair: merge the workgroup partials.
-__global__ void colmerge_kernel(const LowpTask* __restrict__ tasks, LowpOut out, uint32_t bm) {
+__global__ void colmerge_kernel(const LowpTask* __restrict__ tasks, LowpOut out) {
     const LowpTask t = tasks[blockIdx.x];  // pairs on x: gridDim.y is capped at 65535
-    const uint32_t nparts = (t.na + bm - 1) / bm;
+    const uint32_t nparts = (t.na + t.bm - 1) / t.bm;
     for (uint32_t col = blockIdx.y * blockDim.x + threadIdx.x; col < t.nb;
          col += gridDim.y * blockDim.x) {
         float a1 = INFINITY, a2 = INFINITY;
@@ -1144,10 +1144,9 @@ void launch_pair_eps(const LowpTask* tasks, uint32_t n_pairs, uint32_t tbits, do
                        n_pairs, tbits, eps);
 }
 
-void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
-                     hipStream_t s) {
+void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, hipStream_t s) {
     if (!n_pairs) return;
-    hipLaunchKernelGGL(colmerge_kernel, dim3(n_pairs, 8), dim3(256), 0, s, tasks, out, bm);
+    hipLaunchKernelGGL(colmerge_kernel, dim3(n_pairs, 8), dim3(256), 0, s, tasks, out);
 }
 
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,

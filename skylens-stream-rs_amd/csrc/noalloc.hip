@@ -139,7 +139,7 @@ extern "C" int skh_ratio_match_dev_noalloc(skh_ctx* ctx, const float* d_a, size_
         launch_fill_u32(d_out_len, 1, 0u, ctx->stream);  // matching.rs:206: no second neighbour
     } else {
         enqueue_lowp_single(B, d_a, a_rows, d_b, b_rows, ratio, mutual != 0,
-                            precision == SKH_PREC_FP8, lowp_tile(ctx->tile_override, na),
+                            precision == SKH_PREC_FP8, ctx->pick_tile(precision, na, nb),
                             ctx->num_cu, d_out_ij, d_out_len, ctx->stream);
     }
     return ctx->hip(hipGetLastError(), "skh_ratio_match_dev_noalloc: launch");

@@ -32,8 +32,11 @@ void launch_single_setup(const LowpTask* lt, const PairTask& pt, uint32_t nblk, 
                        lt ? *lt : LowpTask{}, pt, nblk, bm, lt ? d_lt : nullptr, d_pt, d_blocks);
 }
 
-int lowp_tile(int tile_override, size_t na) {
-    return tile_override >= 0 ? tile_override : (na <= 128 ? kTile128x64 : kTile256x64);
+int Ctx::pick_tile(int precision, size_t na, size_t nb) const {
+    if (tile_override >= 0) return tile_override;
+    const int p = precision == SKH_PREC_FP8 ? 1 : 0;
+    if (tile_tuned[p]) return tile_table[p][tile_class((uint32_t)na, (uint32_t)nb)];
+    return na <= 128 ? kTile128x64 : kTile256x64;
 }
 
 // The whole low-precision single-pair pass on the buffers of `B`: set-up,
@@ -65,6 +68,7 @@ void enqueue_lowp_single(const SingleBufs& B, const float* d_a, uint32_t na, con
     lt.sb = B.stats + 1;
     lt.na = na;
     lt.nb = nb;
+    lt.bm = bm;
     const PairTask pt{d_a, d_b, na, nb, 0u, 0u};
     launch_single_setup(&lt, pt, nblk, bm, B.ltask, B.task, B.blocks, s);
 
@@ -89,7 +93,7 @@ void enqueue_lowp_single(const SingleBufs& B, const float* d_a, uint32_t na, con
         launch_quantize(d_b, nb, B.q_b, B.nrm_b, B.stats + 1, s);
     }
     launch_mfma_rank(tile, fp8, B.ltask, B.blocks, nblk, o, tbits, s);
-    launch_colmerge(B.ltask, 1, o, bm, s);
+    launch_colmerge(B.ltask, 1, o, s);
     launch_decide(B.ltask, 1, o, ratio, mutual, tbits, s);
     launch_rescore(B.ltask, o, ratio, mutual, tbits, (uint32_t)std::max(64, 4 * num_cu), s);
     launch_compact(B.task, 1, o.row_match, B.out_base, d_out_ij, d_out_len, s);
@@ -129,7 +133,7 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     if (nb > kLowpMaxCols)
         return ratio_match_dev(d_a, na, d_b, nb, ratio, mutual, SKH_PREC_FP32, d_out_ij, d_out_len);
 
-    const int tile = lowp_tile(tile_override, na);
+    const int tile = pick_tile(precision, na, nb);
     const uint32_t bm = lowp_block_rows(tile);
     const uint32_t nblk = (uint32_t)((na + bm - 1) / bm);
     if (fp8) {

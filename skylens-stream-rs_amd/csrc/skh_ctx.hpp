@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "skh_internal.hpp"
+#include "skh_lowp.hpp"
 
 namespace skh {
 
@@ -26,6 +27,9 @@ struct Ctx {
     int device = 0;
     int num_cu = 0;
     int tile_override = -1;  // SKH_TILE_AUTO
+    // Tuned tile per size class (tile_class), for BF16 [0] and FP8 [1] ranking.
+    int tile_table[2][kTileClasses] = {};
+    bool tile_tuned[2] = {false, false};
     uint64_t last_iters = 0;  // hypotheses drawn by the last single-pair RANSAC
     void* noalloc_scratch = nullptr;  // device address of the allocation-free entry's scratch
     hipStream_t stream = nullptr;
@@ -58,6 +62,9 @@ struct Ctx {
     int hip(hipError_t e, const char* what);
     int ensure(DevBuf& b, size_t bytes, const char* what);
 
+    // Ranking tile of a pair: the fixed override, the tuned table, or the default
+    // (256 x 64; 128 x 64 for at most 128 rows).
+    int pick_tile(int precision, size_t na, size_t nb) const;
     int exact_state_single(const float* d_a, size_t na, const float* d_b, size_t nb);
     int ratio_match_dev(const float* d_a, size_t na, const float* d_b, size_t nb, float ratio,
                         int mutual, int precision, uint32_t* d_out_ij, uint32_t* d_out_len);

@@ -40,7 +40,7 @@ struct LowpTask {
     uint32_t col_base;         // per-column arrays (col_top2)
     uint64_t part_base;        // column partials: ceil(na / BM) * nb float2
     uint32_t fp8;              // 1: e4m3 ranking (error bound of the FP8 form)
-    uint32_t pad;
+    uint32_t bm;               // rows per row block of this pair's tile
 };
 
 struct LowpOut {
@@ -82,8 +82,16 @@ void launch_mfma_rank(int tile, bool fp8, const LowpTask* tasks, const RowBlock*
 // Per-pair error bound eps (>= |2 key - D_ref| over every element of the pair).
 void launch_pair_eps(const LowpTask* tasks, uint32_t n_pairs, uint32_t tbits, double* eps,
                      hipStream_t s);
-void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, uint32_t bm,
-                     hipStream_t s);
+void launch_colmerge(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, hipStream_t s);
+
+// Size classes of the tile table: rows and columns of a pair each fall in one
+// of kSizeBuckets buckets (<= 192, <= 384, <= 1024, <= 3072, more).
+constexpr int kSizeBuckets = 5;
+constexpr int kTileClasses = kSizeBuckets * kSizeBuckets;
+inline int size_bucket(uint32_t n) {
+    return n <= 192 ? 0 : n <= 384 ? 1 : n <= 1024 ? 2 : n <= 3072 ? 3 : 4;
+}
+inline int tile_class(uint32_t na, uint32_t nb) { return kSizeBuckets * size_bucket(na) + size_bucket(nb); }
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
                    int mutual, uint32_t tbits, hipStream_t s);
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,

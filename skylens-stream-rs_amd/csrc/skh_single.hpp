@@ -38,7 +38,6 @@ struct SingleBufs {
     uint64_t* out_base;
 };
 
-int lowp_tile(int tile_override, size_t na);
 void launch_single_setup(const LowpTask* lt, const PairTask& pt, uint32_t nblk, uint32_t bm,
                          LowpTask* d_lt, PairTask* d_pt, RowBlock* d_blocks, hipStream_t s);
 void enqueue_lowp_single(const SingleBufs& B, const float* d_a, uint32_t na, const float* d_b,

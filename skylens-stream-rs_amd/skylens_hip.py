@@ -83,7 +83,14 @@ EXPORTS = [
     ("skh_plan_time_distance", ctypes.c_double, [_vp, ctypes.c_int]),
     ("skh_plan_ransac_profile", ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     ("skh_plan_lowp_state", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("skh_ctx_autotune", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    ("skh_ctx_set_tile_table", ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    ("skh_ctx_tile_table", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int)]),
+    ("skh_plan_tiles", ctypes.c_int, [_vp, _vp]),
 ]
+
+TILE_CLASSES = 25
+TILE_NAMES = ["128x64", "128x128", "256x64", "256x128"]
 
 RANSAC_PHASES = ["sampling", "hypotheses", "local_opt", "replay", "pool_lm", "final_gates",
                  "line_test", "model_selection", "batches", "lo_lanes", "total"]
@@ -218,6 +225,29 @@ class Context:
     def set_tile(self, tile: int):
         """MFMA ranking tile (TILES index; -1 = automatic choice)."""
         self._check(self.lib.skh_ctx_set_tile(self.handle, int(tile)), "skh_ctx_set_tile")
+
+    def autotune(self, precision: str = "bf16"):
+        """Times the four ranking tiles per size class and loads the fastest:
+        (table (25,) tile indices, ms (25, 4) per-launch times)."""
+        table = np.zeros(TILE_CLASSES, np.int32)
+        ms = np.zeros((TILE_CLASSES, 4), np.float64)
+        self._check(self.lib.skh_ctx_autotune(self.handle, PRECISIONS[precision], _ptr(table),
+                                              _ptr(ms)), "skh_ctx_autotune")
+        return table, ms
+
+    def set_tile_table(self, precision: str, table):
+        t = None if table is None else np.ascontiguousarray(table, np.int32)
+        self._check(self.lib.skh_ctx_set_tile_table(self.handle, PRECISIONS[precision],
+                                                    None if t is None else _ptr(t)),
+                    "skh_ctx_set_tile_table")
+
+    def tile_table(self, precision: str = "bf16"):
+        """(table (25,), tuned flag)."""
+        table = np.zeros(TILE_CLASSES, np.int32)
+        tuned = ctypes.c_int(0)
+        self._check(self.lib.skh_ctx_tile_table(self.handle, PRECISIONS[precision], _ptr(table),
+                                                ctypes.byref(tuned)), "skh_ctx_tile_table")
+        return table, bool(tuned.value)
 
     @property
     def stream(self) -> int:
@@ -395,6 +425,12 @@ class Plan:
         self.ctx._check(self.lib.skh_plan_ransac_profile(self.handle, _ptr(it), _ptr(ph),
                                                          ctypes.byref(tpm)), "ransac_profile")
         return it[: self.n_pairs], ph[: self.n_pairs], tpm.value
+
+    def tiles(self) -> dict:
+        """Pairs ranked with each tile ({"256x64": count, ...})."""
+        n = np.zeros(4, np.uint32)
+        self.ctx._check(self.lib.skh_plan_tiles(self.handle, _ptr(n)), "skh_plan_tiles")
+        return {TILE_NAMES[k]: int(n[k]) for k in range(4) if n[k]}
 
     def lowp_state(self) -> dict:
         """The BF16/FP8 ranking state of the last run (skh_plan_lowp_state):
