@@ -91,6 +91,38 @@ const char* skh_ctx_last_error(skh_ctx* ctx);
 int skh_candidate_pairs(const uint64_t* views, size_t n, uint64_t temporal, uint64_t cross,
                         uint64_t pow2_max, uint64_t* out_ij, size_t cap, size_t* out_len);
 
+/* PairSchedule (matching.rs:73-99) with its CrossSchedule (:47-71):
+ * cross_kind SKH_CROSS_SPEC pairs different cameras at position differences
+ * 0..=range; SKH_CROSS_FORMATION pairs the front camera (0) at position p
+ * with the right (1) at p + right_min ..= p + max and the left (2) at
+ * p + left_min ..= p + max, every step-th position. */
+#define SKH_CROSS_SPEC 0
+#define SKH_CROSS_FORMATION 1
+typedef struct {
+    uint64_t temporal, pow2_max;
+    int32_t cross_kind;
+    int32_t pad;
+    uint64_t range;                          /* SKH_CROSS_SPEC */
+    uint64_t right_min, left_min, max, step; /* SKH_CROSS_FORMATION */
+} skh_pair_schedule;
+/* PairSchedule::default() (matching.rs:81-89): temporal 5, pow2_max 16,
+ * CrossSchedule::FORMATION (20, 20, 40, 4). */
+void skh_pair_schedule_default(skh_pair_schedule* sch);
+/* PairSchedule::spec() (matching.rs:91-98): cross Spec { range: 4 }. */
+void skh_pair_schedule_spec(skh_pair_schedule* sch);
+/* scheduled_pairs (matching.rs:103-142), the production pair rule
+ * (pipeline.rs:486, sparse.rs:311): views as for skh_candidate_pairs, pairs
+ * (i, j) with i < j, sorted, no duplicates; same capacity contract. */
+int skh_scheduled_pairs(const uint64_t* views, size_t n, const skh_pair_schedule* sch,
+                        uint64_t* out_ij, size_t cap, size_t* out_len);
+
+/* verified_matches (tracks.rs:56-69): the (i, j) of `matches` (2 * n_matches
+ * uint32) whose inlier flag is set, in order, into out_ij; *out_len = their
+ * count. A length mismatch (n_matches != n_inliers) is the reference's
+ * Err((n_matches, n_inliers)): SKH_EINVAL, *out_len untouched. Host-only. */
+int skh_verified_matches(const uint32_t* matches, size_t n_matches, const uint8_t* inliers,
+                         size_t n_inliers, uint32_t* out_ij, size_t* out_len);
+
 /* ratio_match (matching.rs:167-215) on host buffers.
  * a: na x 128 f32 row-major, b: nb x 128 f32 (Feature.desc gathered
  * contiguously). out_ij: 2*cap uint32 (i, j), ascending in i; *out_len = count.
@@ -235,6 +267,14 @@ double skh_plan_time_distance(skh_plan* plan, int reps);
  * FP32 re-score. Any pointer may be NULL. Synchronises. */
 int skh_plan_lowp_state(skh_plan* plan, float* row_k1, uint32_t* row_j1, float* row_k2,
                         float* col_top2, double* eps, uint64_t* queued);
+/* Verified matches of every pair of the last run, the consumer's input
+ * (tracks.rs:56-69 applied per pair; the pipeline keeps a pair only when
+ * RANSAC found a model): verified_len[p] = the inlier count when found[p] = 1,
+ * else 0; verified_ij holds pair p's verified (i, j) at
+ * [match_base[p], match_base[p] + verified_len[p]) (2 uint32 each), in match
+ * order. Computed on the device by the RANSAC graph; either pointer may be
+ * NULL. Synchronises. */
+int skh_plan_verified(skh_plan* plan, uint32_t* verified_len, uint32_t* verified_ij);
 /* Pairs of the plan ranked with each tile (4 entries, SKH_TILE_* order; all
  * zero for an FP32 plan). */
 int skh_plan_tiles(skh_plan* plan, uint32_t* pairs_per_tile);

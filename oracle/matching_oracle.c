@@ -2,10 +2,11 @@
  * CPU oracle, descriptor-matching half. TEST INFRASTRUCTURE ONLY (see
  * oracle.h): the checker for the HIP path, never the product.
  *
- * Restates crates/core/src/matching.rs:14-40 (candidate_pairs),
- * :144-215 (sq_dist, ratio_match) and the test helpers :1141-1154 (Lcg),
- * :1580-1656 (feature, ratio_match_reference, match_fixture),
- * :1867-1878 (pairs_hash), :2345-2357 (sq_dist_spec).
+ * Restates crates/core/src/matching.rs:14-40 (candidate_pairs), :41-142
+ * (PairSchedule, scheduled_pairs), :144-215 (sq_dist, ratio_match) and the
+ * test helpers :1141-1154 (Lcg), :1580-1656 (feature, ratio_match_reference,
+ * match_fixture), :1867-1878 (pairs_hash), :2345-2357 (sq_dist_spec); and
+ * crates/core/src/tracks.rs:56-69 (verified_matches).
  */
 #include "oracle.h"
 
@@ -237,4 +238,85 @@ size_t orc_candidate_pairs(const uint64_t* views, size_t n, uint64_t temporal, u
         }
     }
     return cnt;
+}
+
+/* matching.rs:41-44: camera numbers (synth::CamId::ALL order). */
+#define ORC_CAM_FRONT 0
+#define ORC_CAM_RIGHT 1
+#define ORC_CAM_LEFT 2
+
+static int cmp_pair(const void* x, const void* y) {
+    const uint64_t* a = (const uint64_t*)x;
+    const uint64_t* b = (const uint64_t*)y;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+    return 0;
+}
+
+/* matching.rs:103-142: every ordered (i, j), i != j, tested by the schedule;
+ * accepted ones pushed as (min, max); then sort_unstable and dedup. */
+size_t orc_scheduled_pairs(const uint64_t* views, size_t n, const orc_pair_schedule* sch,
+                           uint64_t* out, size_t cap) {
+    size_t m = 0, mcap = 64;
+    uint64_t* v = (uint64_t*)malloc(sizeof(uint64_t) * 2 * mcap);
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < n; ++j) {
+            if (i == j) continue;
+            uint64_t ca = views[2 * i], pa = views[2 * i + 1];
+            uint64_t cb = views[2 * j], pb = views[2 * j + 1];
+            uint64_t d = pa > pb ? pa - pb : pb - pa;
+            int ok;
+            if (ca == cb) {
+                int pow2 = d != 0 && (d & (d - 1)) == 0; /* usize::is_power_of_two */
+                ok = j > i && d >= 1 && (d <= sch->temporal || (pow2 && d <= sch->pow2_max));
+            } else if (sch->cross_kind == 0) {
+                ok = j > i && d <= sch->range;
+            } else {
+                int has_lo = 0;
+                uint64_t lo = 0;
+                if (ca == ORC_CAM_FRONT && cb == ORC_CAM_RIGHT) { has_lo = 1; lo = sch->right_min; }
+                if (ca == ORC_CAM_FRONT && cb == ORC_CAM_LEFT) { has_lo = 1; lo = sch->left_min; }
+                uint64_t step = sch->step > 1 ? sch->step : 1; /* step.max(1) */
+                ok = has_lo && pb >= pa + lo && pb <= pa + sch->max && (pb - pa - lo) % step == 0;
+            }
+            if (ok) {
+                if (m == mcap) {
+                    mcap *= 2;
+                    v = (uint64_t*)realloc(v, sizeof(uint64_t) * 2 * mcap);
+                }
+                v[2 * m] = i < j ? i : j;
+                v[2 * m + 1] = i < j ? j : i;
+                m++;
+            }
+        }
+    }
+    qsort(v, m, sizeof(uint64_t) * 2, cmp_pair);
+    size_t k = 0;
+    for (size_t e = 0; e < m; ++e) {
+        if (k > 0 && v[2 * (k - 1)] == v[2 * e] && v[2 * (k - 1) + 1] == v[2 * e + 1]) continue;
+        v[2 * k] = v[2 * e];
+        v[2 * k + 1] = v[2 * e + 1];
+        k++;
+    }
+    if (out)
+        for (size_t e = 0; e < k && e < cap; ++e) {
+            out[2 * e] = v[2 * e];
+            out[2 * e + 1] = v[2 * e + 1];
+        }
+    free(v);
+    return k;
+}
+
+/* tracks.rs:56-69. */
+long orc_verified_matches(const uint64_t* ij, size_t n_matches, const uint8_t* inliers,
+                          size_t n_inliers, uint64_t* out) {
+    if (n_matches != n_inliers) return -1;
+    long k = 0;
+    for (size_t e = 0; e < n_matches; ++e) {
+        if (!inliers[e]) continue;
+        out[2 * k] = ij[2 * e];
+        out[2 * k + 1] = ij[2 * e + 1];
+        k++;
+    }
+    return k;
 }

@@ -58,6 +58,23 @@ uint64_t orc_pairs_hash(const uint64_t* ij, size_t n);
 size_t orc_candidate_pairs(const uint64_t* views, size_t n, uint64_t temporal, uint64_t cross,
                            uint64_t pow2_max, uint64_t* out, size_t cap);
 
+/* PairSchedule / CrossSchedule (matching.rs:47-99). cross_kind 0 = Spec
+ * { range }, 1 = Formation { right_min, left_min, max, step }. */
+typedef struct {
+    uint64_t temporal, pow2_max;
+    int32_t cross_kind, pad;
+    uint64_t range;
+    uint64_t right_min, left_min, max, step;
+} orc_pair_schedule;
+/* scheduled_pairs (matching.rs:103-142): returns the pair count, writes up to
+ * cap pairs (2 uint64 each) when out != NULL. */
+size_t orc_scheduled_pairs(const uint64_t* views, size_t n, const orc_pair_schedule* sch,
+                           uint64_t* out, size_t cap);
+/* verified_matches (tracks.rs:56-69): 0 and the kept (i, j) in order, or -1
+ * on a length mismatch. */
+long orc_verified_matches(const uint64_t* ij, size_t n_matches, const uint8_t* inliers,
+                          size_t n_inliers, uint64_t* out);
+
 /* ---- fundamental-matrix RANSAC (matching.rs:217-1104) ---- */
 typedef struct {
     double threshold_px;

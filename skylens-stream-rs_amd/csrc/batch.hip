@@ -44,6 +44,40 @@ __global__ void handoff_kernel(const uint32_t* __restrict__ match_ij,
     }
 }
 
+// tracks.rs:56-69 per pair, on the device: the matches whose inlier flag is
+// set, in match order, for pairs where RANSAC found a model (a pair without a
+// model is dropped, as the pipeline's consumer does). One workgroup per pair,
+// ballot compaction.
+__global__ __launch_bounds__(256) void verify_compact_kernel(
+    const uint32_t* __restrict__ match_ij, const uint32_t* __restrict__ match_len,
+    const uint64_t* __restrict__ match_base, const uint8_t* __restrict__ inliers,
+    const int32_t* __restrict__ found, uint32_t* __restrict__ ver_ij,
+    uint32_t* __restrict__ ver_len) {
+    __shared__ uint32_t wave_cnt[4];
+    const uint32_t p = blockIdx.x;
+    const uint64_t base = match_base[p];
+    const uint32_t len = found[p] == 1 ? match_len[p] : 0u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t total = 0;
+    for (uint32_t b0 = 0; b0 < len; b0 += 256) {
+        const uint32_t k = b0 + threadIdx.x;
+        const bool keep = k < len && inliers[base + k] != 0;
+        const unsigned long long m = __ballot(keep);
+        const uint32_t before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[wave] = __popcll(m);
+        __syncthreads();
+        uint32_t off = total;
+        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+        if (keep) {
+            ver_ij[2 * (base + off + before)] = match_ij[2 * (base + k)];
+            ver_ij[2 * (base + off + before) + 1] = match_ij[2 * (base + k) + 1];
+        }
+        total += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ver_len[p] = total;
+}
+
 // Pseudo-random unit rows for the tile tuner: row r, component k from a hash
 // of (r, k), then scaled to unit length (one 128-thread block per row).
 __global__ void tune_fill_kernel(float* __restrict__ d, uint64_t rows) {
@@ -93,7 +127,7 @@ struct Plan {
     DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
         d_kp_b;
     DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters, d_prof, d_order,
-        d_est, d_rqueue;
+        d_est, d_rqueue, d_ver_ij, d_ver_len;
 
     hipGraph_t g_match = nullptr, g_ransac = nullptr;
     hipGraphExec_t x_match = nullptr, x_ransac = nullptr;
@@ -110,7 +144,7 @@ struct Plan {
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
                 &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue,
-                &d_q8,       &d_meta};
+                &d_q8,       &d_meta,        &d_ver_ij,     &d_ver_len};
     }
 
     ~Plan() {
@@ -312,7 +346,9 @@ int Plan::build() {
             (rc = alloc(d_iters, 8ull * P, "iters")) ||
             (rc = alloc(d_prof, 8ull * 11 * P, "prof")) ||
             (rc = alloc(d_order, 4ull * P, "order")) || (rc = alloc(d_est, 4ull * P, "est")) ||
-            (rc = alloc(d_rqueue, 8, "rqueue")))
+            (rc = alloc(d_rqueue, 8, "rqueue")) ||
+            (rc = alloc(d_ver_ij, 8ull * cap_total, "verified_ij")) ||
+            (rc = alloc(d_ver_len, 4ull * P, "verified_len")))
             return rc;
         for (uint32_t p = 0; p < P; ++p) {
             RansacTask& t = rt[p];
@@ -443,6 +479,10 @@ void Plan::enqueue_ransac(hipStream_t s) {
                        d_x1.as<double2>(), d_x2.as<double2>());
     launch_ransac(d_rtasks.as<RansacTask>(), P, desc.ransac, s, d_rqueue.as<uint32_t>(),
                   d_est.as<uint32_t>(), d_order.as<uint32_t>());
+    hipLaunchKernelGGL(verify_compact_kernel, dim3(P), dim3(256), 0, s, d_match_ij.as<uint32_t>(),
+                       d_match_len.as<uint32_t>(), d_match_base.as<uint64_t>(),
+                       d_inl.as<uint8_t>(), d_found.as<int32_t>(), d_ver_ij.as<uint32_t>(),
+                       d_ver_len.as<uint32_t>());
 }
 
 }  // namespace skh
@@ -640,6 +680,26 @@ extern "C" double skh_plan_stage_ms(skh_plan* plan, int stage) {
     if (hipEventSynchronize(b) != hipSuccess) return -1.0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
     return ms;
+}
+
+extern "C" int skh_plan_verified(skh_plan* plan, uint32_t* verified_len, uint32_t* verified_ij) {
+    if (!plan) return SKH_EINVAL;
+    Ctx* ctx = plan->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!plan->d_ver_len.ptr) return ctx->fail(SKH_EINVAL, "skh_plan_verified: plan runs no RANSAC");
+    int rc;
+    if ((rc = ctx->hip(hipStreamSynchronize(ctx->stream), "sync"))) return rc;
+    const uint32_t P = plan->desc.n_pairs;
+    if (verified_len && P &&
+        (rc = ctx->hip(hipMemcpy(verified_len, plan->d_ver_len.ptr, 4ull * P, hipMemcpyDeviceToHost),
+                       "verified_len")))
+        return rc;
+    if (verified_ij && plan->cap_total &&
+        (rc = ctx->hip(hipMemcpy(verified_ij, plan->d_ver_ij.ptr, 8ull * plan->cap_total,
+                                 hipMemcpyDeviceToHost),
+                       "verified_ij")))
+        return rc;
+    return SKH_OK;
 }
 
 extern "C" int skh_plan_tiles(skh_plan* plan, uint32_t* pairs_per_tile) {

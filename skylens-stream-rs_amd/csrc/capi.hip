@@ -194,6 +194,82 @@ extern "C" int skh_candidate_pairs(const uint64_t* views, size_t n, uint64_t tem
     return cnt > cap ? SKH_EINVAL : SKH_OK;
 }
 
+extern "C" void skh_pair_schedule_default(skh_pair_schedule* sch) {
+    if (!sch) return;
+    *sch = skh_pair_schedule{};
+    sch->temporal = 5;  // matching.rs:7-12 PAIR_TEMPORAL, PAIR_POW2_MAX
+    sch->pow2_max = 16;
+    sch->cross_kind = SKH_CROSS_FORMATION;  // CrossSchedule::FORMATION, :62-71
+    sch->right_min = 20;
+    sch->left_min = 20;
+    sch->max = 40;
+    sch->step = 4;
+}
+
+extern "C" void skh_pair_schedule_spec(skh_pair_schedule* sch) {
+    if (!sch) return;
+    skh_pair_schedule_default(sch);
+    sch->cross_kind = SKH_CROSS_SPEC;
+    sch->range = 4;  // PAIR_CROSS
+}
+
+extern "C" int skh_scheduled_pairs(const uint64_t* views, size_t n, const skh_pair_schedule* sch,
+                                   uint64_t* out_ij, size_t cap, size_t* out_len) {
+    if ((!views && n) || !sch || !out_len ||
+        (sch->cross_kind != SKH_CROSS_SPEC && sch->cross_kind != SKH_CROSS_FORMATION))
+        return SKH_EINVAL;
+    // The reference tests every ordered (i, j) and keeps (min, max), sorted and
+    // deduplicated: an unordered pair {i < j} is in iff either direction
+    // passes, and walking i < j in order yields the sorted list directly.
+    auto pass = [&](size_t i, size_t j) {
+        const uint64_t ca = views[2 * i], pa = views[2 * i + 1];
+        const uint64_t cb = views[2 * j], pb = views[2 * j + 1];
+        const uint64_t d = pa > pb ? pa - pb : pb - pa;
+        if (ca == cb) {
+            const bool pow2 = d != 0 && (d & (d - 1)) == 0;
+            return j > i && d >= 1 && (d <= sch->temporal || (pow2 && d <= sch->pow2_max));
+        }
+        if (sch->cross_kind == SKH_CROSS_SPEC) return j > i && d <= sch->range;
+        uint64_t lo;
+        if (ca == 0 && cb == 1)  // CAM_FRONT -> CAM_RIGHT (matching.rs:41-44)
+            lo = sch->right_min;
+        else if (ca == 0 && cb == 2)  // CAM_FRONT -> CAM_LEFT
+            lo = sch->left_min;
+        else
+            return false;
+        const uint64_t step = sch->step > 1 ? sch->step : 1;
+        return pb >= pa + lo && pb <= pa + sch->max && (pb - pa - lo) % step == 0;
+    };
+    size_t cnt = 0;
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = i + 1; j < n; ++j)
+            if (pass(i, j) || pass(j, i)) {
+                if (out_ij && cnt < cap) {
+                    out_ij[2 * cnt] = i;
+                    out_ij[2 * cnt + 1] = j;
+                }
+                ++cnt;
+            }
+    *out_len = cnt;
+    return cnt > cap ? SKH_EINVAL : SKH_OK;
+}
+
+extern "C" int skh_verified_matches(const uint32_t* matches, size_t n_matches,
+                                    const uint8_t* inliers, size_t n_inliers, uint32_t* out_ij,
+                                    size_t* out_len) {
+    if (!out_len || (n_matches && (!matches || !inliers || !out_ij))) return SKH_EINVAL;
+    if (n_matches != n_inliers) return SKH_EINVAL;  // Err((matches.len(), inliers.len()))
+    size_t k = 0;
+    for (size_t e = 0; e < n_matches; ++e)
+        if (inliers[e]) {
+            out_ij[2 * k] = matches[2 * e];
+            out_ij[2 * k + 1] = matches[2 * e + 1];
+            ++k;
+        }
+    *out_len = k;
+    return SKH_OK;
+}
+
 static bool valid_precision(int p) { return skh_precision_supported(p) != 0; }
 
 extern "C" int skh_ratio_match_dev(skh_ctx* ctx, const float* d_a, size_t na, const float* d_b,

@@ -87,6 +87,11 @@ EXPORTS = [
     ("skh_ctx_set_tile_table", ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     ("skh_ctx_tile_table", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int)]),
     ("skh_plan_tiles", ctypes.c_int, [_vp, _vp]),
+    ("skh_pair_schedule_default", None, [_vp]),
+    ("skh_pair_schedule_spec", None, [_vp]),
+    ("skh_scheduled_pairs", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, ctypes.c_size_t, _c_size_p]),
+    ("skh_verified_matches", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _c_size_p]),
+    ("skh_plan_verified", ctypes.c_int, [_vp, _vp, _vp]),
 ]
 
 TILE_CLASSES = 25
@@ -110,6 +115,39 @@ class RansacConfig(ctypes.Structure):
         for k, v in kw.items():
             setattr(c, k, v)
         return c
+
+
+class PairSchedule(ctypes.Structure):
+    """matching.rs:47-99 (skh_pair_schedule): cross_kind 0 = Spec { range },
+    1 = Formation { right_min, left_min, max, step }."""
+    _fields_ = [("temporal", ctypes.c_uint64), ("pow2_max", ctypes.c_uint64),
+                ("cross_kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("range", ctypes.c_uint64),
+                ("right_min", ctypes.c_uint64), ("left_min", ctypes.c_uint64),
+                ("max", ctypes.c_uint64), ("step", ctypes.c_uint64)]
+
+    @classmethod
+    def default(cls, **kw) -> "PairSchedule":
+        """PairSchedule::default() (matching.rs:81-89)."""
+        c = cls()
+        load_library().skh_pair_schedule_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    @classmethod
+    def spec(cls) -> "PairSchedule":
+        """PairSchedule::spec() (matching.rs:91-98)."""
+        c = cls()
+        load_library().skh_pair_schedule_spec(ctypes.byref(c))
+        return c
+
+
+class LengthMismatch(ValueError):
+    """verified_matches' Err((matches.len(), inliers.len())) (tracks.rs:60-62)."""
+
+    def __init__(self, n_matches: int, n_inliers: int):
+        super().__init__((n_matches, n_inliers))
+        self.lengths = (n_matches, n_inliers)
 
 
 class ModelSelectionC(ctypes.Structure):
@@ -203,10 +241,11 @@ class Context:
             self.lib.skh_ctx_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # At interpreter shutdown the HIP runtime may already be gone; the
-        # process exit releases everything.
-        if sys.is_finalizing():
+        # process exit releases everything. (The default argument keeps the
+        # function reachable after module globals are cleared.)
+        if _finalizing():
             return
         try:
             self.close()
@@ -426,6 +465,18 @@ class Plan:
                                                          ctypes.byref(tpm)), "ransac_profile")
         return it[: self.n_pairs], ph[: self.n_pairs], tpm.value
 
+    def verified(self):
+        """(verified_len (P,), verified_ij (cap, 2)) of the last run: each pair's
+        inlier matches, in match order, at its match_base offset
+        (skh_plan_verified; tracks.rs:56-69 per pair)."""
+        P = self.n_pairs
+        cap = int(self.match_base[-1])
+        vl = np.zeros(max(P, 1), np.uint32)
+        vij = np.zeros((max(cap, 1), 2), np.uint32)
+        self.ctx._check(self.lib.skh_plan_verified(self.handle, _ptr(vl), _ptr(vij)),
+                        "skh_plan_verified")
+        return vl[:P], vij
+
     def tiles(self) -> dict:
         """Pairs ranked with each tile ({"256x64": count, ...})."""
         n = np.zeros(4, np.uint32)
@@ -487,8 +538,8 @@ class Plan:
                 self.lib.skh_plan_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
             return
         try:
             self.close()
@@ -539,6 +590,42 @@ def candidate_pairs(views: Sequence[Tuple[int, int]], temporal: int, cross: int,
                                  min(pow2_max, mx), _ptr(out), n.value, ctypes.byref(n))
     if rc != SKH_OK:
         raise SkhError(f"skh_candidate_pairs: {rc}")
+    return [(int(i), int(j)) for i, j in out[: n.value]]
+
+
+def scheduled_pairs(views: Sequence[Tuple[int, int]],
+                    sch: Optional[PairSchedule] = None) -> List[Tuple[int, int]]:
+    """matching.rs:103-142: the production pair rule; PairSchedule::default() if
+    `sch` is None."""
+    lib = load_library()
+    sch = sch or PairSchedule.default()
+    v = np.ascontiguousarray(np.asarray(views, dtype=np.uint64).reshape(-1, 2))
+    n = ctypes.c_size_t(0)
+    rc = lib.skh_scheduled_pairs(_ptr(v), v.shape[0], ctypes.byref(sch), None, 0, ctypes.byref(n))
+    if rc not in (SKH_OK, SKH_EINVAL):
+        raise SkhError(f"skh_scheduled_pairs: {rc}")
+    out = np.zeros((max(1, n.value), 2), np.uint64)
+    rc = lib.skh_scheduled_pairs(_ptr(v), v.shape[0], ctypes.byref(sch), _ptr(out), n.value,
+                                 ctypes.byref(n))
+    if rc != SKH_OK:
+        raise SkhError(f"skh_scheduled_pairs: {rc}")
+    return [(int(i), int(j)) for i, j in out[: n.value]]
+
+
+def verified_matches(matches, inliers) -> List[Tuple[int, int]]:
+    """tracks.rs:56-69: the matches whose inlier flag is set; raises
+    LengthMismatch((len(matches), len(inliers))) for the reference's Err."""
+    lib = load_library()
+    m = np.ascontiguousarray(np.asarray(matches, dtype=np.uint32).reshape(-1, 2))
+    f = np.ascontiguousarray(np.asarray(inliers, dtype=np.uint8).reshape(-1))
+    if m.shape[0] != f.shape[0]:
+        raise LengthMismatch(m.shape[0], f.shape[0])
+    out = np.zeros((max(1, m.shape[0]), 2), np.uint32)
+    n = ctypes.c_size_t(0)
+    rc = lib.skh_verified_matches(_ptr(m), m.shape[0], _ptr(f), f.shape[0], _ptr(out),
+                                  ctypes.byref(n))
+    if rc != SKH_OK:
+        raise SkhError(f"skh_verified_matches: {rc}")
     return [(int(i), int(j)) for i, j in out[: n.value]]
 
 

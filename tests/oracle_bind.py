@@ -46,6 +46,8 @@ _SIGS = {
     "orc_ratio_match_state": (None, [_vp, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "orc_pairs_hash": (_u64, [_vp, _sz]),
     "orc_candidate_pairs": (_sz, [_vp, _sz, _u64, _u64, _u64, _vp, _sz]),
+    "orc_scheduled_pairs": (_sz, [_vp, _sz, _vp, _vp, _sz]),
+    "orc_verified_matches": (ctypes.c_long, [_vp, _sz, _vp, _sz, _vp]),
 }
 
 _lib = None
@@ -390,3 +392,39 @@ def ransac_traced(x1, x2, cfg=None, cap=4096):
     finally:
         k = lib().orc_ransac_trace_end()
     return res, recs[:k], masks[:k, :n].astype(bool)
+
+
+class PairSched(ctypes.Structure):
+    _fields_ = [("temporal", _u64), ("pow2_max", _u64), ("cross_kind", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("range", _u64), ("right_min", _u64), ("left_min", _u64),
+                ("max", _u64), ("step", _u64)]
+
+
+def pair_schedule(kind="formation", **kw):
+    """PairSchedule::default() (formation) or ::spec(), fields overridable (matching.rs:73-99)."""
+    s = PairSched(temporal=5, pow2_max=16, cross_kind=1, range=4, right_min=20, left_min=20,
+                  max=40, step=4)
+    if kind == "spec":
+        s.cross_kind = 0
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def scheduled_pairs(views, sch):
+    v = np.ascontiguousarray(np.asarray(views, dtype=np.uint64).reshape(-1, 2))
+    n = lib().orc_scheduled_pairs(P(v), v.shape[0], ctypes.byref(sch), None, 0)
+    out = np.zeros((max(1, n), 2), np.uint64)
+    lib().orc_scheduled_pairs(P(v), v.shape[0], ctypes.byref(sch), P(out), n)
+    return [(int(i), int(j)) for i, j in out[:n]]
+
+
+def verified_matches(matches, inliers):
+    """list, or ("err", (len(matches), len(inliers)))."""
+    m = np.ascontiguousarray(np.asarray(matches, dtype=np.uint64).reshape(-1, 2))
+    f = np.ascontiguousarray(np.asarray(inliers, dtype=np.uint8).reshape(-1))
+    out = np.zeros((max(1, m.shape[0]), 2), np.uint64)
+    k = lib().orc_verified_matches(P(m), m.shape[0], P(f), f.shape[0], P(out))
+    if k < 0:
+        return ("err", (m.shape[0], f.shape[0]))
+    return [(int(i), int(j)) for i, j in out[:k]]

@@ -81,6 +81,8 @@ def _c_layout():
                           "parallax_needed", "sigma_px"],
         "skh_batch_desc": ["n_views", "view_offset", "d_desc", "d_kp", "n_pairs", "pairs", "ratio",
                            "mutual", "precision", "run_ransac", "px_offset", "ransac"],
+        "skh_pair_schedule": ["temporal", "pow2_max", "cross_kind", "range", "right_min",
+                              "left_min", "max", "step"],
     }
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "skylens_hip.h"',
              "int main(void) {"]
@@ -106,7 +108,8 @@ def test_struct_layout_matches_c_header():
     got, fields = _c_layout()
     mirrors = {"skh_ransac_cfg": skylens_hip.RansacConfig,
                "skh_model_sel": skylens_hip.ModelSelectionC,
-               "skh_batch_desc": skylens_hip.BatchDesc}
+               "skh_batch_desc": skylens_hip.BatchDesc,
+               "skh_pair_schedule": skylens_hip.PairSchedule}
     for st, cls in mirrors.items():
         assert ctypes.sizeof(cls) == got[st], st
         for f in fields[st]:

@@ -141,3 +141,31 @@ def test_plans_interleaved(ctx):
         plan.close()
     for c in extra:
         c.close()
+
+
+def test_plan_verified_matches(ctx):
+    """The plan's device-side verified lists (tracks.rs:56-69 per pair) equal each
+    pair's matches filtered by its inlier mask, and are empty without a model."""
+    import torch
+
+    sc = _scene(13, 8, 150, 900)
+    desc = torch.from_numpy(sc.desc).cuda()
+    kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
+    for prec in skylens_hip.available_precisions():
+        plan = skylens_hip.Plan(ctx, sc.offsets, desc.data_ptr(), kp.data_ptr(), sc.pairs, 0.8,
+                                True, prec)
+        try:
+            plan.run()
+            r = plan.results()
+            vl, vij = plan.verified()
+            for p in range(len(sc.pairs)):
+                b = int(r.match_base[p])
+                if r.found[p] == 1:
+                    want = skylens_hip.verified_matches(r.matches(p), r.inlier_mask(p))
+                else:
+                    want = []
+                got = [tuple(map(int, x)) for x in vij[b:b + int(vl[p])]]
+                assert got == want, (prec, p)
+            assert int(vl.sum()) > 0
+        finally:
+            plan.close()
