@@ -143,7 +143,7 @@ int skh_ratio_match_dev(skh_ctx* ctx, const float* d_a, size_t na, const float* 
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Same
  * arguments, results and asynchrony as skh_ratio_match_dev; its working
- * buffers are one fixed device array of about 8.6 MB per GPU, held until the
+ * buffers are one fixed device array of about 9.2 MB per GPU, held until the
  * process ends, so na and nb are limited to 8192 rows each (the reference's
  * default feature cap, features.rs:285); larger inputs return SKH_EINVAL.
  * Every call on a GPU shares that array and leaves its contents behind:

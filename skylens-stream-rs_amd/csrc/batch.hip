@@ -44,40 +44,6 @@ __global__ void handoff_kernel(const uint32_t* __restrict__ match_ij,
     }
 }
 
-// tracks.rs:56-69 per pair, on the device: the matches whose inlier flag is
-// set, in match order, for pairs where RANSAC found a model (a pair without a
-// model is dropped, as the pipeline's consumer does). One workgroup per pair,
-// ballot compaction.
-__global__ __launch_bounds__(256) void verify_compact_kernel(
-    const uint32_t* __restrict__ match_ij, const uint32_t* __restrict__ match_len,
-    const uint64_t* __restrict__ match_base, const uint8_t* __restrict__ inliers,
-    const int32_t* __restrict__ found, uint32_t* __restrict__ ver_ij,
-    uint32_t* __restrict__ ver_len) {
-    __shared__ uint32_t wave_cnt[4];
-    const uint32_t p = blockIdx.x;
-    const uint64_t base = match_base[p];
-    const uint32_t len = found[p] == 1 ? match_len[p] : 0u;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t total = 0;
-    for (uint32_t b0 = 0; b0 < len; b0 += 256) {
-        const uint32_t k = b0 + threadIdx.x;
-        const bool keep = k < len && inliers[base + k] != 0;
-        const unsigned long long m = __ballot(keep);
-        const uint32_t before = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_cnt[wave] = __popcll(m);
-        __syncthreads();
-        uint32_t off = total;
-        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
-        if (keep) {
-            ver_ij[2 * (base + off + before)] = match_ij[2 * (base + k)];
-            ver_ij[2 * (base + off + before) + 1] = match_ij[2 * (base + k) + 1];
-        }
-        total += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) ver_len[p] = total;
-}
-
 // Pseudo-random unit rows for the tile tuner: row r, component k from a hash
 // of (r, k), then scaled to unit length (one 128-thread block per row).
 __global__ void tune_fill_kernel(float* __restrict__ d, uint64_t rows) {
@@ -109,13 +75,13 @@ struct Plan {
     uint32_t tbits = 1;
     // Row blocks of the ranking launch, grouped by tile: group g holds the
     // blocks [grp_off[g], grp_off[g] + grp_n[g]) of d_blocks (the exact plan
-    // uses group 0). Groups after the first run on side streams forked from
-    // the plan's stream, so the groups' launches overlap.
+    // uses group 0). The groups launch one after another on the plan's
+    // stream: forked onto side streams they would share the process's few
+    // hardware queues with other plans' long RANSAC launches and wait behind
+    // them, which costs more than the overlap of small groups gains.
     uint32_t grp_off[kNumTiles] = {}, grp_n[kNumTiles] = {}, pairs_per_tile[kNumTiles] = {};
     int grp_tile[kNumTiles] = {};
     int n_groups = 0;
-    hipStream_t side[kNumTiles - 1] = {};
-    hipEvent_t fork_ev = nullptr, join_ev[kNumTiles - 1] = {};
     bool lowp = false;
     bool fp8 = false;
     uint32_t n_blocks = 0;
@@ -124,7 +90,7 @@ struct Plan {
 
     DevBuf d_view_off, d_tasks_exact, d_tasks_lowp, d_blocks, d_stats, d_q, d_nrm, d_q8, d_meta;
     DevBuf d_row_k1, d_row_j1, d_row_k2, d_row_d1, d_row_d2, d_col_key, d_col_part, d_col_top2;
-    DevBuf d_row_match, d_queue, d_queue_len, d_match_ij, d_match_len, d_match_base, d_kp_a,
+    DevBuf d_row_match, d_queue, d_queue_len, d_pre, d_chunks, d_match_ij, d_match_len, d_match_base, d_kp_a,
         d_kp_b;
     DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters, d_prof, d_order,
         d_est, d_rqueue, d_ver_ij, d_ver_len;
@@ -140,7 +106,7 @@ struct Plan {
     std::vector<DevBuf*> bufs() {
         return {&d_view_off, &d_tasks_exact, &d_tasks_lowp, &d_blocks, &d_stats,  &d_q,
                 &d_nrm,      &d_row_k1,      &d_row_j1,     &d_row_k2, &d_row_d1, &d_row_d2,
-                &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len,
+                &d_col_key,  &d_col_part,    &d_col_top2,   &d_row_match, &d_queue, &d_queue_len, &d_pre, &d_chunks,
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
                 &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue,
@@ -154,11 +120,6 @@ struct Plan {
         if (g_ransac) (void)hipGraphDestroy(g_ransac);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
-        if (fork_ev) (void)hipEventDestroy(fork_ev);
-        for (int k = 0; k < kNumTiles - 1; ++k) {
-            if (join_ev[k]) (void)hipEventDestroy(join_ev[k]);
-            if (side[k]) (void)hipStreamDestroy(side[k]);
-        }
         for (DevBuf* b : bufs())
             if (b->ptr) (void)hipFree(b->ptr);
     }
@@ -323,7 +284,9 @@ int Plan::build() {
         if ((rc = alloc(d_row_k1, 4ull * rb, "row_k1")) || (rc = alloc(d_row_k2, 4ull * rb, "row_k2")) ||
             (rc = alloc(d_col_part, 8ull * pb, "col_part")) ||
             (rc = alloc(d_col_top2, 8ull * cb, "col_top2")) ||
-            (rc = alloc(d_queue, 16ull * rb, "queue")) || (rc = alloc(d_queue_len, 4, "queue_len")))
+            (rc = alloc(d_queue, 16ull * rb, "queue")) || (rc = alloc(d_pre, 16ull * rb, "pre")) ||
+            (rc = alloc(d_chunks, 8ull * chunk_capacity(rb, P), "chunks")) ||
+            (rc = alloc(d_queue_len, 8, "queue_len")))
             return rc;
     } else {
         if ((rc = alloc(d_row_d1, 4ull * rb, "row_d1")) || (rc = alloc(d_row_d2, 4ull * rb, "row_d2")) ||
@@ -363,20 +326,17 @@ int Plan::build() {
             t.iters_out = d_iters.as<uint64_t>() + p;
             t.n_dev = d_match_len.as<uint32_t>() + p;
             t.prof = d_prof.as<uint64_t>() + 11ull * p;
+            // tracks.rs:56-69: a pair with a model writes its verified matches
+            // (the inlier matches in match order) beside its match list
+            t.pair_ij = d_match_ij.as<uint32_t>() + 2ull * match_base[p];
+            t.ver_ij = d_ver_ij.as<uint32_t>() + 2ull * match_base[p];
+            t.ver_len = d_ver_len.as<uint32_t>() + p;
         }
         if ((rc = upload(d_rtasks, rt.data(), sizeof(RansacTask) * P, "rtasks"))) return rc;
     }
 
     for (hipEvent_t& e : ev)
         if ((rc = ctx->hip(hipEventCreate(&e), "event"))) return rc;
-    if (n_groups > 1) {
-        if ((rc = ctx->hip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "event")))
-            return rc;
-        for (int g = 0; g + 1 < n_groups; ++g)
-            if ((rc = ctx->hip(hipStreamCreateWithFlags(&side[g], hipStreamNonBlocking), "stream")) ||
-                (rc = ctx->hip(hipEventCreateWithFlags(&join_ev[g], hipEventDisableTiming), "event")))
-                return rc;
-    }
     // Capture the two stage graphs.
     hipStream_t s = ctx->stream;
     if ((rc = ctx->hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
@@ -409,8 +369,10 @@ void Plan::enqueue_match(hipStream_t s) {
         o.row_match = d_row_match.as<uint32_t>();
         o.queue = d_queue.as<uint4>();
         o.queue_len = d_queue_len.as<uint32_t>();
+        o.pre = d_pre.as<uint4>();
+        o.chunks = fp8 ? d_chunks.as<uint2>() : nullptr;
         launch_stats_reset(d_stats.as<ViewStats>(), desc.n_views, s);
-        launch_fill_u32(o.queue_len, 1, 0u, s);
+        launch_fill_u32(o.queue_len, 2, 0u, s);
         if (fp8)
             launch_quantize_views_fp8(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
                                       max_view_rows, d_q8.as<uint8_t>(), d_meta.as<Fp8Meta>(),
@@ -451,17 +413,8 @@ void Plan::launch_distance(hipStream_t s) {
         o.col_part = d_col_part.as<float2>();
         const LowpTask* lt = d_tasks_lowp.as<LowpTask>();
         const RowBlock* bl = d_blocks.as<RowBlock>();
-        if (n_groups > 1) {  // fork: groups 1.. on side streams, joined back into s
-            (void)hipEventRecord(fork_ev, s);
-            for (int g = 1; g < n_groups; ++g) {
-                (void)hipStreamWaitEvent(side[g - 1], fork_ev, 0);
-                launch_mfma_rank(grp_tile[g], fp8, lt, bl + grp_off[g], grp_n[g], o, tbits,
-                                 side[g - 1]);
-                (void)hipEventRecord(join_ev[g - 1], side[g - 1]);
-            }
-        }
-        if (n_groups > 0) launch_mfma_rank(grp_tile[0], fp8, lt, bl + grp_off[0], grp_n[0], o, tbits, s);
-        for (int g = 1; g < n_groups; ++g) (void)hipStreamWaitEvent(s, join_ev[g - 1], 0);
+        for (int g = 0; g < n_groups; ++g)
+            launch_mfma_rank(grp_tile[g], fp8, lt, bl + grp_off[g], grp_n[g], o, tbits, s);
     } else {
         launch_exact_state(d_tasks_exact.as<PairTask>(), d_blocks.as<RowBlock>(), n_blocks,
                            d_row_j1.as<uint32_t>(), d_row_d1.as<float>(), d_row_d2.as<float>(),
@@ -477,12 +430,9 @@ void Plan::enqueue_ransac(hipStream_t s) {
                        d_kp_a.as<uint64_t>(), d_kp_b.as<uint64_t>(),
                        reinterpret_cast<const float2*>(desc.d_kp), desc.px_offset,
                        d_x1.as<double2>(), d_x2.as<double2>());
+    launch_fill_u32(d_ver_len.as<uint32_t>(), P, 0u, s);  // pairs without a model keep 0
     launch_ransac(d_rtasks.as<RansacTask>(), P, desc.ransac, s, d_rqueue.as<uint32_t>(),
                   d_est.as<uint32_t>(), d_order.as<uint32_t>());
-    hipLaunchKernelGGL(verify_compact_kernel, dim3(P), dim3(256), 0, s, d_match_ij.as<uint32_t>(),
-                       d_match_len.as<uint32_t>(), d_match_base.as<uint64_t>(),
-                       d_inl.as<uint8_t>(), d_found.as<int32_t>(), d_ver_ij.as<uint32_t>(),
-                       d_ver_len.as<uint32_t>());
 }
 
 }  // namespace skh

@@ -15,36 +15,15 @@
 // minimum for this rounding sequence.
 #pragma clang fp contract(off)
 
+#include "exact_tile.hpp"
 #include "skh_internal.hpp"
 
 namespace skh {
 namespace {
 
-constexpr int BM = 64;
-constexpr int BN = 64;
-constexpr int NT = 256;
-
-struct Top2 {
-    uint32_t j;
-    float d1, d2;
-};
-
-// Merge two row states over disjoint column sets: d1 = min with the lower
-// index on equal values, d2 = second smallest of the multiset {a1,a2,b1,b2}.
-__device__ __forceinline__ Top2 merge_top2(Top2 a, Top2 b) {
-    Top2 r;
-    bool b_wins = (b.d1 < a.d1) || (b.d1 == a.d1 && b.j < a.j);
-    if (b_wins) {
-        r.j = b.j;
-        r.d1 = b.d1;
-        r.d2 = a.d1 < b.d2 ? a.d1 : b.d2;
-    } else {
-        r.j = a.j;
-        r.d1 = a.d1;
-        r.d2 = b.d1 < a.d2 ? b.d1 : a.d2;
-    }
-    return r;
-}
+constexpr int BM = kExactBM;
+constexpr int BN = kExactBN;
+constexpr int NT = kExactNT;
 
 __global__ __launch_bounds__(NT) void exact_state_kernel(const PairTask* __restrict__ tasks,
                                                          const RowBlock* __restrict__ blocks,
@@ -59,8 +38,8 @@ __global__ __launch_bounds__(NT) void exact_state_kernel(const PairTask* __restr
     if (blk.pair == kNone) return;  // padding of the XCD-interleaved work list
     const PairTask t = tasks[blk.pair];
     const int tid = threadIdx.x;
-    const int tx = tid & 15;  // column group: cols tx*4 .. tx*4+3 of a chunk
-    const int ty = tid >> 4;  // row group: rows ty*4 .. ty*4+3 of the block
+    const int tx = tid & 15;
+    const int ty = tid >> 4;
     const uint32_t rows = min<uint32_t>(BM, t.na - blk.row0);
 
     // Load the 64 a-rows transposed: As[c][r]. Missing rows read as 0 and are
@@ -78,80 +57,8 @@ __global__ __launch_bounds__(NT) void exact_state_kernel(const PairTask* __restr
     }
 
     Top2 st[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) st[r] = Top2{kNone, INFINITY, INFINITY};
-
-    for (uint32_t col0 = 0; col0 < t.nb; col0 += BN) {
-        const uint32_t cols = min<uint32_t>(BN, t.nb - col0);
-        __syncthreads();
-        for (int e = tid; e < BN * (kDescLen / 4); e += NT) {
-            int cc = e / (kDescLen / 4);
-            int c4 = e % (kDescLen / 4);
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((uint32_t)cc < cols)
-                v = *reinterpret_cast<const float4*>(t.b + (size_t)(col0 + cc) * kDescLen + 4 * c4);
-            Bs[4 * c4 + 0][cc] = v.x;
-            Bs[4 * c4 + 1][cc] = v.y;
-            Bs[4 * c4 + 2][cc] = v.z;
-            Bs[4 * c4 + 3][cc] = v.w;
-        }
-        __syncthreads();
-
-        float s[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s[r][q] = 0.f;
-
-        // matching.rs:147-154: lane k takes components k, 8+k, ..., 120+k.
-        for (int k = 0; k < 8; ++k) {
-            float acc[4][4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[r][q] = 0.f;
-#pragma unroll 4
-            for (int i = 0; i < 16; ++i) {
-                const int c = 8 * i + k;
-                float4 av = *reinterpret_cast<const float4*>(&As[c][ty * 4]);
-                float4 bv = *reinterpret_cast<const float4*>(&Bs[c][tx * 4]);
-                float a4[4] = {av.x, av.y, av.z, av.w};
-                float b4[4] = {bv.x, bv.y, bv.z, bv.w};
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        float d = a4[r] - b4[q];
-                        acc[r][q] = acc[r][q] + d * d;
-                    }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) s[r][q] = s[r][q] + acc[r][q];
-        }
-
-        // Row state over this thread's 4 columns, ascending j.
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t cj = tx * 4 + q;
-            if (cj < cols) {
-                const uint32_t j = col0 + cj;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    float d = s[r][q];
-                    if (d < st[r].d1) {
-                        st[r].d2 = st[r].d1;
-                        st[r].d1 = d;
-                        st[r].j = j;
-                    } else if (d < st[r].d2) {
-                        st[r].d2 = d;
-                    }
-                }
-            }
-        }
-
-        // Column winner of this block: rows ascending, strict '<'.
+    // Column winner of each chunk: rows ascending, strict '<' (matching.rs:188-203).
+    exact_sweep(As, Bs, t.b, t.nb, st, [&](const float (&s)[4][4], uint32_t col0, uint32_t cols) {
         unsigned long long part[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -181,20 +88,11 @@ __global__ __launch_bounds__(NT) void exact_state_kernel(const PairTask* __restr
             }
             if (best != kColEmpty) atomicMin(&col_key[t.col_base + col0 + tid], best);
         }
-    }
+    });
 
-    // Merge the 16 column groups of each row (lanes tx of one ty group).
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        Top2 m = st[r];
-#pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) {
-            Top2 o;
-            o.j = __shfl_xor(m.j, off, 16);
-            o.d1 = __shfl_xor(m.d1, off, 16);
-            o.d2 = __shfl_xor(m.d2, off, 16);
-            m = merge_top2(m, o);
-        }
+        const Top2 m = exact_row_merge(st[r]);
         const uint32_t rr = ty * 4 + r;
         if (tx == 0 && rr < rows) {
             const uint32_t g = t.row_base + blk.row0 + rr;

@@ -33,6 +33,7 @@
 //    the exact column scan of their j1) in the reference's order.
 #pragma clang fp contract(off)
 
+#include "exact_tile.hpp"
 #include "skh_internal.hpp"
 #include "skh_lowp.hpp"
 
@@ -767,7 +768,7 @@ __global__ __launch_bounds__(256) void decide_kernel(const LowpTask* __restrict_
                                                      LowpOut out, float ratio, int mutual,
                                                      uint32_t tbits) {
     __shared__ uint32_t wave_tot[4];
-    __shared__ uint32_t run_base;
+    __shared__ uint32_t run_base, chunk_base;
     const LowpTask t = tasks[blockIdx.x];  // one workgroup per pair
     const bool bad = (t.sa->bad | t.sb->bad) != 0;
     const double eps = pair_eps(t, tbits);
@@ -795,15 +796,79 @@ __global__ __launch_bounds__(256) void decide_kernel(const LowpTask* __restrict_
         before += w < wave ? wave_tot[w] : 0u;
         total += wave_tot[w];
     }
-    if (tid == 0) run_base = total ? atomicAdd(out.queue_len, total) : 0u;
+    const uint32_t n_chunks =
+        out.chunks && total >= kGatherMinRows ? (total + kChunkEntries - 1) / kChunkEntries : 0u;
+    if (tid == 0 && total) {
+        run_base = atomicAdd(&out.queue_len[0], total);
+        if (n_chunks) chunk_base = atomicAdd(&out.queue_len[1], n_chunks);
+    }
     __syncthreads();
     if (!total) return;
+    for (uint32_t c = tid; c < n_chunks; c += blockDim.x)
+        out.chunks[chunk_base + c] =
+            make_uint2(run_base + c * kChunkEntries, min(kChunkEntries, total - c * kChunkEntries));
     uint32_t slot = run_base + before + incl - mine;
     for (uint32_t i = tid; i < t.na; i += blockDim.x) {
         uint32_t verdict, j1;
         bool queue, need_row;
         decide_row(t, out, bad, eps, r2, mutual, i, verdict, queue, need_row, j1);
-        if (queue) out.queue[slot++] = make_uint4(blockIdx.x, i, need_row ? kNone : j1, 0u);
+        if (queue) {
+            if (out.chunks) out.pre[slot].w = 0u;
+            out.queue[slot++] = make_uint4(blockIdx.x, i, need_row ? kNone : j1, 0u);
+        }
+    }
+}
+
+// Gathered exact row scans: the row-scan entries of a chunk (up to 64 queue
+// entries of one pair) holding at least kGatherMinRows of them, their a rows
+// staged transposed, against all of b with the exact FP32 sweep (the
+// reference's formula and row update order); results per queue entry in pre.
+// Sparser chunks leave their rows to rescore_kernel, whose per-row scans cost
+// less when few rows share b.
+__global__ __launch_bounds__(kExactNT) void exact_rows_kernel(const LowpTask* __restrict__ tasks,
+                                                              LowpOut out) {
+    __shared__ float As[kDescLen][kExactBM];
+    __shared__ float Bs[kDescLen][kExactBN];
+    __shared__ uint32_t slot_of[kExactBM];
+    __shared__ uint32_t n_rows;
+    const uint32_t n_chunks = out.queue_len[1];
+    const int tid = threadIdx.x;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint2 ck = out.chunks[c];
+        const uint32_t k0 = ck.x;
+        __syncthreads();  // the previous chunk's readers of As / slot_of are done
+        if (tid < 64) {  // one wave compacts the chunk's row-scan entries
+            bool scan = false;
+            if ((uint32_t)tid < ck.y) scan = out.queue[k0 + tid].z == kNone;
+            const unsigned long long m = __ballot(scan);
+            if (scan) slot_of[__popcll(m & ((1ull << tid) - 1ull))] = k0 + tid;
+            if (tid == 0) n_rows = __popcll(m);
+        }
+        __syncthreads();
+        const uint32_t rows = n_rows;
+        if (rows < kGatherMinRows) continue;  // uniform: n_rows is read after the barrier
+        const LowpTask t = tasks[out.queue[k0].x];
+        for (int e = tid; e < kExactBM * (kDescLen / 4); e += kExactNT) {
+            const int r = e / (kDescLen / 4), c4 = e % (kDescLen / 4);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((uint32_t)r < rows)
+                v = *reinterpret_cast<const float4*>(
+                    t.a + (size_t)out.queue[slot_of[r]].y * kDescLen + 4 * c4);
+            As[4 * c4 + 0][r] = v.x;
+            As[4 * c4 + 1][r] = v.y;
+            As[4 * c4 + 2][r] = v.z;
+            As[4 * c4 + 3][r] = v.w;
+        }
+        Top2 st[4];
+        exact_sweep(As, Bs, t.b, t.nb, st, [](const float (&)[4][4], uint32_t, uint32_t) {});
+        const int tx = tid & 15, ty = tid >> 4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const Top2 m = exact_row_merge(st[r]);
+            const uint32_t rr = ty * 4 + r;
+            if (tx == 0 && rr < rows)
+                out.pre[slot_of[rr]] = make_uint4(m.j, __float_as_uint(m.d1), __float_as_uint(m.d2), 1u);
+        }
     }
 }
 
@@ -965,7 +1030,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
     __shared__ Top2x row_res[kRescoreGroup];
     __shared__ uint32_t col_res[kRescoreGroup];
     const float r2 = ratio * ratio;
-    const uint32_t n = *out.queue_len;
+    const uint32_t n = out.queue_len[0];
     for (uint32_t base = blockIdx.x * kRescoreGroup; base < n;
          base += gridDim.x * kRescoreGroup) {
         const uint32_t kend = min(base + (uint32_t)kRescoreGroup, n);
@@ -979,50 +1044,59 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
             const bool bad = (t.sa->bad | t.sb->bad) != 0;
             const double eps = pair_eps(t, tbits);
             const int m = (int)(k1 - k0);
+            // Entry classes as bit masks over r (bit r: entry k0 + r): j1 known (ratio
+            // accepted with certainty), row scan done by exact_rows_kernel, row scan here.
+            // A row scan's slot in ys / row_res is the count of row scans below it.
             uint32_t row_i[kRescoreGroup], j1[kRescoreGroup];
             bool accept[kRescoreGroup], col_needed[kRescoreGroup];
-            int row_slot[kRescoreGroup];
-            int mr = 0;
+            uint32_t known_m = 0, pre_m = 0, scan_m = 0;
 #pragma unroll
             for (int r = 0; r < kRescoreGroup; ++r) {
                 row_i[r] = kNone;
                 j1[r] = kNone;
-                accept[r] = false;
-                col_needed[r] = false;
-                row_slot[r] = -1;
                 if (r < m) {
                     const uint4 item = out.queue[k0 + r];
                     row_i[r] = item.y;
                     j1[r] = item.z;
-                    col_needed[r] = true;
-                    if (item.z == kNone) row_slot[r] = mr++;
-                    else accept[r] = true;  // ratio accepted with certainty, j1 certain
+                    if (item.z != kNone) known_m |= 1u << r;
+                    else if (out.chunks && out.pre[k0 + r].w) pre_m |= 1u << r;
+                    else scan_m |= 1u << r;
                 }
+                accept[r] = (known_m >> r) & 1u;
+                col_needed[r] = r < m;
             }
+            const int mr = __popc(scan_m);
             // row scans of the entries that need one
             if (mr > 0) {
                 __syncthreads();
 #pragma unroll
                 for (int r = 0; r < kRescoreGroup; ++r)
-                    if (row_slot[r] >= 0 && threadIdx.x < kDescLen)
-                        ys[row_slot[r]][threadIdx.x] = t.a[(size_t)row_i[r] * kDescLen + threadIdx.x];
+                    if (((scan_m >> r) & 1u) && threadIdx.x < kDescLen)
+                        ys[__popc(scan_m & ((1u << r) - 1u))][threadIdx.x] =
+                            t.a[(size_t)row_i[r] * kDescLen + threadIdx.x];
                 __syncthreads();
                 exact_row_top2_group(t, ys, mr, red_t, row_res);
+            }
 #pragma unroll
-                for (int r = 0; r < kRescoreGroup; ++r) {
-                    if (row_slot[r] < 0) continue;
-                    const Top2x st = row_res[row_slot[r]];
-                    j1[r] = st.j;
-                    accept[r] = st.j != kNone && isfinite(st.d2) && !(st.d1 >= r2 * st.d2);
-                    if (accept[r] && mutual && !bad) {
-                        const float2 c = out.col_top2[t.col_base + st.j];
-                        const double d1 = st.d1;
-                        if ((2.0 * (double)c.y) - d1 > eps) {
-                            col_needed[r] = false;  // row i is the unique column winner
-                        } else if (d1 - 2.0 * (double)c.x > eps) {
-                            accept[r] = false;  // another row is strictly closer
-                            col_needed[r] = false;
-                        }
+            for (int r = 0; r < kRescoreGroup; ++r) {
+                if (!(((scan_m | pre_m) >> r) & 1u)) continue;
+                Top2x st;
+                if ((pre_m >> r) & 1u) {
+                    const uint4 pr = out.pre[k0 + r];
+                    st = Top2x{pr.x, __uint_as_float(pr.y), __uint_as_float(pr.z)};
+                } else {
+                    st = row_res[__popc(scan_m & ((1u << r) - 1u))];
+                }
+                j1[r] = st.j;
+                accept[r] = st.j != kNone && isfinite(st.d2) && !(st.d1 >= r2 * st.d2);
+                if (accept[r] && mutual && !bad) {
+                    const float2 c = out.col_top2[t.col_base + st.j];
+                    const double d1 = st.d1;
+                    if ((2.0 * (double)c.y) - d1 > eps) {
+                        col_needed[r] = false;  // row i is the unique column winner
+                    } else if (d1 - 2.0 * (double)c.x > eps) {
+                        accept[r] = false;  // another row is strictly closer
+                        col_needed[r] = false;
                     }
                 }
             }
@@ -1158,6 +1232,9 @@ void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, 
 
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
                     uint32_t tbits, uint32_t grid, hipStream_t s) {
+    if (out.chunks)
+        hipLaunchKernelGGL(exact_rows_kernel, dim3(std::max(grid / 2, 1u)), dim3(kExactNT), 0, s,
+                           tasks, out);
     hipLaunchKernelGGL(rescore_kernel, dim3(grid), dim3(256), 0, s, tasks, out, ratio, mutual,
                        tbits);
 }

@@ -81,9 +81,11 @@ void enqueue_lowp_single(const SingleBufs& B, const float* d_a, uint32_t na, con
     o.row_match = B.row_match;
     o.queue = B.queue;
     o.queue_len = B.queue_len;
+    o.pre = B.pre;
+    o.chunks = fp8 ? B.chunks : nullptr;
 
     launch_stats_reset(B.stats, 2, s);
-    launch_fill_u32(o.queue_len, 1, 0u, s);
+    launch_fill_u32(o.queue_len, 2, 0u, s);
     launch_fill_u64(reinterpret_cast<unsigned long long*>(B.out_base), 1, 0ull, s);
     if (fp8) {
         launch_quantize_fp8(d_a, na, B.q8_a, B.meta_a, B.stats, s);
@@ -154,8 +156,9 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
         (rc = ensure(col_part, sizeof(float2) * (size_t)nblk * nb, "col_part")) ||
         (rc = ensure(col_top2, sizeof(float2) * nb, "col_top2")) ||
         (rc = ensure(row_match, 4 * na, "row_match")) ||
-        (rc = ensure(queue, sizeof(uint4) * na, "queue")) ||
-        (rc = ensure(queue_len, 4, "queue_len")) || (rc = ensure(out_base, 8, "out_base")) ||
+        (rc = ensure(queue, sizeof(uint4) * na, "queue")) || (rc = ensure(pre, sizeof(uint4) * na, "pre")) ||
+        (rc = ensure(chunks, sizeof(uint2) * chunk_capacity(na, 1), "chunks")) ||
+        (rc = ensure(queue_len, 8, "queue_len")) || (rc = ensure(out_base, 8, "out_base")) ||
         (rc = ensure(tasks, sizeof(PairTask), "tasks")))
         return rc;
 
@@ -180,6 +183,8 @@ int Ctx::ratio_match_lowp(const float* d_a, size_t na, const float* d_b, size_t 
     B.row_match = row_match.as<uint32_t>();
     B.queue = queue.as<uint4>();
     B.queue_len = queue_len.as<uint32_t>();
+    B.pre = pre.as<uint4>();
+    B.chunks = chunks.as<uint2>();
     B.out_base = out_base.as<uint64_t>();
     enqueue_lowp_single(B, d_a, (uint32_t)na, d_b, (uint32_t)nb, ratio, mutual, fp8, tile, num_cu,
                         d_out_ij, d_out_len, stream);

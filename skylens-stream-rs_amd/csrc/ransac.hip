@@ -1992,7 +1992,22 @@ __device__ void run_pair(RansacTask t, const RansacCfgDev& cfg, const bool big) 
         *t.sel = selo;
         *t.found = 1;
     }
-    for (uint32_t i = lane; i < n; i += NL) t.inliers[i] = (fin_mask[i >> 5] >> (i & 31)) & 1u;
+    uint32_t kept = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += NL) {
+        const uint32_t i = i0 + lane;
+        const bool keep = i < n && ((fin_mask[i >> 5] >> (i & 31)) & 1u);
+        if (i < n) t.inliers[i] = keep ? 1u : 0u;
+        if (t.ver_ij) {  // tracks.rs:56-69: the inlier matches in match order
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const uint32_t k = kept + (uint32_t)__popcll(m & lanes_below());
+                t.ver_ij[2 * k] = t.pair_ij[2 * i];
+                t.ver_ij[2 * k + 1] = t.pair_ij[2 * i + 1];
+            }
+            kept += (uint32_t)__popcll(m);
+        }
+    }
+    if (t.ver_len && lane == 0) *t.ver_len = kept;
     lap(7);
     flush_prof();
 }

@@ -42,7 +42,7 @@ struct Ctx {
     DevBuf tasks, blocks, row_j1, row_d1, row_d2, col_key, row_match, out_base;
     // low-precision state
     DevBuf q_a, q_b, nrm_a, nrm_b, stats, ltasks, row_k1, row_k2, col_part, col_top2, queue,
-        queue_len, q8_a, q8_b, meta_a, meta_b;
+        queue_len, q8_a, q8_b, meta_a, meta_b, pre, chunks;
     // single-pair RANSAC
     DevBuf r_x1, r_x2, r_out, r_scratch, r_task, r_queue;
 
@@ -52,7 +52,7 @@ struct Ctx {
         return {&in_a,   &in_b,   &out_ij,    &out_len,  &tasks,    &blocks, &row_j1,
                 &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a,    &q_b,
                 &nrm_a,  &nrm_b,  &stats,     &ltasks,   &row_k1,   &row_k2, &col_part,
-                &col_top2, &queue, &queue_len, &q8_a, &q8_b, &meta_a, &meta_b, &r_x1, &r_x2, &r_out,
+                &col_top2, &queue, &queue_len, &q8_a, &q8_b, &meta_a, &meta_b, &pre, &chunks, &r_x1, &r_x2, &r_out,
                 &r_scratch, &r_task, &r_queue};
     }
 

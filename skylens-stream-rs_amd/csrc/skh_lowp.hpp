@@ -50,9 +50,22 @@ struct LowpOut {
     float2* col_part;          // per workgroup column (c1, c2)
     float2* col_top2;          // merged column (c1, c2)
     uint32_t* row_match;       // final j or kNone
-    uint4* queue;              // (pair, row, j1 | kNone, 0) needing exact work
-    uint32_t* queue_len;
+    // Exact re-score queue, one contiguous run of entries per pair. Entry
+    // (pair, row, j1 | kNone, 0): j1 = kNone needs the row scan, else only the
+    // column check of j1.
+    uint4* queue;
+    uint32_t* queue_len;       // [0] queue entries, [1] chunks
+    uint4* pre;                // per entry: exact (j1, d1 bits, d2 bits, 1) of a gathered row scan
+    // Chunks (first entry, entries <= kChunkEntries) of the runs of pairs with
+    // at least kGatherMinRows queued rows, for the gathered row scans. Null:
+    // no gathered pass (pre unused); the BF16 queue is too sparse to gain.
+    uint2* chunks;
 };
+
+constexpr uint32_t kChunkEntries = 64;
+constexpr uint32_t kGatherMinRows = 16;
+// Chunk list capacity of a batch with `rows` a-side rows over `pairs` pairs.
+constexpr uint64_t chunk_capacity(uint64_t rows, uint64_t pairs) { return rows / kChunkEntries + pairs; }
 
 enum TileShape : int {
     kTile128x64 = 0,   // 4 waves x 32 rows, 64-column chunks

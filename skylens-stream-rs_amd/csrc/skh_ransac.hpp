@@ -28,6 +28,12 @@ struct RansacTask {
     uint64_t* iters_out;  // hypotheses drawn (diagnostics), may be null
     const uint32_t* n_dev;  // if set, the device-side count (<= n) replaces n
     uint64_t* prof;         // if set, 11 phase clocks (wall_clock64 ticks)
+    // If set, a pair with a model also writes its verified matches: the
+    // (i, j) words of pair_ij whose inlier flag is set, in order, to ver_ij,
+    // and their count to *ver_len (left untouched otherwise).
+    const uint32_t* pair_ij;
+    uint32_t* ver_ij;
+    uint32_t* ver_len;
 };
 
 uint32_t ransac_scratch_words(uint32_t n);

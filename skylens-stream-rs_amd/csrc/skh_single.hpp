@@ -25,8 +25,10 @@ struct SingleBufs {
     float* row_k2;
     float2* col_part;
     float2* col_top2;
-    uint4* queue;
-    uint32_t* queue_len;
+    uint4* queue;         // na entries
+    uint32_t* queue_len;  // 2
+    uint4* pre;           // na entries
+    uint2* chunks;        // chunk_capacity(na, 1) entries
     // exact FP32
     float* row_d1;
     float* row_d2;

@@ -1,5 +1,5 @@
 """The tile autotuner: a tuned table per low precision, plans that mix tiles by
-size class (one ranking launch per tile, overlapped on forked streams) and
+size class (one ranking launch per tile, in sequence on the plan's stream) and
 still return the oracle's results for every pair of the full scene and the
 uneven batch (fixtures written by the oracle)."""
 import numpy as np
