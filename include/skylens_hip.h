@@ -16,6 +16,7 @@
  *   ransac_fundamental:941-1104 -> skh_ransac_fundamental
  *   per-pair driver   benches/pipeline.rs:219-305 -> skh_plan_create / skh_plan_run /
  *                                                      skh_plan_results
+ *   detect_and_describe  features.rs:594-648 -> skh_detect_and_describe
  */
 #ifndef SKYLENS_HIP_H
 #define SKYLENS_HIP_H
@@ -286,6 +287,45 @@ int skh_plan_set_timing(skh_plan* plan, int on);
  * 5 final refits and gates, 6 line test, 7 model selection, 8 batches,
  * 9 LO lanes, 10 total); *ticks_per_ms = the wall clock's rate. */
 int skh_plan_ransac_profile(skh_plan* plan, uint64_t* iters, uint64_t* phase, double* ticks_per_ms);
+
+/* ---- feature detector: detect_and_describe (features.rs:594-648) ----------
+ * The producer of the path's descriptors: DoG extrema with sub-pixel /
+ * sub-scale refinement, edge test, dominant orientations and the 128-d
+ * descriptor, on a single-channel f32 image (GrayImage, row-major, w x h,
+ * features.rs:47-52; the Rust wrapper passes GrayImage::data()).
+ * Results equal the reference's algorithm in f32 with its operation order;
+ * the per-pixel f32 transcendentals (exp, atan2, sin, cos, pow) are the
+ * correctly rounded values (the reference's come from the platform libm,
+ * which misrounds some of them in the last bit). */
+typedef struct {
+    uint64_t octaves, scales; /* DetectorConfig (features.rs:259-276), usize as uint64 */
+    float sigma0, contrast, edge_ratio;
+    uint32_t upscale_fill; /* bool */
+    uint64_t max_features, upscale_below;
+} skh_detector_cfg;
+/* Keypoint (features.rs:295-304). */
+typedef struct {
+    float x, y, sigma, response, angle;
+} skh_keypoint;
+/* DetectorConfig::default() (features.rs:277-289): 4 octaves, 3 scales,
+ * sigma0 1.6, contrast 0.01, edge ratio 10, 8192 features, upscale below 800. */
+void skh_detector_cfg_default(skh_detector_cfg* cfg);
+/* detect_and_describe (features.rs:594-648) on a host image: writes the
+ * features in the reference's order, keypoints to kps (cap entries) and
+ * descriptors to desc (cap x 128 f32; NULL skips them: detect(),
+ * features.rs:487-493), and their count to *out_len. cap >= max_features
+ * always suffices; SKH_EINVAL if cap is short (*out_len holds the count).
+ * This build rejects scales outside 1..14 and Gaussian radii above 64
+ * (sigma0 up to ~21). Blocks until the results are on the host. */
+int skh_detect_and_describe(skh_ctx* ctx, const float* img, size_t w, size_t h,
+                            const skh_detector_cfg* cfg, skh_keypoint* kps, float* desc,
+                            size_t cap, size_t* out_len);
+/* The same on a device-resident image, writing device outputs (d_desc may be
+ * NULL). Returns once the count is known (*out_len on the host); the
+ * keypoint and descriptor kernels may still run on the context stream. */
+int skh_detect_and_describe_dev(skh_ctx* ctx, const float* d_img, size_t w, size_t h,
+                                const skh_detector_cfg* cfg, skh_keypoint* d_kps, float* d_desc,
+                                size_t cap, size_t* out_len);
 
 #ifdef __cplusplus
 }

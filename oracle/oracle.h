@@ -184,6 +184,44 @@ void orc_noisy_lines(double sigma, uint64_t seed, double* l1, double* l2);
 void orc_line_plus_general(size_t k, double sigma, uint64_t seed, double* x1, double* x2,
                            double* h1, double* h2);
 
+/* ---- feature detector (features.rs:55-781, features_oracle.c) ---- */
+/* DetectorConfig (features.rs:259-289); usize fields as uint64. */
+typedef struct {
+    uint64_t octaves, scales;
+    float sigma0, contrast, edge_ratio;
+    uint32_t upscale_fill;
+    uint64_t max_features, upscale_below;
+} orc_detector_cfg;
+/* Keypoint (features.rs:295-304). */
+typedef struct {
+    float x, y, sigma, response, angle;
+} orc_keypoint;
+void orc_detector_cfg_default(orc_detector_cfg* c);
+/* detect_and_describe (features.rs:594-648): writes min(total, cap)
+ * features, returns the total (or -1 on allocation failure). */
+long orc_detect_and_describe(const float* img, size_t w, size_t h, const orc_detector_cfg* cfg,
+                             orc_keypoint* kps, float* desc, size_t cap);
+/* gaussian_blur (features.rs:214-256); taps of it (returns r, -1 if cap short). */
+int orc_gaussian_blur(const float* img, size_t w, size_t h, float sigma, float* out);
+long orc_gaussian_taps(float sigma, float* k, size_t cap);
+void orc_upsample2(const float* img, size_t w, size_t h, float* out);   /* features.rs:55-81 */
+void orc_downsample(const float* img, size_t w, size_t h, float* out);  /* :200-210 */
+/* dominant_orientations (features.rs:314-323), describe (:509-519). */
+size_t orc_dominant_orientations(const float* img, size_t w, size_t h, float x, float y,
+                                 float sigma, float* out);
+void orc_describe(const float* img, size_t w, size_t h, float x, float y, float sigma,
+                  float angle, float* desc);
+/* f32 transcendentals evaluated in double (see features_oracle.c) and their
+ * double kernels. */
+float orc_expf(float x);
+float orc_atan2f(float y, float x);
+void orc_sincosf(float a, float* s, float* c);
+float orc_powf(float b, float y);
+double orc_xexp(double x);
+double orc_xlog(double x);
+double orc_xatan2(double y, double x);
+void orc_xsincos(double a, double* s, double* c);
+
 #ifdef __cplusplus
 }
 #endif

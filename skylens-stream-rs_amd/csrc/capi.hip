@@ -51,6 +51,7 @@ int Ctx::ensure(DevBuf& b, size_t bytes, const char* what) {
 }
 
 Ctx::~Ctx() {
+    destroy_detector(detector);
     for (DevBuf* b : all_bufs()) {
         if (b->ptr) (void)hipFree(b->ptr);
     }

@@ -32,6 +32,7 @@ struct Ctx {
     bool tile_tuned[2] = {false, false};
     uint64_t last_iters = 0;  // hypotheses drawn by the last single-pair RANSAC
     void* noalloc_scratch = nullptr;  // device address of the allocation-free entry's scratch
+    void* detector = nullptr;         // feature detector state (features.hip), lazily created
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string last_error;
@@ -73,6 +74,7 @@ struct Ctx {
 };
 
 int noalloc_scratch_address(int device, void** out);
+void destroy_detector(void* detector);
 size_t noalloc_max_rows();
 
 }  // namespace skh

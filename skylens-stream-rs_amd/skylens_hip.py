@@ -92,6 +92,11 @@ EXPORTS = [
     ("skh_scheduled_pairs", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, ctypes.c_size_t, _c_size_p]),
     ("skh_verified_matches", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _c_size_p]),
     ("skh_plan_verified", ctypes.c_int, [_vp, _vp, _vp]),
+    ("skh_detector_cfg_default", None, [_vp]),
+    ("skh_detect_and_describe", ctypes.c_int,
+     [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_size_t, _c_size_p]),
+    ("skh_detect_and_describe_dev", ctypes.c_int,
+     [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_size_t, _c_size_p]),
 ]
 
 TILE_CLASSES = 25
@@ -115,6 +120,25 @@ class RansacConfig(ctypes.Structure):
         for k, v in kw.items():
             setattr(c, k, v)
         return c
+
+
+class DetectorConfig(ctypes.Structure):
+    """DetectorConfig (features.rs:259-289), skh_detector_cfg."""
+    _fields_ = [("octaves", ctypes.c_uint64), ("scales", ctypes.c_uint64),
+                ("sigma0", ctypes.c_float), ("contrast", ctypes.c_float),
+                ("edge_ratio", ctypes.c_float), ("upscale_fill", ctypes.c_uint32),
+                ("max_features", ctypes.c_uint64), ("upscale_below", ctypes.c_uint64)]
+
+    @classmethod
+    def default(cls, **kw) -> "DetectorConfig":
+        c = cls()
+        load_library().skh_detector_cfg_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+KEYPOINT_FIELDS = ("x", "y", "sigma", "response", "angle")  # features.rs:295-304
 
 
 class PairSchedule(ctypes.Structure):
@@ -375,6 +399,37 @@ class Context:
                                                   ctypes.c_float(ratio), int(bool(mutual)),
                                                   PRECISIONS[precision], out_ij_ptr, out_len_ptr)
         self._check(rc, "skh_ratio_match_dev_noalloc")
+
+
+    def detect_and_describe(self, img, cfg: Optional[DetectorConfig] = None,
+                            describe: bool = True):
+        """detect_and_describe (features.rs:594-648) of a (h, w) float32 image:
+        (keypoints (n, 5) float32 in KEYPOINT_FIELDS order, descriptors
+        (n, 128) float32 or None). ``describe=False`` is detect()
+        (features.rs:487-493)."""
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        if img.ndim != 2:
+            raise ValueError("image must be a (height, width) array")
+        cfg = cfg or DetectorConfig.default()
+        h, w = img.shape
+        cap = max(int(cfg.max_features), 1)
+        kps = np.empty((cap, 5), np.float32)
+        desc = np.empty((cap, 128), np.float32) if describe else None
+        n = ctypes.c_size_t(0)
+        self._check(self.lib.skh_detect_and_describe(
+            self.handle, _ptr(img), w, h, ctypes.byref(cfg), _ptr(kps),
+            _ptr(desc) if describe else None, cap, ctypes.byref(n)), "skh_detect_and_describe")
+        k = n.value
+        return kps[:k].copy(), (desc[:k].copy() if describe else None)
+
+    def detect_and_describe_dev(self, img_ptr: int, w: int, h: int, cfg: DetectorConfig,
+                                kp_ptr: int, desc_ptr: int, cap: int) -> int:
+        """Device-resident form: returns the feature count."""
+        n = ctypes.c_size_t(0)
+        self._check(self.lib.skh_detect_and_describe_dev(
+            self.handle, img_ptr, w, h, ctypes.byref(cfg), kp_ptr, desc_ptr or None, cap,
+            ctypes.byref(n)), "skh_detect_and_describe_dev")
+        return n.value
 
 
 class BatchDesc(ctypes.Structure):

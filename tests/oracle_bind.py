@@ -48,6 +48,20 @@ _SIGS = {
     "orc_candidate_pairs": (_sz, [_vp, _sz, _u64, _u64, _u64, _vp, _sz]),
     "orc_scheduled_pairs": (_sz, [_vp, _sz, _vp, _vp, _sz]),
     "orc_verified_matches": (ctypes.c_long, [_vp, _sz, _vp, _sz, _vp]),
+    "orc_detector_cfg_default": (None, [_vp]),
+    "orc_detect_and_describe": (ctypes.c_long, [_vp, _sz, _sz, _vp, _vp, _vp, _sz]),
+    "orc_gaussian_blur": (ctypes.c_int, [_vp, _sz, _sz, ctypes.c_float, _vp]),
+    "orc_gaussian_taps": (ctypes.c_long, [ctypes.c_float, _vp, _sz]),
+    "orc_upsample2": (None, [_vp, _sz, _sz, _vp]),
+    "orc_downsample": (None, [_vp, _sz, _sz, _vp]),
+    "orc_dominant_orientations": (_sz, [_vp, _sz, _sz, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, _vp]),
+    "orc_describe": (None, [_vp, _sz, _sz, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, _vp]),
+    "orc_expf": (ctypes.c_float, [ctypes.c_float]),
+    "orc_atan2f": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_powf": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_sincosf": (None, [ctypes.c_float, _vp, _vp]),
 }
 
 _lib = None
@@ -428,3 +442,165 @@ def verified_matches(matches, inliers):
     if k < 0:
         return ("err", (m.shape[0], f.shape[0]))
     return [(int(i), int(j)) for i, j in out[:k]]
+
+
+# ---- feature detector (features.rs, oracle/features_oracle.c) ----
+
+class DetCfg(ctypes.Structure):
+    """DetectorConfig (features.rs:259-289)."""
+    _fields_ = [("octaves", _u64), ("scales", _u64), ("sigma0", ctypes.c_float),
+                ("contrast", ctypes.c_float), ("edge_ratio", ctypes.c_float),
+                ("upscale_fill", ctypes.c_uint32), ("max_features", _u64),
+                ("upscale_below", _u64)]
+
+
+def detector_cfg(**kw) -> DetCfg:
+    c = DetCfg()
+    lib().orc_detector_cfg_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _f32img(img):
+    return np.ascontiguousarray(img, dtype=np.float32)
+
+
+def detect_and_describe(img, cfg=None, describe=True):
+    """(keypoints (n, 5) float32 [x, y, sigma, response, angle], descriptors (n, 128))."""
+    img = _f32img(img)
+    cfg = cfg or detector_cfg()
+    h, w = img.shape
+    n = lib().orc_detect_and_describe(P(img), w, h, ctypes.byref(cfg), None, None, 0)
+    assert n >= 0, "oracle allocation failure"
+    kps = np.zeros((max(n, 1), 5), np.float32)
+    desc = np.zeros((max(n, 1), 128), np.float32)
+    lib().orc_detect_and_describe(P(img), w, h, ctypes.byref(cfg), P(kps), P(desc), n)
+    return kps[:n], (desc[:n] if describe else None)
+
+
+def gaussian_blur(img, sigma):
+    img = _f32img(img)
+    out = np.zeros_like(img)
+    assert lib().orc_gaussian_blur(P(img), img.shape[1], img.shape[0], ctypes.c_float(sigma),
+                                   P(out)) == 0
+    return out
+
+
+def gaussian_taps(sigma):
+    k = np.zeros(1025, np.float32)
+    r = lib().orc_gaussian_taps(ctypes.c_float(sigma), P(k), 1025)
+    return k[:2 * r + 1]
+
+
+def upsample2(img):
+    img = _f32img(img)
+    out = np.zeros((2 * img.shape[0], 2 * img.shape[1]), np.float32)
+    lib().orc_upsample2(P(img), img.shape[1], img.shape[0], P(out))
+    return out
+
+
+def dominant_orientations(img, x, y, sigma):
+    img = _f32img(img)
+    out = np.zeros(36, np.float32)
+    n = lib().orc_dominant_orientations(P(img), img.shape[1], img.shape[0], ctypes.c_float(x),
+                                        ctypes.c_float(y), ctypes.c_float(sigma), P(out))
+    return out[:n]
+
+
+def describe(img, x, y, sigma, angle):
+    img = _f32img(img)
+    out = np.zeros(128, np.float32)
+    lib().orc_describe(P(img), img.shape[1], img.shape[0], ctypes.c_float(x), ctypes.c_float(y),
+                       ctypes.c_float(sigma), ctypes.c_float(angle), P(out))
+    return out
+
+
+def mathf(name, *args):
+    """orc_expf / orc_atan2f / orc_powf, or orc_sincosf -> (sin, cos)."""
+    if name == "sincos":
+        s, c = ctypes.c_float(), ctypes.c_float()
+        lib().orc_sincosf(ctypes.c_float(args[0]), ctypes.byref(s), ctypes.byref(c))
+        return s.value, c.value
+    return getattr(lib(), "orc_" + name)(*[ctypes.c_float(a) for a in args])
+
+
+# image generators of the reference's detector tests (features.rs:823-985),
+# in float32 numpy (their last bits need not equal the reference's: the tests
+# using them are statistical, and GPU parity feeds both sides the same image)
+
+def blob(w, h, cx, cy, s):
+    """features.rs:823-832."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    r2 = (x - np.float32(cx)) ** 2 + (y - np.float32(cy)) ** 2
+    return (np.float32(0.2) + np.float32(0.6) * np.exp(-r2 / np.float32(2.0 * s * s))).astype(np.float32)
+
+
+def _lcg_stream(seed):
+    st = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        st = (st * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        yield np.float32(st >> 33) / np.float32(1 << 31)
+
+
+def texture(w, h, seed):
+    """features.rs:915-943: 400 random Gaussian blobs on 0.5."""
+    g = _lcg_stream(seed)
+    blobs = [[next(g) * np.float32(400.0), next(g) * np.float32(400.0),
+              np.float32(1.5) + next(g) * np.float32(5.0), next(g) - np.float32(0.5)]
+             for _ in range(400)]
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    px = x - np.float32(w / 2.0) + np.float32(200.0)
+    py = y - np.float32(h / 2.0) + np.float32(200.0)
+    v = np.full((h, w), 0.5, np.float32)
+    for bx, by, bs, ba in blobs:
+        d2 = (px - bx) ** 2 + (py - by) ** 2
+        m = d2 < np.float32(16.0) * bs * bs
+        v[m] += np.float32(0.5) * ba * np.exp(-d2[m] / (np.float32(2.0) * bs * bs))
+    return v
+
+
+_TEX_CACHE = {}
+
+
+def texture_warped(w, h, seed, th, sc):
+    """features.rs:946-966: the 800 x 800 texture rotated by th, scaled by sc."""
+    if seed not in _TEX_CACHE:
+        _TEX_CACHE[seed] = texture(800, 800, seed)
+    big = _TEX_CACHE[seed]
+    c, s = np.float32(np.cos(th)), np.float32(np.sin(th))
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    fx, fy = x - np.float32(w / 2.0), y - np.float32(h / 2.0)
+    ox = (c * fx + s * fy) / np.float32(sc) + np.float32(400.0)
+    oy = (-s * fx + c * fy) / np.float32(sc) + np.float32(400.0)
+    x0 = np.floor(ox).astype(np.int64)
+    y0 = np.floor(oy).astype(np.int64)
+    ax, ay = ox - x0, oy - y0
+
+    def at(xx, yy):
+        return big[np.minimum(yy, 799), np.minimum(xx, 799)]
+    return ((1 - ay) * ((1 - ax) * at(x0, y0) + ax * at(x0 + 1, y0))
+            + ay * ((1 - ax) * at(x0, y0 + 1) + ax * at(x0 + 1, y0 + 1))).astype(np.float32)
+
+
+def scene_texture(w, h, seed, n_blobs=None):
+    """A textured test image of any size (the reference's synthetic drone
+    renderer, synth.rs, is outside this path): random Gaussian blobs of radius
+    1.5-6.5 px at a density of one per 400 px^2 over a smooth gradient."""
+    rng = np.random.default_rng(seed)
+    n = n_blobs if n_blobs is not None else max(1, (w * h) // 400)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    v = (np.float32(0.4) + np.float32(0.2) * x / np.float32(max(w, 1))).astype(np.float32)
+    bx = rng.uniform(0, w, n).astype(np.float32)
+    by = rng.uniform(0, h, n).astype(np.float32)
+    bs = rng.uniform(1.5, 6.5, n).astype(np.float32)
+    ba = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    for i in range(n):
+        r = int(4 * bs[i]) + 1
+        x0, x1 = max(0, int(bx[i]) - r), min(w, int(bx[i]) + r + 1)
+        y0, y1 = max(0, int(by[i]) - r), min(h, int(by[i]) + r + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        d2 = (x[y0:y1, x0:x1] - bx[i]) ** 2 + (y[y0:y1, x0:x1] - by[i]) ** 2
+        v[y0:y1, x0:x1] += np.float32(0.5) * ba[i] * np.exp(-d2 / (np.float32(2.0) * bs[i] * bs[i]))
+    return v

@@ -83,6 +83,9 @@ def _c_layout():
                            "mutual", "precision", "run_ransac", "px_offset", "ransac"],
         "skh_pair_schedule": ["temporal", "pow2_max", "cross_kind", "range", "right_min",
                               "left_min", "max", "step"],
+        "skh_detector_cfg": ["octaves", "scales", "sigma0", "contrast", "edge_ratio",
+                             "upscale_fill", "max_features", "upscale_below"],
+        "skh_keypoint": ["x", "y", "sigma", "response", "angle"],
     }
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "skylens_hip.h"',
              "int main(void) {"]
@@ -109,7 +112,11 @@ def test_struct_layout_matches_c_header():
     mirrors = {"skh_ransac_cfg": skylens_hip.RansacConfig,
                "skh_model_sel": skylens_hip.ModelSelectionC,
                "skh_batch_desc": skylens_hip.BatchDesc,
-               "skh_pair_schedule": skylens_hip.PairSchedule}
+               "skh_pair_schedule": skylens_hip.PairSchedule,
+               "skh_detector_cfg": skylens_hip.DetectorConfig}
+    assert got["skh_keypoint"] == 4 * len(skylens_hip.KEYPOINT_FIELDS)
+    for i, f in enumerate(skylens_hip.KEYPOINT_FIELDS):
+        assert got[f"skh_keypoint.{f}"] == 4 * i
     for st, cls in mirrors.items():
         assert ctypes.sizeof(cls) == got[st], st
         for f in fields[st]:
