@@ -305,6 +305,82 @@ def single_pair_latency(ctx):
     return out, (a, b, x1, x2, fit)
 
 
+def detector_line(n_ctx=4, reps=12):
+    """detect_and_describe (features.rs:594-648) on 1920 x 1080 textured images
+    resident in HBM (the reference's timing case, features.rs:1196-1208, is
+    one 1920 x 1080 image in <= 0.4 s on the CPU): one context issuing images
+    back to back, and n_ctx contexts on their own streams and host threads,
+    one image each in flight."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import skylens_hip
+    import workloads
+
+    W, H = 1920, 1080
+    cfg = skylens_hip.DetectorConfig.default()
+    cap = int(cfg.max_features)
+    imgs = [workloads.textured_image(W, H, 31 + k) for k in range(n_ctx)]
+    d_imgs = [torch.from_numpy(i).cuda() for i in imgs]
+    ctxs = [skylens_hip.Context(0) for _ in range(n_ctx)]
+    d_kp = [torch.zeros((cap, 5), dtype=torch.float32, device="cuda") for _ in range(n_ctx)]
+    d_desc = [torch.zeros((cap, 128), dtype=torch.float32, device="cuda") for _ in range(n_ctx)]
+    torch.cuda.synchronize()
+
+    def run(k, times):
+        n = 0
+        for _ in range(times):
+            n = ctxs[k].detect_and_describe_dev(d_imgs[k].data_ptr(), W, H, cfg,
+                                                d_kp[k].data_ptr(), d_desc[k].data_ptr(), cap)
+        torch.cuda.synchronize()
+        return n
+
+    counts = [run(k, 2) for k in range(n_ctx)]  # warm-up, buffers grown
+    t = time.perf_counter()
+    run(0, reps)
+    one = (time.perf_counter() - t) / reps
+    ths = [threading.Thread(target=run, args=(k, reps)) for k in range(n_ctx)]
+    t = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    multi = time.perf_counter() - t
+    t = time.perf_counter()
+    hk, hd = ctxs[0].detect_and_describe(imgs[0], cfg)  # host image in, results out
+    host_ms = (time.perf_counter() - t) * 1e3
+    out = {"workload": f"detect_and_describe, {W} x {H} textured images, default DetectorConfig",
+           "features_per_image": counts, "ms_per_image_one_stream": round(one * 1e3, 3),
+           "images_per_s_one_stream": round(1.0 / one, 2),
+           "images_per_s": round(n_ctx * reps / multi, 2), "streams": n_ctx,
+           "ms_per_image_host_buffers": round(host_ms, 3)}
+    for c in ctxs:
+        c.close()
+    return out, (imgs[0], hk, hd)
+
+
+def detector_cpu(det):
+    """The detector oracle on the first image, single-threaded, and its
+    bit-for-bit agreement with the GPU's features."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    img, hk, hd = det
+    t = time.perf_counter()
+    ok, od = ob.detect_and_describe(img)
+    sec = time.perf_counter() - t
+    same = (ok.shape == hk.shape and np.array_equal(ok.view(np.uint32), hk.view(np.uint32))
+            and np.array_equal(od.view(np.uint32), hd.view(np.uint32)))
+    return {"value": round(1.0 / sec, 3), "unit": "images/s", "cores": 1, "kind": "port",
+            "sample": "one 1920 x 1080 image through the detector oracle on one host thread",
+            "ms_per_image": round(sec * 1e3, 1), "features": int(ok.shape[0]),
+            "identical_to_gpu": bool(same)}
+
+
 def cpu_baseline(batch, gpu_res, threads, seconds, single, fixed):
     """The oracle (C restatement of the reference's CPU path) on a sample of the
     headline batch's pairs: ratio_match then ransac_fundamental per pair, pairs
@@ -522,6 +598,7 @@ def main():
 
     fixed = None
     single = None
+    det = None
     if not args.no_secondary:
         sec = {}
         sec["full_scene"], fb, _ = secondary("full", ["bf16", "fp8"], ctxs[0])
@@ -530,6 +607,8 @@ def main():
         fixed = (xb, xres["fp8"])
         sec["single_pair_latency"], single = single_pair_latency(ctxs[0])
         log(f"single pair: {sec['single_pair_latency']}")
+        sec["detector"], det = detector_line()
+        log(f"detector: {sec['detector']}")
         extra["secondary"] = sec
 
     cpu = None
@@ -538,6 +617,9 @@ def main():
         if single is None:
             _, single = single_pair_latency(ctxs[0])
         cpu = cpu_baseline(batches[0], res[0], threads, args.cpu_seconds, single, fixed)
+        if det is not None:
+            cpu["detector"] = detector_cpu(det)
+            log(f"detector cpu: {cpu['detector']}")
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,

@@ -274,75 +274,101 @@ struct GradSet {
     int w[kMaxOctaves], h[kMaxOctaves];
 };
 
-// Dominant orientations (features.rs:361-413) of candidate order[c]; writes
-// one sort key per angle: (~|response| bits) << 32 | c << 5 | angle index,
-// and the angle's bits as the value.
-__global__ __launch_bounds__(128) void orient_kernel(const GradSet* __restrict__ G,
-                                                     Cand* __restrict__ cands,
-                                                     const uint32_t* __restrict__ order,
-                                                     uint32_t n, float sigma0, float kstep,
-                                                     uint64_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ nfeat) {
+// Dominant orientations (features.rs:361-413) of candidate order[c], one
+// wave per candidate. The window's pixels are taken 64 at a time in the
+// reference's raster order: each lane stages one pixel's bin and weighted
+// magnitude in LDS, then lane b < 36 (the owner of bin b) adds the staged
+// values of its bin in pixel order, which is the reference's summation order
+// for every bin. Writes one sort key per angle: (~|response| bits) << 32 |
+// c << 5 | angle index, and the angle's bits as the value.
+__global__ __launch_bounds__(64) void orient_kernel(const GradSet* __restrict__ G,
+                                                    Cand* __restrict__ cands,
+                                                    const uint32_t* __restrict__ order,
+                                                    uint32_t n, float sigma0, float kstep,
+                                                    uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ nfeat) {
     constexpr int NB = 36;
-    __shared__ float hist[NB][128];
-    const uint32_t c = blockIdx.x * 128 + threadIdx.x;
+    __shared__ int sbin[64];
+    __shared__ float sval[64];
+    __shared__ float hbuf[2][NB];
+    const uint32_t c = blockIdx.x;
+    const int lane = threadIdx.x;
     if (c >= n) return;
-    Cand& cd = cands[order[c]];
-    const float sig = sigma0 * fm::powf_pos(kstep, cd.s);
-    cd.sig = sig;
-    const int w = G->w[cd.octave], h = G->h[cd.octave];
-    const float2* g = G->g[cd.octave][cd.layer];
-    const float x = cd.x, y = cd.y;
+    Cand* cd = &cands[order[c]];
+    const float sig = sigma0 * fm::powf_pos(kstep, cd->s);
+    if (lane == 0) cd->sig = sig;
+    const int w = G->w[cd->octave], h = G->h[cd->octave];
+    const float2* g = G->g[cd->octave][cd->layer];
+    const float x = cd->x, y = cd->y;
     const float sw = 1.5f * sig;
     const int r = to_int(roundf(3.0f * sw));
     const int xi = to_int(roundf(x)), yi = to_int(roundf(y));
-    float* hs = &hist[0][threadIdx.x];
-    for (int i = 0; i < NB; ++i) hs[128 * i] = 0.0f;
-    for (int dy = -r; dy <= r; ++dy) {
-        const int py = yi + dy;
-        if (py < 1 || py >= h - 1) continue;
-        for (int dx = -r; dx <= r; ++dx) {
-            const int px = xi + dx;
-            if (px < 1 || px >= w - 1) continue;
-            const float2 mg = g[(size_t)py * w + px];
-            const float fx = (float)px - x, fy = (float)py - y;
-            const float wgt = fm::expf_cr(-(fx * fx + fy * fy) / (2.0f * sw * sw));
-            const float ang = rem_euclid(mg.y, kTau);
-            const int bin = to_int(roundf(ang / kTau * (float)NB)) % NB;
-            hs[128 * bin] = hs[128 * bin] + wgt * mg.x;
+    const int side = 2 * r + 1, npx = side * side;
+    float hv = 0.0f;  // bin `lane` (lanes < 36)
+    for (int p0 = 0; p0 < npx; p0 += 64) {
+        const int p = p0 + lane;
+        int bin = -1;
+        float val = 0.0f;
+        if (p < npx) {
+            const int px = xi + (p % side - r), py = yi + (p / side - r);
+            if (px >= 1 && py >= 1 && px < w - 1 && py < h - 1) {
+                const float2 mg = g[(size_t)py * w + px];
+                const float fx = (float)px - x, fy = (float)py - y;
+                const float wgt = fm::expf_cr(-(fx * fx + fy * fy) / (2.0f * sw * sw));
+                const float ang = rem_euclid(mg.y, kTau);
+                bin = to_int(roundf(ang / kTau * (float)NB)) % NB;
+                val = wgt * mg.x;
+            }
         }
+        const uint64_t valid = __ballot(bin >= 0);
+        if (valid == 0) continue;
+        const int cnt = __popcll(valid);
+        if (bin >= 0) {
+            const int slot = __popcll(valid & ((1ull << lane) - 1ull));
+            sbin[slot] = bin;
+            sval[slot] = val;
+        }
+        __syncthreads();
+        // +0.0 for the other bins leaves the (never -0.0) sums unchanged
+#pragma unroll 4
+        for (int j = 0; j < cnt; ++j) hv = hv + (sbin[j] == lane ? sval[j] : 0.0f);
+        __syncthreads();
     }
-    float hv[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) hv[i] = hs[128 * i];
-#pragma unroll
+    // six passes of [1, 1, 1] / 3, then the peaks
     for (int pass = 0; pass < 6; ++pass) {
-        float prev[NB];
-#pragma unroll
-        for (int i = 0; i < NB; ++i) prev[i] = hv[i];
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-            hv[i] = (prev[(i + NB - 1) % NB] + prev[i] + prev[(i + 1) % NB]) / 3.0f;
+        if (lane < NB) hbuf[0][lane] = hv;
+        __syncthreads();
+        if (lane < NB)
+            hv = (hbuf[0][(lane + NB - 1) % NB] + hbuf[0][lane] + hbuf[0][(lane + 1) % NB]) / 3.0f;
+        __syncthreads();
     }
+    if (lane < NB) hbuf[1][lane] = hv;
+    __syncthreads();
     float mx = 0.0f;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) mx = fmaxf(mx, hv[i]);
+    for (int i = 0; i < NB; ++i) mx = fmaxf(mx, hbuf[1][i]);
     if (mx <= 0.0f) return;
-    const float resp = fabsf(cd.value);
-    const uint64_t hi = (uint64_t)(0xffffffffu - __float_as_uint(resp)) << 32;
-    uint32_t a = 0;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        const float l = hv[(i + NB - 1) % NB], cc = hv[i], rr = hv[(i + 1) % NB];
+    bool peak = false;
+    float angle = 0.0f;
+    if (lane < NB) {
+        const float l = hbuf[1][(lane + NB - 1) % NB], cc = hv, rr = hbuf[1][(lane + 1) % NB];
         if (cc >= 0.8f * mx && cc > l && cc > rr) {
+            peak = true;
             const float off = 0.5f * (l - rr) / (l - 2.0f * cc + rr);
-            const float angle = rem_euclid(((float)i + off) / (float)NB * kTau, kTau);
-            const uint32_t slot = atomicAdd(nfeat, 1u);
-            keys[slot] = hi | ((uint64_t)c << 5) | a;
-            vals[slot] = __float_as_uint(angle);
-            ++a;
+            angle = rem_euclid(((float)lane + off) / (float)NB * kTau, kTau);
         }
+    }
+    const uint64_t pm = __ballot(peak);
+    if (pm == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(nfeat, (uint32_t)__popcll(pm));
+    base = __shfl(base, 0);
+    if (peak) {
+        const uint32_t a = (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+        const float resp = fabsf(cd->value);
+        const uint64_t hi = (uint64_t)(0xffffffffu - __float_as_uint(resp)) << 32;
+        keys[base + a] = hi | ((uint64_t)c << 5) | a;
+        vals[base + a] = __float_as_uint(angle);
     }
 }
 
@@ -378,13 +404,23 @@ __device__ __forceinline__ void feature_kp(const DescJob& J, uint32_t f, skh_key
     }
 }
 
-// describe_with (features.rs:521-592): one thread per kept feature, its
-// 128-bin histogram in LDS.
+// describe_with (features.rs:521-592), one wave per kept feature. The
+// window's pixels are taken 64 at a time in the reference's raster order:
+// each lane computes one pixel's weighted magnitude m and its trilinear cell
+// (v0, u0, o0 and the fractions) and stages them in LDS; then lane l, the
+// owner of bins 2l and 2l + 1 (spatial cell l / 4, orientations 2 (l % 4) and
+// 2 (l % 4) + 1), adds each staged pixel's contribution m * wv * wu * wo to
+// its bins, in pixel order -- every bin's sum has the reference's operands in
+// the reference's order.
 __global__ __launch_bounds__(64) void describe_kernel(DescJob J, skh_keypoint* __restrict__ kps,
                                                       float* __restrict__ desc) {
     constexpr int NC = 4, NO = 8;
-    __shared__ float hist[128][64];
-    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    __shared__ float4 sent[64];  // m, du, dv, dob of the staged pixels
+    __shared__ int spk[64];      // 1 | (v0 + 1) << 1 | (u0 + 1) << 4 | o0 << 7
+    __shared__ float hist[128];
+    __shared__ float snorm;
+    const uint32_t k = blockIdx.x;
+    const int lane = threadIdx.x;
     if (k >= J.n) return;
     const uint32_t f = J.sel ? J.sel[k] : k;
     skh_keypoint kp;
@@ -392,7 +428,7 @@ __global__ __launch_bounds__(64) void describe_kernel(DescJob J, skh_keypoint* _
     float angle;
     feature_kp(J, f, kp, cd, angle);
     const uint32_t o = J.out0 + k;
-    if (kps) kps[o] = kp;
+    if (kps && lane == 0) kps[o] = kp;
     if (!desc) return;
     const int w = J.G->w[cd->octave], h = J.G->h[cd->octave];
     const float2* g = J.G->g[cd->octave][cd->layer];
@@ -402,60 +438,96 @@ __global__ __launch_bounds__(64) void describe_kernel(DescJob J, skh_keypoint* _
     float sn, cs;
     fm::sincosf_cr(angle, &sn, &cs);
     const int xi = to_int(roundf(x)), yi = to_int(roundf(y));
-    float* hs = &hist[0][threadIdx.x];
-    for (int i = 0; i < 128; ++i) hs[64 * i] = 0.0f;
-    for (int dy = -r; dy <= r; ++dy) {
-        const int py = yi + dy;
-        if (py < 1 || py >= h - 1) continue;
-        for (int dx = -r; dx <= r; ++dx) {
-            const int px = xi + dx;
-            if (px < 1 || px >= w - 1) continue;
-            const float fx = (float)px - x, fy = (float)py - y;
-            const float u = (cs * fx + sn * fy) / cell;
-            const float v = (-sn * fx + cs * fy) / cell;
-            const float bu = u + (float)NC / 2.0f - 0.5f, bv = v + (float)NC / 2.0f - 0.5f;
-            if (bu <= -1.0f || bv <= -1.0f || bu >= (float)NC || bv >= (float)NC) continue;
-            const float2 mg = g[(size_t)py * w + px];
-            const float ori = rem_euclid(mg.y - angle, kTau);
-            const float bo = ori / kTau * (float)NO;
-            const float wgt = fm::expf_cr(-(u * u + v * v) / 8.0f);  // 2 * (NC/2)^2 = 8
-            const float m = mg.x * wgt;
-            const float u0 = floorf(bu), v0 = floorf(bv), o0 = floorf(bo);
-            const float du = bu - u0, dv = bv - v0, dob = bo - o0;
-            const int iv0 = to_int(v0), iu0 = to_int(u0);
-            const uint32_t io0 = (uint32_t)to_int(o0);
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int iv = iv0 + a;
-                const float wv = a ? dv : 1.0f - dv;
-                if (iv < 0 || iv >= NC) continue;
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const int iu = iu0 + b;
-                    const float wu = b ? du : 1.0f - du;
-                    if (iu < 0 || iu >= NC) continue;
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const uint32_t io = (io0 + (uint32_t)q) % NO;
-                        const float wo = q ? dob : 1.0f - dob;
-                        const int bin = (iv * NC + iu) * NO + (int)io;
-                        hs[64 * bin] = hs[64 * bin] + m * wv * wu * wo;
-                    }
+    const int side = 2 * r + 1, npx = side * side;
+    const int my_cell = lane >> 2, cv = my_cell >> 2, cu = my_cell & 3;
+    const int oa = 2 * (lane & 3), ob = oa + 1;
+    float ha = 0.0f, hb = 0.0f;  // bins 2 lane, 2 lane + 1
+    for (int p0 = 0; p0 < npx; p0 += 64) {
+        const int p = p0 + lane;
+        int pk = 0;
+        float m = 0.0f, du = 0.0f, dv = 0.0f, dob = 0.0f;
+        if (p < npx) {
+            const int px = xi + (p % side - r), py = yi + (p / side - r);
+            if (px >= 1 && py >= 1 && px < w - 1 && py < h - 1) {
+                const float fx = (float)px - x, fy = (float)py - y;
+                const float u = (cs * fx + sn * fy) / cell;
+                const float v = (-sn * fx + cs * fy) / cell;
+                const float bu = u + (float)NC / 2.0f - 0.5f, bv = v + (float)NC / 2.0f - 0.5f;
+                if (!(bu <= -1.0f || bv <= -1.0f || bu >= (float)NC || bv >= (float)NC)) {
+                    const float2 mg = g[(size_t)py * w + px];
+                    const float ori = rem_euclid(mg.y - angle, kTau);
+                    const float bo = ori / kTau * (float)NO;
+                    const float wgt = fm::expf_cr(-(u * u + v * v) / 8.0f);  // 2 (NC/2)^2 = 8
+                    m = mg.x * wgt;
+                    const float u0 = floorf(bu), v0 = floorf(bv), o0 = floorf(bo);
+                    du = bu - u0;
+                    dv = bv - v0;
+                    dob = bo - o0;
+                    pk = 1 | ((to_int(v0) + 1) << 1) | ((to_int(u0) + 1) << 4) | (to_int(o0) << 7);
                 }
             }
         }
+        // stage the contributing pixels only, compacted in pixel order
+        const uint64_t valid = __ballot(pk != 0);
+        if (valid == 0) continue;
+        const int cnt = __popcll(valid);
+        if (pk) {
+            const int slot = __popcll(valid & ((1ull << lane) - 1ull));
+            sent[slot] = make_float4(m, du, dv, dob);
+            spk[slot] = pk;
+        }
+        __syncthreads();
+        // Every entry is added to both bins, as the product of the reference's
+        // weights when the bin is one of the pixel's eight, else as +0.0: the
+        // partial sums are never -0.0 (every term is >= +0), so adding +0.0
+        // leaves them bit for bit unchanged.
+#pragma unroll 4
+        for (int j = 0; j < cnt; ++j) {
+            const int q = spk[j];
+            const float4 e = sent[j];
+            const int a = cv - (((q >> 1) & 7) - 1);  // 0: row v0 (weight 1 - dv), 1: v0 + 1
+            const int b = cu - (((q >> 4) & 7) - 1);
+            const bool cellin = (unsigned)a <= 1u && (unsigned)b <= 1u;
+            const int io0 = (q >> 7) & 7, io1 = ((q >> 7) + 1) & 7;
+            const float wv = a ? e.z : 1.0f - e.z;
+            const float wu = b ? e.y : 1.0f - e.y;
+            const float mvu = e.x * wv * wu;
+            const float woa = oa == io0 ? 1.0f - e.w : e.w;
+            const float wob = ob == io0 ? 1.0f - e.w : e.w;
+            const float ca = mvu * woa, cb = mvu * wob;
+            ha = ha + ((cellin && (oa == io0 || oa == io1)) ? ca : 0.0f);
+            hb = hb + ((cellin && (ob == io0 || ob == io1)) ? cb : 0.0f);
+        }
+        __syncthreads();
     }
-    float* out = desc + (size_t)o * 128;
+    // normalise (sum of squares in bin order), clip at 0.2, normalise again
+    hist[2 * lane] = ha;
+    hist[2 * lane + 1] = hb;
+    __syncthreads();
     for (int pass = 0; pass < 2; ++pass) {
-        float s = -0.0f;
-        for (int i = 0; i < 128; ++i) s = s + hs[64 * i] * hs[64 * i];
-        const float nrm = sqrtf(s);
-        if (nrm > 0.0f)
-            for (int i = 0; i < 128; ++i) hs[64 * i] = hs[64 * i] / nrm;
-        if (pass == 0)
-            for (int i = 0; i < 128; ++i) hs[64 * i] = fminf(hs[64 * i], 0.2f);
+        if (lane == 0) {
+            float s = -0.0f;
+            for (int i = 0; i < 128; ++i) s = s + hist[i] * hist[i];
+            snorm = sqrtf(s);
+        }
+        __syncthreads();
+        const float nrm = snorm;
+        float va = hist[2 * lane], vb = hist[2 * lane + 1];
+        if (nrm > 0.0f) {
+            va = va / nrm;
+            vb = vb / nrm;
+        }
+        if (pass == 0) {
+            va = fminf(va, 0.2f);
+            vb = fminf(vb, 0.2f);
+        }
+        __syncthreads();
+        hist[2 * lane] = va;
+        hist[2 * lane + 1] = vb;
+        __syncthreads();
     }
-    for (int i = 0; i < 128; ++i) out[i] = hs[64 * i];
+    float2* out = reinterpret_cast<float2*>(desc + (size_t)o * 128);
+    out[lane] = make_float2(hist[2 * lane], hist[2 * lane + 1]);
 }
 
 // Extra (upsampled) feature e duplicates an original one when it lies within
@@ -752,7 +824,7 @@ int detect_core(Ctx* ctx, Detector& D, Pyramid& P, const float* d_img, int w0, i
                                       ctx->stream),
                        "copy grad set")))
         return rc;
-    orient_kernel<<<(nc + 127) / 128, 128, 0, ctx->stream>>>(
+    orient_kernel<<<nc, 64, 0, ctx->stream>>>(
         (const GradSet*)P.gset.p, (Cand*)P.cands.p, (const uint32_t*)P.order.p, nc, cfg.sigma0,
         kstep, (uint64_t*)P.keys_in.p, (uint32_t*)P.vals_in.p, (uint32_t*)P.ncand.p + 1);
     if ((rc = launch_check(ctx, "orient_kernel"))) return rc;
@@ -813,8 +885,7 @@ int detect_dev(Ctx* ctx, const float* d_img, size_t w, size_t h, const skh_detec
             if ((rc = grow(ctx, D.orig_kp, sizeof(skh_keypoint) * n0, "detector keypoints")))
                 return rc;
             const DescJob J = job_of(P, n0, 0, nullptr, 0);
-            describe_kernel<<<(n0 + 63) / 64, 64, 0, ctx->stream>>>(
-                J, (skh_keypoint*)D.orig_kp.p, nullptr);
+            describe_kernel<<<n0, 64, 0, ctx->stream>>>(J, (skh_keypoint*)D.orig_kp.p, nullptr);
             if ((rc = launch_check(ctx, "describe_kernel (keypoints)"))) return rc;
         }
         if ((rc = grow(ctx, D.up, sizeof(float) * 4 * w * h, "detector upsample"))) return rc;
@@ -844,11 +915,11 @@ int detect_dev(Ctx* ctx, const float* d_img, size_t w, size_t h, const skh_detec
         return ctx->fail(SKH_EINVAL, "detector: %zu features exceed the capacity %zu", total, cap);
     if (n0 > 0) {
         const DescJob J = job_of(P, n0, 0, nullptr, 0);
-        describe_kernel<<<(n0 + 63) / 64, 64, 0, ctx->stream>>>(J, d_kps, d_desc);
+        describe_kernel<<<n0, 64, 0, ctx->stream>>>(J, d_kps, d_desc);
     }
     if (n_add > 0) {
         const DescJob J = job_of(U, n_add, n0, (const uint32_t*)D.sel.p, 1);
-        describe_kernel<<<(n_add + 63) / 64, 64, 0, ctx->stream>>>(J, d_kps, d_desc);
+        describe_kernel<<<n_add, 64, 0, ctx->stream>>>(J, d_kps, d_desc);
     }
     return launch_check(ctx, "describe_kernel");
 }

@@ -25,6 +25,8 @@ gathers `kp.x as f64` without +0.5).
     ratio in [10 %, 90 %] (o_v ~ U[0.051, 0.684]); the first 4000 of the 4047
     candidate pairs (48P - 177 has no P giving exactly 4000).
   - ``fixed_scene()``: 240 views with 8192 features each.
+* ``textured_image(w, h, seed)`` -- a grey image for the detector: random
+  Gaussian blobs (radius 1.5-6.5 px, one per 400 px^2) over a gentle gradient.
 """
 from __future__ import annotations
 
@@ -310,3 +312,27 @@ def uneven_scene(seed: int = 4, positions: int = 88, n_pairs: int = 4000) -> Sce
 
 def fixed_scene(seed: int = 3, positions: int = 80, features: int = 8192) -> Scene:
     return scene(seed, positions, np.full(3 * positions, features), 0.106, 0.368)
+
+
+def textured_image(w: int, h: int, seed: int, n_blobs: Optional[int] = None) -> np.ndarray:
+    """A (h, w) float32 detector test image: random Gaussian blobs of radius
+    1.5-6.5 px and contrast +-0.25 at one per 400 px^2 over a 0.4 -> 0.6
+    horizontal gradient (the reference renders a synthetic drone scene,
+    synth.rs, which is outside this path)."""
+    rng = np.random.default_rng(seed)
+    n = n_blobs if n_blobs is not None else max(1, (w * h) // 400)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    v = (np.float32(0.4) + np.float32(0.2) * x / np.float32(max(w, 1))).astype(np.float32)
+    bx = rng.uniform(0, w, n).astype(np.float32)
+    by = rng.uniform(0, h, n).astype(np.float32)
+    bs = rng.uniform(1.5, 6.5, n).astype(np.float32)
+    ba = rng.uniform(-0.5, 0.5, n).astype(np.float32)
+    for i in range(n):
+        r = int(4 * bs[i]) + 1
+        x0, x1 = max(0, int(bx[i]) - r), min(w, int(bx[i]) + r + 1)
+        y0, y1 = max(0, int(by[i]) - r), min(h, int(by[i]) + r + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        d2 = (x[y0:y1, x0:x1] - bx[i]) ** 2 + (y[y0:y1, x0:x1] - by[i]) ** 2
+        v[y0:y1, x0:x1] += np.float32(0.5) * ba[i] * np.exp(-d2 / (np.float32(2.0) * bs[i] * bs[i]))
+    return v

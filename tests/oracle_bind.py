@@ -584,23 +584,6 @@ def texture_warped(w, h, seed, th, sc):
 
 
 def scene_texture(w, h, seed, n_blobs=None):
-    """A textured test image of any size (the reference's synthetic drone
-    renderer, synth.rs, is outside this path): random Gaussian blobs of radius
-    1.5-6.5 px at a density of one per 400 px^2 over a smooth gradient."""
-    rng = np.random.default_rng(seed)
-    n = n_blobs if n_blobs is not None else max(1, (w * h) // 400)
-    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
-    v = (np.float32(0.4) + np.float32(0.2) * x / np.float32(max(w, 1))).astype(np.float32)
-    bx = rng.uniform(0, w, n).astype(np.float32)
-    by = rng.uniform(0, h, n).astype(np.float32)
-    bs = rng.uniform(1.5, 6.5, n).astype(np.float32)
-    ba = rng.uniform(-0.5, 0.5, n).astype(np.float32)
-    for i in range(n):
-        r = int(4 * bs[i]) + 1
-        x0, x1 = max(0, int(bx[i]) - r), min(w, int(bx[i]) + r + 1)
-        y0, y1 = max(0, int(by[i]) - r), min(h, int(by[i]) + r + 1)
-        if x0 >= x1 or y0 >= y1:
-            continue
-        d2 = (x[y0:y1, x0:x1] - bx[i]) ** 2 + (y[y0:y1, x0:x1] - by[i]) ** 2
-        v[y0:y1, x0:x1] += np.float32(0.5) * ba[i] * np.exp(-d2 / (np.float32(2.0) * bs[i] * bs[i]))
-    return v
+    """workloads.textured_image: the detector's textured test image."""
+    import workloads
+    return workloads.textured_image(w, h, seed, n_blobs)
