@@ -479,6 +479,9 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="build the first batch's plan, run it once, then launch the distance "
                          "kernel 3 times (for rocprofv3 counter runs); prints no JSON")
+    ap.add_argument("--detector-only", action="store_true",
+                    help="run only the detector line (1920 x 1080 images) and print it; for "
+                         "profiler runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run a plain device-to-device copy of BYTES three times (the reference "
                          "kernel for HBM counter calibration); prints no JSON")
@@ -501,6 +504,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copied {4 * n} bytes 3 times")
+        return
+
+    if args.detector_only:
+        out, _ = detector_line(n_ctx=args.streams)
+        print(json.dumps({"detector": out}), flush=True)
         return
 
     S = args.streams
