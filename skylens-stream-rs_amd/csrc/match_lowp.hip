@@ -981,21 +981,58 @@ __device__ void exact_row_top2_group(const LowpTask& t, const float (*ys)[kDescL
     __syncthreads();
 }
 
-// Exact column argmin (lowest row index among equal distances) of the m b rows in ys over all
-// of a; kNone when every distance is infinite.
-__device__ void exact_col_argmin_group(const LowpTask& t, const float (*ys)[kDescLen], int m,
-                                       unsigned long long (*red)[4], uint32_t* res) {
+// Exact column argmin (lowest row index among equal distances) of the m b rows in ys over
+// a; kNone when every distance is infinite. Only the row blocks that can hold the argmin are
+// scanned: with eps bounding |2 key - d^2| for every element of the pair, a row r can be the
+// exact minimum (or tie with it) of column j only if 2 key(r, j) <= 2 C1(j) + 2 eps, C1 the
+// column's smallest key, so a row block whose own smallest key for j (its column partial)
+// exceeds that bound holds no candidate for j; every row of an unscanned block is strictly
+// farther than the exact minimum. The blocks scanned are the union over the m columns
+// (scanning extra rows never changes an exact argmin). cols[r] is the column index of ys[r].
+__device__ void exact_col_argmin_group(const LowpTask& t, const LowpOut& out, double eps,
+                                       const uint32_t* cols, const float (*ys)[kDescLen], int m,
+                                       unsigned long long (*red)[4], unsigned long long* blk_mask,
+                                       uint32_t* res) {
+    const uint32_t bm = t.bm ? t.bm : 128u;
+    const uint32_t nblk = (t.na + bm - 1) / bm;
+    // candidate row blocks (all of them when the block size is unknown, there are more than
+    // 64 blocks or eps is not finite)
+    if (threadIdx.x < 4) blk_mask[threadIdx.x] = 0ull;
+    __syncthreads();
+    const bool all = t.bm == 0 || nblk > 64 || !(eps < INFINITY);
+    {
+        bool cand = false;
+        const uint32_t b = threadIdx.x;
+        if (!all && b < nblk) {
+            for (int r = 0; r < m; ++r) {
+                const uint32_t j = cols[r];
+                const double thr = 2.0 * (double)out.col_top2[t.col_base + j].x + 2.0 * eps;
+                const double kb = 2.0 * (double)out.col_part[t.part_base + (size_t)b * t.nb + j].x;
+                cand |= kb <= thr;
+            }
+        }
+        const unsigned long long bits = __ballot(cand);
+        if ((threadIdx.x & 63) == 0) blk_mask[threadIdx.x >> 6] = bits;
+    }
+    __syncthreads();
+    const unsigned long long mask =
+        all ? ~0ull : (blk_mask[0] | blk_mask[1] | blk_mask[2] | blk_mask[3]);
     unsigned long long best[kRescoreGroup];
 #pragma unroll
     for (int r = 0; r < kRescoreGroup; ++r) best[r] = kColEmpty;
-    for (uint32_t i = threadIdx.x; i < t.na; i += blockDim.x) {
-        float d[kRescoreGroup];
-        exact_dist_group(t.a + (size_t)i * kDescLen, ys, m, d);
+    for (uint32_t b = 0; b < nblk; ++b) {
+        if (!all && !((mask >> b) & 1ull)) continue;
+        const uint32_t i_end = min(t.na, (b + 1) * bm);
+        for (uint32_t i = b * bm + threadIdx.x; i < i_end; i += blockDim.x) {
+            float d[kRescoreGroup];
+            exact_dist_group(t.a + (size_t)i * kDescLen, ys, m, d);
 #pragma unroll
-        for (int r = 0; r < kRescoreGroup; ++r) {
-            if (d[r] < INFINITY) {
-                const unsigned long long key = ((unsigned long long)__float_as_uint(d[r]) << 32) | i;
-                best[r] = key < best[r] ? key : best[r];
+            for (int r = 0; r < kRescoreGroup; ++r) {
+                if (d[r] < INFINITY) {
+                    const unsigned long long key =
+                        ((unsigned long long)__float_as_uint(d[r]) << 32) | i;
+                    best[r] = key < best[r] ? key : best[r];
+                }
             }
         }
     }
@@ -1027,6 +1064,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
     __shared__ __attribute__((aligned(16))) float ys[kRescoreGroup][kDescLen];
     __shared__ Top2x red_t[kRescoreGroup][4];
     __shared__ unsigned long long red_c[kRescoreGroup][4];
+    __shared__ unsigned long long blk_mask[4];
     __shared__ Top2x row_res[kRescoreGroup];
     __shared__ uint32_t col_res[kRescoreGroup];
     const float r2 = ratio * ratio;
@@ -1120,13 +1158,17 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
                 }
             }
             if (mc > 0) {
+                uint32_t col_j[kRescoreGroup];
                 __syncthreads();
 #pragma unroll
-                for (int r = 0; r < kRescoreGroup; ++r)
+                for (int r = 0; r < kRescoreGroup; ++r) {
+                    if (col_first[r]) col_j[col_slot[r]] = j1[r];
                     if (col_first[r] && threadIdx.x < kDescLen)
                         ys[col_slot[r]][threadIdx.x] = t.b[(size_t)j1[r] * kDescLen + threadIdx.x];
+                }
                 __syncthreads();
-                exact_col_argmin_group(t, ys, mc, red_c, col_res);
+                exact_col_argmin_group(t, out, bad ? (double)INFINITY : eps, col_j, ys, mc, red_c,
+                                       blk_mask, col_res);
 #pragma unroll
                 for (int r = 0; r < kRescoreGroup; ++r)
                     if (col_slot[r] >= 0) accept[r] = col_res[col_slot[r]] == row_i[r];
