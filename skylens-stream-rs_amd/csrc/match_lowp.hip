@@ -102,14 +102,44 @@ __device__ __forceinline__ void atomic_max_pos(float* p, float v) {
 // under the padding key and inside the f32 accumulator's range.
 constexpr float kMaxSq = 1e28f;
 
-__device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* __restrict__ q,
-                             uint32_t* __restrict__ nrm, ViewStats* __restrict__ st);
+// A row's contribution to its view's bound statistics (all zero for a bad row
+// or a thread without a row).
+struct RowStat {
+    float norm, res, qnorm, sq;
+    uint32_t bad;
+};
+
+// Maxima and the bad flag of a wave's rows (every row of one view), then one
+// lane updates the view's statistics: one atomic per wave and statistic
+// instead of one per row (the maxima do not depend on the order).
+__device__ __forceinline__ void commit_stats(ViewStats* st, RowStat r) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        r.norm = fmaxf(r.norm, __shfl_xor(r.norm, off));
+        r.res = fmaxf(r.res, __shfl_xor(r.res, off));
+        r.qnorm = fmaxf(r.qnorm, __shfl_xor(r.qnorm, off));
+        r.sq = fmaxf(r.sq, __shfl_xor(r.sq, off));
+        r.bad |= (uint32_t)__shfl_xor((int)r.bad, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (r.bad) atomicOr(&st->bad, 1u);
+        if (r.norm > 0.f) atomic_max_pos(&st->max_norm, r.norm);
+        if (r.res > 0.f) atomic_max_pos(&st->max_res, r.res);
+        if (r.qnorm > 0.f) atomic_max_pos(&st->max_qnorm, r.qnorm);
+        if (r.sq > 0.f) atomic_max_pos(&st->max_sq, r.sq);
+    }
+}
+
+__device__ RowStat quantize_row(const float* __restrict__ x, uint32_t i,
+                                uint16_t* __restrict__ q, uint32_t* __restrict__ nrm);
 
 // One thread per row: bf16 quantisation, packed norm, bound statistics.
 __global__ void quantize_kernel(const float* __restrict__ x, uint32_t n, uint16_t* __restrict__ q,
                                 uint32_t* __restrict__ nrm, ViewStats* __restrict__ st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) quantize_row(x, i, q, nrm, st);
+    RowStat rs{0.f, 0.f, 0.f, 0.f, 0u};
+    if (i < n) rs = quantize_row(x, i, q, nrm);
+    commit_stats(st, rs);
 }
 
 // All views of a scene: blockIdx.x = view, rows [off[v], off[v+1]) in blocks of
@@ -119,11 +149,13 @@ __global__ void quantize_views_kernel(const float* __restrict__ x, const uint64_
                                       ViewStats* __restrict__ st) {
     const uint32_t v = blockIdx.x;
     const uint64_t r = off[v] + (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (r < off[v + 1]) quantize_row(x, (uint32_t)r, q, nrm, st + v);
+    RowStat rs{0.f, 0.f, 0.f, 0.f, 0u};
+    if (r < off[v + 1]) rs = quantize_row(x, (uint32_t)r, q, nrm);
+    commit_stats(st + v, rs);
 }
 
-__device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* __restrict__ q,
-                             uint32_t* __restrict__ nrm, ViewStats* __restrict__ st) {
+__device__ RowStat quantize_row(const float* __restrict__ x, uint32_t i,
+                                uint16_t* __restrict__ q, uint32_t* __restrict__ nrm) {
     const float* r = x + (size_t)i * kDescLen;
     float s = 0.f, res = 0.f, qq = 0.f;
     bool bad = false;
@@ -158,23 +190,18 @@ __device__ void quantize_row(const float* __restrict__ x, uint32_t i, uint16_t* 
         lo16 = 0;
     }
     nrm[i] = hi | (lo16 << 16);
-    if (bad) {
-        atomicOr(&st->bad, 1u);
-        return;
-    }
+    if (bad) return RowStat{0.f, 0.f, 0.f, 0.f, 1u};
     // |a|, |a - q| and |q| from f32 sums of squares: the sums carry a relative
     // error below 130 * 2^-24, covered by the 1.0001 factor.
-    atomic_max_pos(&st->max_norm, up(sqrtf(s) * 1.0001f));
-    atomic_max_pos(&st->max_res, up(sqrtf(res) * 1.0001f));
-    atomic_max_pos(&st->max_qnorm, up(sqrtf(qq) * 1.0001f));
-    atomic_max_pos(&st->max_sq, up(s * 1.0001f));
+    return RowStat{up(sqrtf(s) * 1.0001f), up(sqrtf(res) * 1.0001f), up(sqrtf(qq) * 1.0001f),
+                   up(s * 1.0001f), 0u};
 }
 
 // FP8 form of quantize_row: e4m3 bytes of a / 2^e, the row's E8M0 scale code
 // and the three-term fold of |a|^2, plus the same bound statistics in real
 // units (q(a) = e4m3 value * 2^e).
-__device__ void quantize_row_fp8(const float* __restrict__ x, uint32_t i, uint8_t* __restrict__ q,
-                                 Fp8Meta* __restrict__ meta, ViewStats* __restrict__ st) {
+__device__ RowStat quantize_row_fp8(const float* __restrict__ x, uint32_t i,
+                                    uint8_t* __restrict__ q, Fp8Meta* __restrict__ meta) {
     const float* r = x + (size_t)i * kDescLen;
     float s = 0.f, mx = 0.f;
     bool bad = false;
@@ -201,8 +228,7 @@ __device__ void quantize_row_fp8(const float* __restrict__ x, uint32_t i, uint8_
     if (bad) {
         for (int c = 0; c < kDescLen / 4; ++c) qo[c] = 0u;
         meta[i] = Fp8Meta{127u | (127u << 8), 0u};
-        atomicOr(&st->bad, 1u);
-        return;
+        return RowStat{0.f, 0.f, 0.f, 0.f, 1u};
     }
     float res = 0.f, qq = 0.f;
     for (int c = 0; c < kDescLen; c += 4) {
@@ -231,17 +257,17 @@ __device__ void quantize_row_fp8(const float* __restrict__ x, uint32_t i, uint8_
         fold = hi | (mid << 8) | (lo << 16);
     }
     meta[i] = Fp8Meta{(uint32_t)(e + 127) | ((uint32_t)(f + 127) << 8), fold};
-    atomic_max_pos(&st->max_norm, up(sqrtf(s) * 1.0001f));
-    atomic_max_pos(&st->max_res, up(sqrtf(res) * 1.0001f));
-    atomic_max_pos(&st->max_qnorm, up(sqrtf(qq) * 1.0001f));
-    atomic_max_pos(&st->max_sq, up(s * 1.0001f));
+    return RowStat{up(sqrtf(s) * 1.0001f), up(sqrtf(res) * 1.0001f), up(sqrtf(qq) * 1.0001f),
+                   up(s * 1.0001f), 0u};
 }
 
 __global__ void quantize_fp8_kernel(const float* __restrict__ x, uint32_t n,
                                     uint8_t* __restrict__ q, Fp8Meta* __restrict__ meta,
                                     ViewStats* __restrict__ st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) quantize_row_fp8(x, i, q, meta, st);
+    RowStat rs{0.f, 0.f, 0.f, 0.f, 0u};
+    if (i < n) rs = quantize_row_fp8(x, i, q, meta);
+    commit_stats(st, rs);
 }
 
 __global__ void quantize_views_fp8_kernel(const float* __restrict__ x,
@@ -250,7 +276,9 @@ __global__ void quantize_views_fp8_kernel(const float* __restrict__ x,
                                           ViewStats* __restrict__ st) {
     const uint32_t v = blockIdx.x;
     const uint64_t r = off[v] + (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (r < off[v + 1]) quantize_row_fp8(x, (uint32_t)r, q, meta, st + v);
+    RowStat rs{0.f, 0.f, 0.f, 0.f, 0u};
+    if (r < off[v + 1]) rs = quantize_row_fp8(x, (uint32_t)r, q, meta);
+    commit_stats(st + v, rs);
 }
 
 __global__ void stats_reset_kernel(ViewStats* st, uint32_t n) {

@@ -2094,24 +2094,36 @@ void ransac_probe_kernel(const RansacTask* __restrict__ tasks, const RansacCfgDe
 // Pair p's rank when the pairs are ordered by descending cost estimate
 // (ties by index): order[rank] = p. The costliest problems start first, so
 // the longest workgroups do not trail at the end of the launch.
+// Pairs in descending estimated cost, ties by index: pair i's rank is the number
+// of pairs ahead of it. A workgroup ranks 64 pairs; its four waves count over
+// interleaved quarters of the list (four independent counters per lane), then
+// the partial counts are added.
 __global__ __launch_bounds__(256) void ransac_order_kernel(const uint32_t* __restrict__ count,
                                                            uint32_t P, uint32_t* __restrict__ order) {
-    __shared__ uint32_t tile[1024];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ uint32_t part[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 64u + lane;
     const uint32_t ki = i < P ? count[i] : 0u;
-    uint32_t rank = 0;
-    for (uint32_t base = 0; base < P; base += 1024) {
-        for (uint32_t j = threadIdx.x; j < 1024; j += 256)
-            tile[j] = base + j < P ? count[base + j] : 0u;
-        __syncthreads();
-        const uint32_t e = P - base < 1024 ? P - base : 1024;
-        for (uint32_t j = 0; j < e; ++j) {
-            const uint32_t kj = tile[j];
-            rank += (kj > ki || (kj == ki && base + j < i)) ? 1u : 0u;
+    uint32_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    // groups of four consecutive pairs, group g to wave g mod 4
+    for (uint32_t j = 4u * wave; j < P; j += 16u) {
+        if (j + 3u < P) {
+            const uint32_t k0 = count[j], k1 = count[j + 1], k2 = count[j + 2], k3 = count[j + 3];
+            r0 += (k0 > ki || (k0 == ki && j < i)) ? 1u : 0u;
+            r1 += (k1 > ki || (k1 == ki && j + 1u < i)) ? 1u : 0u;
+            r2 += (k2 > ki || (k2 == ki && j + 2u < i)) ? 1u : 0u;
+            r3 += (k3 > ki || (k3 == ki && j + 3u < i)) ? 1u : 0u;
+        } else {
+            for (uint32_t jj = j; jj < P; ++jj) {
+                const uint32_t kj = count[jj];
+                r0 += (kj > ki || (kj == ki && jj < i)) ? 1u : 0u;
+            }
         }
-        __syncthreads();
     }
-    if (i < P) order[rank] = i;
+    part[wave][lane] = r0 + r1 + r2 + r3;
+    __syncthreads();
+    if (wave == 0 && i < P)
+        order[part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]] = i;
 }
 
 void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev& cfg,
@@ -2119,7 +2131,7 @@ void launch_ransac(const RansacTask* tasks, uint32_t n_tasks, const RansacCfgDev
     if (!n_tasks) return;
     if (est && order) {
         hipLaunchKernelGGL(ransac_probe_kernel, dim3(n_tasks), dim3(NL), 0, s, tasks, cfg, est);
-        hipLaunchKernelGGL(ransac_order_kernel, dim3((n_tasks + 255) / 256), dim3(256), 0, s,
+        hipLaunchKernelGGL(ransac_order_kernel, dim3((n_tasks + 63) / 64), dim3(256), 0, s,
                            (const uint32_t*)est, n_tasks, order);
     } else {
         order = nullptr;
