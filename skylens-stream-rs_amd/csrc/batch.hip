@@ -385,7 +385,7 @@ void Plan::enqueue_match(hipStream_t s) {
         launch_distance(s);
         launch_colmerge(lt, P, o, s);
         launch_decide(lt, P, o, desc.ratio, desc.mutual, tbits, s);
-        launch_rescore(lt, o, desc.ratio, desc.mutual, tbits, (uint32_t)std::max(64, 4 * ctx->num_cu),
+        launch_rescore(lt, o, desc.ratio, desc.mutual, tbits, (uint32_t)std::max(64, 16 * ctx->num_cu),
                        s);
     } else {
         launch_fill_u64(d_col_key.as<unsigned long long>(), total_cols, kColEmpty, s);

@@ -143,14 +143,90 @@ __global__ void quantize_kernel(const float* __restrict__ x, uint32_t n, uint16_
 }
 
 // All views of a scene: blockIdx.x = view, rows [off[v], off[v+1]) in blocks of
-// blockDim.x along y.
-__global__ void quantize_views_kernel(const float* __restrict__ x, const uint64_t* __restrict__ off,
-                                      uint16_t* __restrict__ q, uint32_t* __restrict__ nrm,
-                                      ViewStats* __restrict__ st) {
+// 256 along y, one thread per row. The rows are staged through LDS 32
+// components (one 128-byte line per row) at a time, read by eight
+// consecutive threads per row, and the bf16 rows leave the same way, so the
+// global accesses are whole lines; each thread still takes its row's
+// components in order (the same sums, bit for bit, as quantize_row).
+__global__ __launch_bounds__(256) void quantize_views_kernel(const float* __restrict__ x,
+                                                             const uint64_t* __restrict__ off,
+                                                             uint16_t* __restrict__ q,
+                                                             uint32_t* __restrict__ nrm,
+                                                             ViewStats* __restrict__ st) {
+    constexpr int CH = 32;  // components per stage
+    __shared__ float xs[256][CH + 1];
+    __shared__ uint32_t qs[256][CH / 2 + 1];
     const uint32_t v = blockIdx.x;
-    const uint64_t r = off[v] + (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    const uint64_t r0 = off[v] + (uint64_t)blockIdx.y * 256u;
+    const uint64_t r_end = off[v + 1];
+    if (r0 >= r_end) return;  // the whole workgroup: no row of this view here
+    const uint32_t nrows = (uint32_t)min((uint64_t)256, r_end - r0);
+    const uint32_t t = threadIdx.x;
+    const bool mine = t < nrows;
+    float s = 0.f, res = 0.f, qq = 0.f;
+    bool bad = false;
+    for (int c0 = 0; c0 < kDescLen; c0 += CH) {
+        // stage: element e = row * 8 + piece (8 float4 pieces per row)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t e = t + 256u * k, row = e >> 3, piece = e & 7u;
+            if (row < nrows) {
+                const float4 w = *reinterpret_cast<const float4*>(
+                    x + (size_t)(r0 + row) * kDescLen + c0 + 4 * piece);
+                xs[row][4 * piece] = w.x;
+                xs[row][4 * piece + 1] = w.y;
+                xs[row][4 * piece + 2] = w.z;
+                xs[row][4 * piece + 3] = w.w;
+            }
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+            for (int c = 0; c < CH; c += 2) {
+                uint32_t h[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float val = xs[t][c + e];
+                    bad |= !(fabsf(val) <= 1e15f);  // NaN, inf, or large enough to overflow
+                    h[e] = bf16_rne(val);
+                    const float qf = bf16_to_f(h[e]);
+                    const float d = val - qf;  // exact: a rounding residual is representable
+                    s = s + val * val;
+                    res = res + d * d;
+                    qq = qq + qf * qf;
+                }
+                qs[t][c / 2] = h[0] | (h[1] << 16);
+            }
+        }
+        __syncthreads();
+        // write back: each row's 64 bytes of this stage as four 16-byte pieces
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t e = t + 256u * k, row = e >> 2, piece = e & 3u;
+            if (row < nrows) {
+                const uint4 w = make_uint4(qs[row][4 * piece], qs[row][4 * piece + 1],
+                                           qs[row][4 * piece + 2], qs[row][4 * piece + 3]);
+                *reinterpret_cast<uint4*>(q + (size_t)(r0 + row) * kDescLen + c0 + 8 * piece) = w;
+            }
+        }
+    }
     RowStat rs{0.f, 0.f, 0.f, 0.f, 0u};
-    if (r < off[v + 1]) rs = quantize_row(x, (uint32_t)r, q, nrm);
+    if (mine) {
+        // the tail of quantize_row: norm split, bad rows, statistics
+        const uint32_t i = (uint32_t)(r0 + t);
+        bad |= !(s <= kMaxSq);
+        uint32_t hi = bf16_rne(s);
+        const float lo = s - bf16_to_f(hi);
+        uint32_t lo16 = bf16_rne(lo);
+        if (bad) {
+            hi = 0;
+            lo16 = 0;
+        }
+        nrm[i] = hi | (lo16 << 16);
+        rs = bad ? RowStat{0.f, 0.f, 0.f, 0.f, 1u}
+                 : RowStat{up(sqrtf(s) * 1.0001f), up(sqrtf(res) * 1.0001f),
+                           up(sqrtf(qq) * 1.0001f), up(s * 1.0001f), 0u};
+    }
     commit_stats(st + v, rs);
 }
 
