@@ -1006,7 +1006,7 @@ constexpr int kRescoreGroup = 8;  // queued rows of one pair re-scored together
 // its negation square to the same value, so x may be either side.
 __device__ __forceinline__ void exact_dist_group(const float* __restrict__ x,
                                                  const float (*ys)[kDescLen], int m,
-                                                 float* d) {
+                                                 float* d, uint32_t need = 0xffu) {
     float acc[kRescoreGroup][8];
 #pragma unroll
     for (int r = 0; r < kRescoreGroup; ++r)
@@ -1018,7 +1018,7 @@ __device__ __forceinline__ void exact_dist_group(const float* __restrict__ x,
         const float4 x1 = *reinterpret_cast<const float4*>(x + c + 4);
 #pragma unroll
         for (int r = 0; r < kRescoreGroup; ++r) {
-            if (r < m) {
+            if (r < m && ((need >> r) & 1u)) {
                 const float4 y0 = *reinterpret_cast<const float4*>(&ys[r][c]);
                 const float4 y1 = *reinterpret_cast<const float4*>(&ys[r][c + 4]);
                 float e;
@@ -1095,44 +1095,48 @@ __device__ void exact_row_top2_group(const LowpTask& t, const float (*ys)[kDescL
 // (scanning extra rows never changes an exact argmin). cols[r] is the column index of ys[r].
 __device__ void exact_col_argmin_group(const LowpTask& t, const LowpOut& out, double eps,
                                        const uint32_t* cols, const float (*ys)[kDescLen], int m,
-                                       unsigned long long (*red)[4], unsigned long long* blk_mask,
-                                       uint32_t* res) {
+                                       unsigned long long (*red)[4],
+                                       unsigned long long (*blk_mask)[4], uint32_t* res) {
     const uint32_t bm = t.bm ? t.bm : 128u;
     const uint32_t nblk = (t.na + bm - 1) / bm;
-    // candidate row blocks (all of them when the block size is unknown, there are more than
-    // 64 blocks or eps is not finite)
-    if (threadIdx.x < 4) blk_mask[threadIdx.x] = 0ull;
-    __syncthreads();
+    // candidate row blocks of each column (all of them when the block size is unknown,
+    // there are more than 64 blocks or eps is not finite)
     const bool all = t.bm == 0 || nblk > 64 || !(eps < INFINITY);
-    {
+    for (int r = 0; r < m; ++r) {
         bool cand = false;
         const uint32_t b = threadIdx.x;
         if (!all && b < nblk) {
-            for (int r = 0; r < m; ++r) {
-                const uint32_t j = cols[r];
-                const double thr = 2.0 * (double)out.col_top2[t.col_base + j].x + 2.0 * eps;
-                const double kb = 2.0 * (double)out.col_part[t.part_base + (size_t)b * t.nb + j].x;
-                cand |= kb <= thr;
-            }
+            const uint32_t j = cols[r];
+            const double thr = 2.0 * (double)out.col_top2[t.col_base + j].x + 2.0 * eps;
+            const double kb = 2.0 * (double)out.col_part[t.part_base + (size_t)b * t.nb + j].x;
+            cand = kb <= thr;
         }
         const unsigned long long bits = __ballot(cand);
-        if ((threadIdx.x & 63) == 0) blk_mask[threadIdx.x >> 6] = bits;
+        if ((threadIdx.x & 63) == 0) blk_mask[r][threadIdx.x >> 6] = bits;
     }
     __syncthreads();
-    const unsigned long long mask =
-        all ? ~0ull : (blk_mask[0] | blk_mask[1] | blk_mask[2] | blk_mask[3]);
+    unsigned long long mask[kRescoreGroup];
+#pragma unroll
+    for (int r = 0; r < kRescoreGroup; ++r)
+        mask[r] = r >= m ? 0ull
+                         : (all ? ~0ull
+                                : (blk_mask[r][0] | blk_mask[r][1] | blk_mask[r][2] | blk_mask[r][3]));
     unsigned long long best[kRescoreGroup];
 #pragma unroll
     for (int r = 0; r < kRescoreGroup; ++r) best[r] = kColEmpty;
     for (uint32_t b = 0; b < nblk; ++b) {
-        if (!all && !((mask >> b) & 1ull)) continue;
+        // the columns that need this block (uniform across the workgroup)
+        uint32_t need = 0;
+#pragma unroll
+        for (int r = 0; r < kRescoreGroup; ++r) need |= (uint32_t)((mask[r] >> (b & 63u)) & 1ull) << r;
+        if (!need) continue;
         const uint32_t i_end = min(t.na, (b + 1) * bm);
         for (uint32_t i = b * bm + threadIdx.x; i < i_end; i += blockDim.x) {
             float d[kRescoreGroup];
-            exact_dist_group(t.a + (size_t)i * kDescLen, ys, m, d);
+            exact_dist_group(t.a + (size_t)i * kDescLen, ys, m, d, need);
 #pragma unroll
             for (int r = 0; r < kRescoreGroup; ++r) {
-                if (d[r] < INFINITY) {
+                if (((need >> r) & 1u) && d[r] < INFINITY) {
                     const unsigned long long key =
                         ((unsigned long long)__float_as_uint(d[r]) << 32) | i;
                     best[r] = key < best[r] ? key : best[r];
@@ -1168,7 +1172,7 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
     __shared__ __attribute__((aligned(16))) float ys[kRescoreGroup][kDescLen];
     __shared__ Top2x red_t[kRescoreGroup][4];
     __shared__ unsigned long long red_c[kRescoreGroup][4];
-    __shared__ unsigned long long blk_mask[4];
+    __shared__ unsigned long long blk_mask[kRescoreGroup][4];
     __shared__ Top2x row_res[kRescoreGroup];
     __shared__ uint32_t col_res[kRescoreGroup];
     const float r2 = ratio * ratio;
