@@ -73,6 +73,7 @@ struct Plan {
     int tile = kTile256x64;  // the fixed tile, or the tuned plan's most used one
     int forced_tile = -1;    // set by the tuner: every pair on this tile
     uint32_t tbits = 1;
+    uint32_t tbits_g = 1;  // FP8 plans: tile bits of the BF16 re-rank (64-column chunks)
     // Row blocks of the ranking launch, grouped by tile: group g holds the
     // blocks [grp_off[g], grp_off[g] + grp_n[g]) of d_blocks (the exact plan
     // uses group 0). The groups launch one after another on the plan's
@@ -94,6 +95,8 @@ struct Plan {
         d_kp_b;
     DevBuf d_x1, d_x2, d_inl, d_F, d_found, d_sel, d_scratch, d_rtasks, d_iters, d_prof, d_order,
         d_est, d_rqueue, d_ver_ij, d_ver_len;
+    // FP8 plans: the BF16 form of the views and tasks for the re-rank of open rows
+    DevBuf d_stats_b, d_tasks_b;
 
     hipGraph_t g_match = nullptr, g_ransac = nullptr;
     hipGraphExec_t x_match = nullptr, x_ransac = nullptr;
@@ -110,7 +113,7 @@ struct Plan {
                 &d_match_ij, &d_match_len,   &d_match_base, &d_kp_a,   &d_kp_b,   &d_x1,
                 &d_x2,       &d_inl,         &d_F,          &d_found,  &d_sel,    &d_scratch,
                 &d_rtasks,   &d_iters,       &d_prof,      &d_order,  &d_est,    &d_rqueue,
-                &d_q8,       &d_meta,        &d_ver_ij,     &d_ver_len};
+                &d_q8,       &d_meta,        &d_ver_ij,     &d_ver_len, &d_stats_b, &d_tasks_b};
     }
 
     ~Plan() {
@@ -212,7 +215,8 @@ int Plan::build() {
         if ((rc = alloc(d_q8, kDescLen * total_feat, "q8")) ||
             (rc = alloc(d_meta, sizeof(Fp8Meta) * total_feat, "meta")))
             return rc;
-    } else if (lowp) {
+    }
+    if (lowp) {  // FP8 plans keep the BF16 form too, for the re-rank of open rows
         if ((rc = alloc(d_q, 2ull * kDescLen * total_feat, "q")) ||
             (rc = alloc(d_nrm, 4ull * total_feat, "nrm")))
             return rc;
@@ -236,6 +240,26 @@ int Plan::build() {
             lt[p].sa = d_stats.as<ViewStats>() + i;
             lt[p].sb = d_stats.as<ViewStats>() + j;
         }
+    }
+    if (fp8) {
+        // the BF16 tasks of the re-rank: same pairs and bases, bf16 rows, norms, statistics
+        if ((rc = alloc(d_stats_b, sizeof(ViewStats) * V, "stats_b"))) return rc;
+        std::vector<LowpTask> ltb(lt);
+        for (uint32_t p = 0; p < P; ++p) {
+            const uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+            LowpTask& l = ltb[p];
+            l.q8a = l.q8b = nullptr;
+            l.meta_a = l.meta_b = nullptr;
+            l.fp8 = 0;
+            l.qa = d_q.as<uint16_t>() + view_off[i] * kDescLen;
+            l.qb = d_q.as<uint16_t>() + view_off[j] * kDescLen;
+            l.nrm_a = d_nrm.as<uint32_t>() + view_off[i];
+            l.nrm_b = d_nrm.as<uint32_t>() + view_off[j];
+            l.sa = d_stats_b.as<ViewStats>() + i;
+            l.sb = d_stats_b.as<ViewStats>() + j;
+        }
+        if ((rc = upload(d_tasks_b, ltb.data(), sizeof(LowpTask) * P, "tasks_b"))) return rc;
+        tbits_g = tile_bits(max_view_rows, 64);
     }
 
     // XCD-aware, longest-first row-block order, one list per tile group
@@ -373,11 +397,15 @@ void Plan::enqueue_match(hipStream_t s) {
         o.chunks = fp8 ? d_chunks.as<uint2>() : nullptr;
         launch_stats_reset(d_stats.as<ViewStats>(), desc.n_views, s);
         launch_fill_u32(o.queue_len, 2, 0u, s);
-        if (fp8)
+        if (fp8) {
             launch_quantize_views_fp8(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
                                       max_view_rows, d_q8.as<uint8_t>(), d_meta.as<Fp8Meta>(),
                                       d_stats.as<ViewStats>(), s);
-        else
+            launch_stats_reset(d_stats_b.as<ViewStats>(), desc.n_views, s);
+            launch_quantize_views(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
+                                  max_view_rows, d_q.as<uint16_t>(), d_nrm.as<uint32_t>(),
+                                  d_stats_b.as<ViewStats>(), s);
+        } else
             launch_quantize_views(desc.d_desc, d_view_off.as<uint64_t>(), desc.n_views,
                                   max_view_rows, d_q.as<uint16_t>(), d_nrm.as<uint32_t>(),
                                   d_stats.as<ViewStats>(), s);
@@ -385,6 +413,9 @@ void Plan::enqueue_match(hipStream_t s) {
         launch_distance(s);
         launch_colmerge(lt, P, o, s);
         launch_decide(lt, P, o, desc.ratio, desc.mutual, tbits, s);
+        if (fp8)
+            launch_rerank_bf16(d_tasks_b.as<LowpTask>(), lt, o, desc.ratio, desc.mutual, tbits,
+                               tbits_g, (uint32_t)std::max(64, 8 * ctx->num_cu), s);
         launch_rescore(lt, o, desc.ratio, desc.mutual, tbits, (uint32_t)std::max(64, 16 * ctx->num_cu),
                        s);
     } else {

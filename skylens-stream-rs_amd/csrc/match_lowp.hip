@@ -943,7 +943,10 @@ __global__ __launch_bounds__(kExactNT) void exact_rows_kernel(const LowpTask* __
         __syncthreads();  // the previous chunk's readers of As / slot_of are done
         if (tid < 64) {  // one wave compacts the chunk's row-scan entries
             bool scan = false;
-            if ((uint32_t)tid < ck.y) scan = out.queue[k0 + tid].z == kNone;
+            if ((uint32_t)tid < ck.y) {
+                const uint4 e = out.queue[k0 + tid];
+                scan = e.z == kNone && e.w == 0u;  // w != 0: settled by the BF16 re-rank
+            }
             const unsigned long long m = __ballot(scan);
             if (scan) slot_of[__popcll(m & ((1ull << tid) - 1ull))] = k0 + tid;
             if (tid == 0) n_rows = __popcll(m);
@@ -1163,6 +1166,189 @@ __device__ void exact_col_argmin_group(const LowpTask& t, const LowpOut& out, do
     __syncthreads();
 }
 
+// BF16 re-rank of the FP8 path's open row decisions. The FP8 ranking's error
+// bound is about 15 times BF16's, so it leaves ~10x more rows undecided; each
+// would cost an exact FP32 scan of all of b. Instead, per chunk (up to 64 queue
+// entries of one pair), the entries that still need a row scan are ranked again
+// against all of b with the BF16 MFMA form of mfma_rank_kernel (K = 144 with
+// the norm fold, four waves = 2 row tiles x 2 column tiles of each 64-column
+// chunk, row top-2 only) and decided with the BF16 pair bound eps_b:
+//   certain reject (lo1 >= r^2 hi2)                          -> entry.w = 1;
+//   certain accept with j1 certain, not mutual                -> z = j1, w = 2;
+//   mutual: the FP8 column state (c1, c2 of j1, bound eps_8) decides it when
+//     2 c2 - eps_8 > e1 + eps_b (row i is the column's unique exact winner: any
+//     other row's exact distance is at least 2 c2 - eps_8, except the row
+//     holding c1, which must then be i) -> z = j1, w = 2, or when
+//     e1 - eps_b > 2 c1 + eps_8 (a closer row exists)      -> w = 1;
+//     otherwise z = j1, w = 0: the exact column check of rescore_kernel;
+//   anything else stays a row scan (exact_rows_kernel / rescore_kernel).
+// Every decision follows from bounds that hold for every element of the pair,
+// so the match list is the exact FP32 one whatever this kernel settles.
+__global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __restrict__ tasks_b,
+                                                          const LowpTask* __restrict__ tasks_8,
+                                                          LowpOut out, float ratio, int mutual,
+                                                          uint32_t tbits_8, uint32_t tbits_g) {
+    using Ops = RankOps<false>;
+    constexpr int STRIDE = Ops::kStride;
+    constexpr int PIECES = Ops::kRowBytes / 16;  // 16 per bf16 row
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[64 * STRIDE];
+    __shared__ uint32_t Bn[64];
+    __shared__ uint32_t slot_of[64];
+    __shared__ uint32_t n_rows;
+    __shared__ float mk1[2][32], mk2[2][32];  // the column-tile-1 waves' row results
+    __shared__ uint32_t mj[2][32];
+    const uint32_t n_chunks = out.queue_len[1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l32 = lane & 31;
+    const bool lo_half = lane < 32;
+    const int rt = wave >> 1, ct = wave & 1;  // this wave's row tile and column tile
+    const uint32_t tmask = (1u << tbits_g) - 1u;
+    const float r2 = ratio * ratio;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint2 ck = out.chunks[c];
+        const uint32_t k0 = ck.x;
+        __syncthreads();  // the previous chunk's readers of slot_of / Bs are done
+        if (tid < 64) {
+            bool scan = false;
+            if ((uint32_t)tid < ck.y) {
+                const uint4 e = out.queue[k0 + tid];
+                scan = e.z == kNone && e.w == 0u;
+            }
+            const unsigned long long m = __ballot(scan);
+            if (scan) slot_of[__popcll(m & ((1ull << tid) - 1ull))] = k0 + tid;
+            if (tid == 0) n_rows = __popcll(m);
+        }
+        __syncthreads();
+        const uint32_t rows = n_rows;
+        if (rows == 0) continue;  // uniform
+        const uint32_t pair = out.queue[k0].x;
+        const LowpTask t = tasks_b[pair];
+        const LowpTask t8 = tasks_8[pair];
+        if ((t.sa->bad | t.sb->bad | t8.sa->bad | t8.sb->bad) != 0) continue;  // exact path
+        // a fragments of this wave's row tile (gathered rows, padding beyond `rows`)
+        const uint32_t lr = (uint32_t)(rt * 32 + l32);
+        const bool valid = lr < rows;
+        const uint32_t arow = valid ? out.queue[slot_of[lr]].y : 0u;
+        typename Ops::AFrag afrag;
+        Ops::load_a(afrag, Ops::rows_a(t) + (size_t)arow * Ops::kRowBytes, valid,
+                    valid ? Ops::meta_a(t, arow) : Ops::pad_meta(), lo_half);
+        float m1[16], m2[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            m1[r] = INFINITY;
+            m2[r] = INFINITY;
+        }
+        uint32_t keep_v;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(keep_v) : "s"(~tmask));
+        const uint8_t* qb = Ops::rows_b(t);
+        for (uint32_t col0 = 0; col0 < t.nb; col0 += 64) {
+            __syncthreads();  // the previous chunk's fragments are read
+#pragma unroll
+            for (int k = 0; k < 64 * PIECES / 256; ++k) {
+                const int e = tid + 256 * k, cc = e / PIECES, seg = e % PIECES;
+                const uint32_t col = col0 + cc;
+                const uint4 v = col < t.nb ? *reinterpret_cast<const uint4*>(
+                                                 qb + (size_t)col * Ops::kRowBytes + 16 * seg)
+                                           : make_uint4(0, 0, 0, 0);
+                *reinterpret_cast<uint4*>(&Bs[cc * STRIDE + 16 * seg]) = v;
+            }
+            if (tid < 64) Bn[tid] = col0 + tid < t.nb ? Ops::meta_b(t, col0 + tid) : Ops::pad_meta();
+            __syncthreads();
+            typename Ops::BFrag bfrag;
+            Ops::load_b(bfrag, &Bs[(ct * 32 + l32) * STRIDE], Bn[ct * 32 + l32], lo_half);
+            f32x16 acc = f32x16{};
+#pragma unroll
+            for (int st = 0; st < Ops::kSteps; ++st) acc = Ops::mfma(afrag, bfrag, st, acc);
+            const uint32_t tile = ((col0 >> 5) + (uint32_t)ct) & tmask;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float pk = __uint_as_float((__float_as_uint(acc[r]) & keep_v) | tile);
+                top2_neg(m1[r], m2[r], pk);
+            }
+        }
+        // resolve each register's row over the 32 lanes of a half, then the two
+        // column tiles (waves ct = 0, 1 of the same row tile)
+        float rk1[16], rk2[16];
+        uint32_t rj[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float k1 = m1[r], k2 = m2[r];
+            const uint32_t tl = __float_as_uint(-k1) & tmask;
+            uint32_t j = k1 < INFINITY ? (tl * 32u + l32) : kNone;
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const float ok1 = __shfl_xor(k1, off);
+                const float ok2 = __shfl_xor(k2, off);
+                const uint32_t oj = __shfl_xor(j, off);
+                const bool other = (ok1 < k1) || (ok1 == k1 && oj < j);
+                const float n2 = other ? fminf(k1, ok2) : fminf(ok1, k2);
+                if (other) {
+                    k1 = ok1;
+                    j = oj;
+                }
+                k2 = n2;
+            }
+            rk1[r] = k1;
+            rk2[r] = k2;
+            rj[r] = j;
+        }
+        __syncthreads();
+        if (ct == 1 && l32 == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                mk1[rt][row] = rk1[r];
+                mk2[rt][row] = rk2[r];
+                mj[rt][row] = rj[r];
+            }
+        }
+        __syncthreads();
+        if (ct == 0 && l32 == 0) {
+            const double eps_b = pair_eps(t, tbits_g);
+            const double eps_8 = pair_eps(t8, tbits_8);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const uint32_t g = (uint32_t)(rt * 32 + row);
+                if (g >= rows) continue;
+                // merge with the other column tile (same rule as the lane merge)
+                float k1 = rk1[r], k2 = rk2[r];
+                uint32_t j = rj[r];
+                const float ok1 = mk1[rt][row], ok2 = mk2[rt][row];
+                const uint32_t oj = mj[rt][row];
+                const bool other = (ok1 < k1) || (ok1 == k1 && oj < j);
+                const float n2 = other ? fminf(k1, ok2) : fminf(ok1, k2);
+                if (other) {
+                    k1 = ok1;
+                    j = oj;
+                }
+                k2 = n2;
+                const double e1 = 2.0 * (double)k1, e2 = 2.0 * (double)k2;
+                const bool j1_certain = (e2 - e1) > 2.0 * eps_b && j < t.nb;
+                const float lo1 = __double2float_rd(e1 - eps_b), hi1 = __double2float_ru(e1 + eps_b);
+                const float lo2 = __double2float_rd(e2 - eps_b), hi2 = __double2float_ru(e2 + eps_b);
+                const uint32_t slot = slot_of[g];
+                if (lo1 >= r2 * hi2) {
+                    out.queue[slot].w = 1u;  // certain reject
+                } else if (hi1 < r2 * fmaxf(lo2, 0.0f) && j1_certain) {
+                    uint32_t w = 2u;
+                    if (mutual) {
+                        const float2 cc = out.col_top2[t8.col_base + j];
+                        const double c1 = 2.0 * (double)cc.x, c2 = 2.0 * (double)cc.y;
+                        if (c2 - eps_8 > e1 + eps_b)
+                            w = 2u;
+                        else if (e1 - eps_b > c1 + eps_8)
+                            w = 1u;
+                        else
+                            w = 0u;  // the exact column check decides
+                    }
+                    out.queue[slot].z = j;
+                    out.queue[slot].w = w;
+                }
+            }
+        }
+    }
+}
+
 // Queued rows, kRescoreGroup consecutive queue entries per step; consecutive entries of one
 // pair (the decide kernel reserves a wave's entries together) share each streamed row of b
 // (row scans) or of a (column scans).
@@ -1195,21 +1381,25 @@ __global__ __launch_bounds__(256) void rescore_kernel(const LowpTask* __restrict
             // A row scan's slot in ys / row_res is the count of row scans below it.
             uint32_t row_i[kRescoreGroup], j1[kRescoreGroup];
             bool accept[kRescoreGroup], col_needed[kRescoreGroup];
-            uint32_t known_m = 0, pre_m = 0, scan_m = 0;
+            uint32_t known_m = 0, pre_m = 0, scan_m = 0, done_m = 0;
 #pragma unroll
             for (int r = 0; r < kRescoreGroup; ++r) {
                 row_i[r] = kNone;
                 j1[r] = kNone;
+                bool settled_accept = false;
                 if (r < m) {
                     const uint4 item = out.queue[k0 + r];
                     row_i[r] = item.y;
                     j1[r] = item.z;
-                    if (item.z != kNone) known_m |= 1u << r;
+                    if (item.w != 0u) {  // settled by the BF16 re-rank: 1 reject, 2 accept
+                        done_m |= 1u << r;
+                        settled_accept = item.w == 2u;
+                    } else if (item.z != kNone) known_m |= 1u << r;
                     else if (out.chunks && out.pre[k0 + r].w) pre_m |= 1u << r;
                     else scan_m |= 1u << r;
                 }
-                accept[r] = (known_m >> r) & 1u;
-                col_needed[r] = r < m;
+                accept[r] = ((known_m >> r) & 1u) || settled_accept;
+                col_needed[r] = r < m && !((done_m >> r) & 1u);
             }
             const int mr = __popc(scan_m);
             // row scans of the entries that need one
@@ -1378,6 +1568,14 @@ void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, 
     if (!n_pairs) return;
     hipLaunchKernelGGL(decide_kernel, dim3(n_pairs), dim3(256), 0, s, tasks, out, ratio, mutual,
                        tbits);
+}
+
+void launch_rerank_bf16(const LowpTask* tasks_b, const LowpTask* tasks_8, const LowpOut& out,
+                        float ratio, int mutual, uint32_t tbits_8, uint32_t tbits_g,
+                        uint32_t grid, hipStream_t s) {
+    if (!out.chunks) return;
+    hipLaunchKernelGGL(rerank_bf16_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, tasks_b,
+                       tasks_8, out, ratio, mutual, tbits_8, tbits_g);
 }
 
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,

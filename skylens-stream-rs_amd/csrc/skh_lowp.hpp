@@ -51,8 +51,9 @@ struct LowpOut {
     float2* col_top2;          // merged column (c1, c2)
     uint32_t* row_match;       // final j or kNone
     // Exact re-score queue, one contiguous run of entries per pair. Entry
-    // (pair, row, j1 | kNone, 0): j1 = kNone needs the row scan, else only the
-    // column check of j1.
+    // (pair, row, j1 | kNone, w): j1 = kNone needs the row scan, else only the
+    // column check of j1; w = 0 (open), or, set by the FP8 path's BF16
+    // re-rank, 1 (settled: rejected) or 2 (settled: accepted with j1).
     uint4* queue;
     uint32_t* queue_len;       // [0] queue entries, [1] chunks
     uint4* pre;                // per entry: exact (j1, d1 bits, d2 bits, 1) of a gathered row scan
@@ -107,6 +108,13 @@ inline int size_bucket(uint32_t n) {
 inline int tile_class(uint32_t na, uint32_t nb) { return kSizeBuckets * size_bucket(na) + size_bucket(nb); }
 void launch_decide(const LowpTask* tasks, uint32_t n_pairs, const LowpOut& out, float ratio,
                    int mutual, uint32_t tbits, hipStream_t s);
+// FP8 plans: BF16 re-rank of the row decisions the FP8 bound left open (per
+// queue chunk), settling entries with the BF16 bound before the exact passes.
+// tasks_b: the pairs' BF16 tasks (bf16 rows, norms, statistics); tbits_g: tile
+// bits of 64-column chunks over the widest view.
+void launch_rerank_bf16(const LowpTask* tasks_b, const LowpTask* tasks_8, const LowpOut& out,
+                        float ratio, int mutual, uint32_t tbits_8, uint32_t tbits_g,
+                        uint32_t grid, hipStream_t s);
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
                     uint32_t tbits, uint32_t grid, hipStream_t s);
 
