@@ -1182,8 +1182,15 @@ __device__ void exact_col_argmin_group(const LowpTask& t, const LowpOut& out, do
 //     e1 - eps_b > 2 c1 + eps_8 (a closer row exists)      -> w = 1;
 //     otherwise z = j1, w = 0: the exact column check of rescore_kernel;
 //   anything else stays a row scan (exact_rows_kernel / rescore_kernel).
+// COL = true is the column form for the open mutual checks (entries with j1
+// known, w = 0): the gathered vectors are the b rows j1, swept against all of
+// a, which gives each column's BF16 top-2 over the rows with the argmin row r1.
+// When k2 - k1 > eps_b, r1 is the column's unique exact winner (its exact
+// distance is at most 2 k1 + eps_b, every other row's at least 2 k2 - eps_b):
+// the entry is accepted (w = 2) if r1 is its row, else rejected (w = 1).
 // Every decision follows from bounds that hold for every element of the pair,
 // so the match list is the exact FP32 one whatever this kernel settles.
+template <bool COL>
 __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __restrict__ tasks_b,
                                                           const LowpTask* __restrict__ tasks_8,
                                                           LowpOut out, float ratio, int mutual,
@@ -1211,7 +1218,7 @@ __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __rest
             bool scan = false;
             if ((uint32_t)tid < ck.y) {
                 const uint4 e = out.queue[k0 + tid];
-                scan = e.z == kNone && e.w == 0u;
+                scan = e.w == 0u && (COL ? e.z != kNone : e.z == kNone);
             }
             const unsigned long long m = __ballot(scan);
             if (scan) slot_of[__popcll(m & ((1ull << tid) - 1ull))] = k0 + tid;
@@ -1224,13 +1231,22 @@ __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __rest
         const LowpTask t = tasks_b[pair];
         const LowpTask t8 = tasks_8[pair];
         if ((t.sa->bad | t.sb->bad | t8.sa->bad | t8.sb->bad) != 0) continue;  // exact path
-        // a fragments of this wave's row tile (gathered rows, padding beyond `rows`)
+        // the gathered side (rows of a; for COL the b rows j1) and the swept side
+        const uint8_t* gsrc = COL ? Ops::rows_b(t) : Ops::rows_a(t);
+        const uint8_t* qb = COL ? Ops::rows_a(t) : Ops::rows_b(t);
+        const uint32_t nsw = COL ? t.na : t.nb;
+        // a fragments of this wave's row tile (gathered vectors, padding beyond `rows`)
         const uint32_t lr = (uint32_t)(rt * 32 + l32);
         const bool valid = lr < rows;
-        const uint32_t arow = valid ? out.queue[slot_of[lr]].y : 0u;
+        uint32_t arow = 0u;
+        if (valid) {
+            const uint4 e = out.queue[slot_of[lr]];
+            arow = COL ? e.z : e.y;
+        }
         typename Ops::AFrag afrag;
-        Ops::load_a(afrag, Ops::rows_a(t) + (size_t)arow * Ops::kRowBytes, valid,
-                    valid ? Ops::meta_a(t, arow) : Ops::pad_meta(), lo_half);
+        Ops::load_a(afrag, gsrc + (size_t)arow * Ops::kRowBytes, valid,
+                    valid ? (COL ? Ops::meta_b(t, arow) : Ops::meta_a(t, arow)) : Ops::pad_meta(),
+                    lo_half);
         float m1[16], m2[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -1239,19 +1255,21 @@ __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __rest
         }
         uint32_t keep_v;
         asm volatile("v_mov_b32 %0, %1" : "=v"(keep_v) : "s"(~tmask));
-        const uint8_t* qb = Ops::rows_b(t);
-        for (uint32_t col0 = 0; col0 < t.nb; col0 += 64) {
+        for (uint32_t col0 = 0; col0 < nsw; col0 += 64) {
             __syncthreads();  // the previous chunk's fragments are read
 #pragma unroll
             for (int k = 0; k < 64 * PIECES / 256; ++k) {
                 const int e = tid + 256 * k, cc = e / PIECES, seg = e % PIECES;
                 const uint32_t col = col0 + cc;
-                const uint4 v = col < t.nb ? *reinterpret_cast<const uint4*>(
-                                                 qb + (size_t)col * Ops::kRowBytes + 16 * seg)
-                                           : make_uint4(0, 0, 0, 0);
+                const uint4 v = col < nsw ? *reinterpret_cast<const uint4*>(
+                                                qb + (size_t)col * Ops::kRowBytes + 16 * seg)
+                                          : make_uint4(0, 0, 0, 0);
                 *reinterpret_cast<uint4*>(&Bs[cc * STRIDE + 16 * seg]) = v;
             }
-            if (tid < 64) Bn[tid] = col0 + tid < t.nb ? Ops::meta_b(t, col0 + tid) : Ops::pad_meta();
+            if (tid < 64)
+                Bn[tid] = col0 + tid >= nsw ? Ops::pad_meta()
+                          : COL             ? Ops::meta_a(t, col0 + tid)
+                                            : Ops::meta_b(t, col0 + tid);
             __syncthreads();
             typename Ops::BFrag bfrag;
             Ops::load_b(bfrag, &Bs[(ct * 32 + l32) * STRIDE], Bn[ct * 32 + l32], lo_half);
@@ -1323,10 +1341,16 @@ __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __rest
                 }
                 k2 = n2;
                 const double e1 = 2.0 * (double)k1, e2 = 2.0 * (double)k2;
+                const uint32_t slot = slot_of[g];
+                if (COL) {
+                    // j: the column's BF16 argmin row, its exact winner when the gap exceeds eps_b
+                    if (j < t.na && (e2 - eps_b) > (e1 + eps_b))
+                        out.queue[slot].w = j == out.queue[slot].y ? 2u : 1u;
+                    continue;
+                }
                 const bool j1_certain = (e2 - e1) > 2.0 * eps_b && j < t.nb;
                 const float lo1 = __double2float_rd(e1 - eps_b), hi1 = __double2float_ru(e1 + eps_b);
                 const float lo2 = __double2float_rd(e2 - eps_b), hi2 = __double2float_ru(e2 + eps_b);
-                const uint32_t slot = slot_of[g];
                 if (lo1 >= r2 * hi2) {
                     out.queue[slot].w = 1u;  // certain reject
                 } else if (hi1 < r2 * fmaxf(lo2, 0.0f) && j1_certain) {
@@ -1574,8 +1598,11 @@ void launch_rerank_bf16(const LowpTask* tasks_b, const LowpTask* tasks_8, const 
                         float ratio, int mutual, uint32_t tbits_8, uint32_t tbits_g,
                         uint32_t grid, hipStream_t s) {
     if (!out.chunks) return;
-    hipLaunchKernelGGL(rerank_bf16_kernel, dim3(std::max(grid, 1u)), dim3(256), 0, s, tasks_b,
-                       tasks_8, out, ratio, mutual, tbits_8, tbits_g);
+    hipLaunchKernelGGL(rerank_bf16_kernel<false>, dim3(std::max(grid, 1u)), dim3(256), 0, s,
+                       tasks_b, tasks_8, out, ratio, mutual, tbits_8, tbits_g);
+    if (mutual)  // the open column checks, including the ones the row pass just opened
+        hipLaunchKernelGGL(rerank_bf16_kernel<true>, dim3(std::max(grid, 1u)), dim3(256), 0, s,
+                           tasks_b, tasks_8, out, ratio, mutual, tbits_8, tbits_g);
 }
 
 void launch_rescore(const LowpTask* tasks, const LowpOut& out, float ratio, int mutual,
