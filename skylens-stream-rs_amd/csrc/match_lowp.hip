@@ -396,6 +396,27 @@ constexpr uint32_t kPadCodes = 127u | ((127u + 92u) << 8);
 constexpr uint32_t kPadFold = 0x7eu;
 constexpr uint32_t kNegHalf3 = 0x00b0b0b0u;  // three e4m3 -1/2 bytes
 
+// Loads through the global address space: global_load instructions, which
+// only the vector-memory counter tracks. Through a generic pointer they would
+// be flat loads, which the LDS counter tracks as well, so the LDS waits of the
+// tile loop would also wait for the next chunk's prefetch.
+template <class T>
+__device__ __forceinline__ T ldg_as(const void* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ Fp8Meta ldg_meta8(const Fp8Meta* p) {
+    typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+    const v2u v = *(const __attribute__((address_space(1))) v2u*)p;
+    return Fp8Meta{v.x, v.y};
+}
+// 16 bytes as a native vector (a uint4 copy would go through its generic
+// copy constructor, a flat load again)
+__device__ __forceinline__ uint4 ldg_u4(const void* p) {
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    const v4u v = *(const __attribute__((address_space(1))) v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Operand handling of the two ranking precisions. Per 32 x 32 tile the matrix
 // core runs kSteps MFMAs; a row of b takes kRowBytes in global memory and
 // kStride bytes in LDS (16 bytes of padding keep the 16-byte reads of 8
@@ -417,10 +438,10 @@ struct RankOps<false> {
     };
     static __device__ __forceinline__ Meta pad_meta() { return kPadNorm; }
     static __device__ __forceinline__ Meta meta_a(const LowpTask& t, uint32_t row) {
-        return t.nrm_a[row];
+        return ldg_as<Meta>(t.nrm_a + row);
     }
     static __device__ __forceinline__ Meta meta_b(const LowpTask& t, uint32_t col) {
-        return t.nrm_b[col];
+        return ldg_as<Meta>(t.nrm_b + col);
     }
     static __device__ __forceinline__ const uint8_t* rows_a(const LowpTask& t) {
         return reinterpret_cast<const uint8_t*>(t.qa);
@@ -478,10 +499,10 @@ struct RankOps<true> {
     };
     static __device__ __forceinline__ Meta pad_meta() { return Fp8Meta{kPadCodes, kPadFold}; }
     static __device__ __forceinline__ Meta meta_a(const LowpTask& t, uint32_t row) {
-        return t.meta_a[row];
+        return ldg_meta8(t.meta_a + row);
     }
     static __device__ __forceinline__ Meta meta_b(const LowpTask& t, uint32_t col) {
-        return t.meta_b[col];
+        return ldg_meta8(t.meta_b + col);
     }
     static __device__ __forceinline__ const uint8_t* rows_a(const LowpTask& t) { return t.q8a; }
     static __device__ __forceinline__ const uint8_t* rows_b(const LowpTask& t) { return t.q8b; }
@@ -598,8 +619,7 @@ __global__ __launch_bounds__(256, RB == 2 && CB == 4 ? 2 : 1) void mfma_rank_ker
         for (int k = 0; k < PF; ++k) {
             const int e = tid + 256 * k, c = e / PIECES, seg = e % PIECES;
             const uint32_t col = col0 + c;
-            pf[k] = col < t.nb ? *reinterpret_cast<const uint4*>(qb + (size_t)col * Ops::kRowBytes +
-                                                                  16 * seg)
+            pf[k] = col < t.nb ? ldg_u4(qb + (size_t)col * Ops::kRowBytes + 16 * seg)
                                : make_uint4(0, 0, 0, 0);
         }
         if (tid < BN) pfn = (col0 + tid < t.nb) ? Ops::meta_b(t, col0 + tid) : Ops::pad_meta();
@@ -1261,8 +1281,7 @@ __global__ __launch_bounds__(256) void rerank_bf16_kernel(const LowpTask* __rest
             for (int k = 0; k < 64 * PIECES / 256; ++k) {
                 const int e = tid + 256 * k, cc = e / PIECES, seg = e % PIECES;
                 const uint32_t col = col0 + cc;
-                const uint4 v = col < nsw ? *reinterpret_cast<const uint4*>(
-                                                qb + (size_t)col * Ops::kRowBytes + 16 * seg)
+                const uint4 v = col < nsw ? ldg_u4(qb + (size_t)col * Ops::kRowBytes + 16 * seg)
                                           : make_uint4(0, 0, 0, 0);
                 *reinterpret_cast<uint4*>(&Bs[cc * STRIDE + 16 * seg]) = v;
             }
