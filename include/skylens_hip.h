@@ -17,6 +17,7 @@
  *   per-pair driver   benches/pipeline.rs:219-305 -> skh_plan_create / skh_plan_run /
  *                                                      skh_plan_results
  *   detect_and_describe  features.rs:594-648 -> skh_detect_and_describe
+ *   build_tracks      tracks.rs:190-508 -> skh_build_tracks
  */
 #ifndef SKYLENS_HIP_H
 #define SKYLENS_HIP_H
@@ -123,6 +124,42 @@ int skh_scheduled_pairs(const uint64_t* views, size_t n, const skh_pair_schedule
  * Err((n_matches, n_inliers)): SKH_EINVAL, *out_len untouched. Host-only. */
 int skh_verified_matches(const uint32_t* matches, size_t n_matches, const uint8_t* inliers,
                          size_t n_inliers, uint32_t* out_ij, size_t* out_len);
+
+/* ---- multi-view tracks (tracks.rs:127-800) ---- */
+/* ConflictPolicy (tracks.rs:103-107). */
+#define SKH_TRACKS_DROP 0
+#define SKH_TRACKS_SPLIT 1
+/* TrackConfig (tracks.rs:109-125). */
+typedef struct {
+    int32_t policy; /* SKH_TRACKS_DROP or SKH_TRACKS_SPLIT */
+    int32_t pad;
+    uint64_t min_length, max_tracks;
+} skh_track_cfg;
+/* TrackStats (tracks.rs:145-155). */
+typedef struct {
+    uint64_t edges, invalid_matches, conflicts, vote_cuts, too_short, inconsistent, truncated,
+        tracks;
+} skh_track_stats;
+/* TrackConfig::default() (tracks.rs:118-125): Split, min_length 3,
+ * max_tracks MAX_TRACKS = 100000. */
+void skh_track_cfg_default(skh_track_cfg* cfg);
+/* build_tracks (tracks.rs:190-508); replaces the Rust call
+ * build_tracks(&pairs, &keypoints, &cfg).
+ * Pair p (PairMatches): images image_a[p], image_b[p]; matches (fa, fb) =
+ * m_ab[2 k], m_ab[2 k + 1] for k in [m_off[p], m_off[p + 1]). Image i's
+ * keypoints (x, y) = kp[2 x], kp[2 x + 1] for x in [kp_off[i], kp_off[i + 1]).
+ * Tracks out as CSR: track t's observations (image, feature) = obs[2 q],
+ * obs[2 q + 1] for q in [track_off[t], track_off[t + 1]) (the pixel is
+ * keypoints[image][feature]). *n_tracks and *stats are always written; if
+ * track_cap (entries of track_off, n_tracks + 1 needed) or obs_cap
+ * (observations) is short the call returns SKH_EINVAL (total keypoints + 1
+ * and total keypoints always suffice). The Split policy's displacement
+ * filter runs on the device; blocks until done. */
+int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* image_a,
+                     const uint64_t* image_b, const uint64_t* m_off, const uint64_t* m_ab,
+                     size_t n_images, const uint64_t* kp_off, const double* kp,
+                     const skh_track_cfg* cfg, uint64_t* track_off, size_t track_cap,
+                     uint64_t* obs, size_t obs_cap, size_t* n_tracks, skh_track_stats* stats);
 
 /* ratio_match (matching.rs:167-215) on host buffers.
  * a: na x 128 f32 row-major, b: nb x 128 f32 (Feature.desc gathered

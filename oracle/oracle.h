@@ -222,6 +222,38 @@ double orc_xlog(double x);
 double orc_xatan2(double y, double x);
 void orc_xsincos(double a, double* s, double* c);
 
+/* ---- multi-view tracks (tracks.rs:127-800) ---- */
+/* TrackConfig (tracks.rs:109-125): policy 0 = Drop, 1 = Split. */
+typedef struct {
+    int32_t policy, pad;
+    uint64_t min_length, max_tracks;
+} orc_track_cfg;
+/* TrackStats (tracks.rs:145-155). */
+typedef struct {
+    uint64_t edges, invalid_matches, conflicts, vote_cuts, too_short, inconsistent, truncated,
+        tracks;
+} orc_track_stats;
+/* build_tracks (tracks.rs:190-508). Pair p: images image_a[p], image_b[p],
+ * matches (fa, fb) = m_ab[2 k], m_ab[2 k + 1] for k in [m_off[p], m_off[p+1]).
+ * Image i's keypoints: kp[2 x], kp[2 x + 1] for x in [kp_off[i], kp_off[i+1]).
+ * Tracks out as CSR: track t's observations (image, feature) = obs[2 q],
+ * obs[2 q + 1] for q in [track_off[t], track_off[t+1]). Returns the track
+ * count, or -1 if track_cap (entries of track_off) or obs_cap (observations)
+ * is short or an allocation fails. */
+long orc_build_tracks(size_t n_pairs, const uint64_t* image_a, const uint64_t* image_b,
+                      const uint64_t* m_off, const uint64_t* m_ab, size_t n_images,
+                      const uint64_t* kp_off, const double* kp, const orc_track_cfg* cfg,
+                      uint64_t* track_off, size_t track_cap, uint64_t* obs, size_t obs_cap,
+                      orc_track_stats* st);
+/* displacement_outliers_with_floor (tracks.rs:519-670) of n deduplicated
+ * (fa, fb) of one pair: out[k] = 1 for a flagged match. */
+void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
+                               double floor_, double cap_, uint8_t* out);
+/* select_tracks (tracks.rs:786-799): the selected tracks' indices in output
+ * order; returns their count (-1 on allocation failure). */
+long orc_select_tracks(const uint64_t* track_off, const uint64_t* obs, size_t n_tracks,
+                       size_t max_tracks, uint64_t* order_out);
+
 #ifdef __cplusplus
 }
 #endif

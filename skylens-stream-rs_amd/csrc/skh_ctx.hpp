@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <initializer_list>
 #include <mutex>
@@ -46,6 +47,8 @@ struct Ctx {
         queue_len, q8_a, q8_b, meta_a, meta_b, pre, chunks;
     // single-pair RANSAC
     DevBuf r_x1, r_x2, r_out, r_scratch, r_task, r_queue;
+    // track builder: the displacement filter's pairs, points, grid keys, flags
+    DevBuf t_pairs, t_point_pair, t_pts, t_keys, t_flags;
 
     std::vector<RowBlock> h_blocks;
 
@@ -54,7 +57,8 @@ struct Ctx {
                 &row_d1, &row_d2, &col_key,   &row_match, &out_base, &q_a,    &q_b,
                 &nrm_a,  &nrm_b,  &stats,     &ltasks,   &row_k1,   &row_k2, &col_part,
                 &col_top2, &queue, &queue_len, &q8_a, &q8_b, &meta_a, &meta_b, &pre, &chunks, &r_x1, &r_x2, &r_out,
-                &r_scratch, &r_task, &r_queue};
+                &r_scratch, &r_task, &r_queue, &t_pairs, &t_point_pair, &t_pts, &t_keys,
+                &t_flags};
     }
 
     ~Ctx();
@@ -62,6 +66,11 @@ struct Ctx {
     int fail(int code, const char* fmt, ...);
     int hip(hipError_t e, const char* what);
     int ensure(DevBuf& b, size_t bytes, const char* what);
+    // Displacement-filter flags (tracks.rs:519-670) of every pair's sorted,
+    // deduplicated (fa, fb) against its images' keypoints (x, y pairs).
+    int displacement_flags(const std::vector<std::vector<std::array<uint64_t, 2>>>& valid,
+                           const std::vector<std::pair<const double*, const double*>>& kps,
+                           std::vector<std::vector<uint8_t>>& flags);
 
     // Ranking tile of a pair: the fixed override, the tuned table, or the default
     // (256 x 64; 128 x 64 for at most 128 rows).

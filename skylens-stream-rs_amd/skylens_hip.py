@@ -97,6 +97,10 @@ EXPORTS = [
      [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_size_t, _c_size_p]),
     ("skh_detect_and_describe_dev", ctypes.c_int,
      [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_size_t, _c_size_p]),
+    ("skh_track_cfg_default", None, [_vp]),
+    ("skh_build_tracks", ctypes.c_int,
+     [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp,
+      ctypes.c_size_t, _vp, ctypes.c_size_t, _c_size_p, _vp]),
 ]
 
 TILE_CLASSES = 25
@@ -139,6 +143,31 @@ class DetectorConfig(ctypes.Structure):
 
 
 KEYPOINT_FIELDS = ("x", "y", "sigma", "response", "angle")  # features.rs:295-304
+
+TRACKS_DROP, TRACKS_SPLIT = 0, 1  # ConflictPolicy (tracks.rs:103-107)
+
+
+class TrackConfig(ctypes.Structure):
+    """TrackConfig (tracks.rs:109-125), skh_track_cfg."""
+    _fields_ = [("policy", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("min_length", ctypes.c_uint64), ("max_tracks", ctypes.c_uint64)]
+
+    @classmethod
+    def default(cls, **kw) -> "TrackConfig":
+        c = cls()
+        load_library().skh_track_cfg_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+TRACK_STATS = ("edges", "invalid_matches", "conflicts", "vote_cuts", "too_short", "inconsistent",
+               "truncated", "tracks")
+
+
+class TrackStats(ctypes.Structure):
+    """TrackStats (tracks.rs:145-155), skh_track_stats."""
+    _fields_ = [(k, ctypes.c_uint64) for k in TRACK_STATS]
 
 
 class PairSchedule(ctypes.Structure):
@@ -421,6 +450,40 @@ class Context:
             _ptr(desc) if describe else None, cap, ctypes.byref(n)), "skh_detect_and_describe")
         k = n.value
         return kps[:k].copy(), (desc[:k].copy() if describe else None)
+
+    def build_tracks(self, pairs, keypoints, cfg: Optional[TrackConfig] = None):
+        """build_tracks (tracks.rs:190-508). ``pairs``: (image_a, image_b,
+        [(fa, fb), ...]) per PairMatches; ``keypoints``: one (n_i, 2) array of
+        pixels per image. Returns (tracks, stats): each track a list of
+        (image, feature) observations (the pixel is keypoints[image][feature]),
+        stats a dict of TrackStats."""
+        cfg = cfg or TrackConfig.default()
+        ia = np.ascontiguousarray([p[0] for p in pairs], dtype=np.uint64)
+        ib = np.ascontiguousarray([p[1] for p in pairs], dtype=np.uint64)
+        lens = [len(p[2]) for p in pairs]
+        m_off = np.zeros(len(pairs) + 1, np.uint64)
+        m_off[1:] = np.cumsum(lens)
+        m = (np.concatenate([np.asarray(p[2], np.uint64).reshape(-1, 2) for p in pairs])
+             if sum(lens) else np.zeros((0, 2), np.uint64))
+        m = np.ascontiguousarray(m, dtype=np.uint64)
+        kl = [len(k) for k in keypoints]
+        kp_off = np.zeros(len(keypoints) + 1, np.uint64)
+        kp_off[1:] = np.cumsum(kl)
+        kp = (np.concatenate([np.asarray(k, np.float64).reshape(-1, 2) for k in keypoints])
+              if sum(kl) else np.zeros((0, 2), np.float64))
+        kp = np.ascontiguousarray(kp, dtype=np.float64)
+        n = int(kp_off[-1])
+        toff = np.zeros(n + 2, np.uint64)
+        obs = np.zeros((max(1, n), 2), np.uint64)
+        nt = ctypes.c_size_t(0)
+        st = TrackStats()
+        self._check(self.lib.skh_build_tracks(
+            self.handle, len(pairs), _ptr(ia), _ptr(ib), _ptr(m_off), _ptr(m), len(keypoints),
+            _ptr(kp_off), _ptr(kp), ctypes.byref(cfg), _ptr(toff), toff.shape[0], _ptr(obs), n,
+            ctypes.byref(nt), ctypes.byref(st)), "skh_build_tracks")
+        tracks = [[(int(a), int(b)) for a, b in obs[toff[t]:toff[t + 1]]]
+                  for t in range(nt.value)]
+        return tracks, {k: int(getattr(st, k)) for k in TRACK_STATS}
 
     def detect_and_describe_dev(self, img_ptr: int, w: int, h: int, cfg: DetectorConfig,
                                 kp_ptr: int, desc_ptr: int, cap: int) -> int:

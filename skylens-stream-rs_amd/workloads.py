@@ -336,3 +336,65 @@ def textured_image(w: int, h: int, seed: int, n_blobs: Optional[int] = None) -> 
         d2 = (x[y0:y1, x0:x1] - bx[i]) ** 2 + (y[y0:y1, x0:x1] - by[i]) ** 2
         v[y0:y1, x0:x1] += np.float32(0.5) * ba[i] * np.exp(-d2 / (np.float32(2.0) * bs[i] * bs[i]))
     return v
+
+
+@dataclass
+class TrackScene:
+    """Verified pairwise matches of a sequence of views of one planar scene:
+    keypoints[i] (n_i, 2) pixels of image i, gt[i][f] the scene point of
+    feature f (negative for clutter), pairs (image_a, image_b, (m, 2) matches)."""
+    keypoints: List[np.ndarray]
+    gt: List[np.ndarray]
+    pairs: List[Tuple[int, int, np.ndarray]]
+    inliers: int
+    outliers: int
+
+
+def track_scene(seed: int = 0, images: int = 12, points: int = 2500, keep: float = 1.0,
+                outlier_per_mille: int = 10, temporal: int = 5, width: float = 960.0,
+                height: float = 540.0) -> TrackScene:
+    """Views sliding along a strip of scene points (80 px per image, a small
+    rotation and scale per view, 0.3 px noise), 3 % clutter features per
+    image, pairs (i, j) with 1 <= j - i <= temporal (every other pair stored
+    in the (j, i) direction), each shared point matched with probability
+    ``keep`` plus ``outlier_per_mille`` random wrong matches per 1000 true ones."""
+    rng = np.random.default_rng(seed)
+    span = width + 80.0 * images
+    world = np.stack([rng.uniform(0.0, span, points), rng.uniform(0.0, height, points)], 1)
+    kps, gts = [], []
+    for i in range(images):
+        th, sc = 0.004 * (i % 5) - 0.008, 1.0 + 0.006 * (i % 3)
+        c, s = np.cos(th), np.sin(th)
+        local = (world - np.array([80.0 * i, 0.0])) * sc
+        px = np.stack([c * local[:, 0] - s * local[:, 1], s * local[:, 0] + c * local[:, 1]], 1)
+        px = px + rng.normal(0.0, 0.3, px.shape)
+        vis = np.nonzero((px[:, 0] >= 0) & (px[:, 0] < width) & (px[:, 1] >= 0)
+                         & (px[:, 1] < height))[0]
+        n_cl = int(0.03 * len(vis))
+        clutter = np.stack([rng.uniform(0, width, n_cl), rng.uniform(0, height, n_cl)], 1)
+        feats = np.concatenate([px[vis], clutter])
+        g = np.concatenate([vis, -1 - np.arange(n_cl) - 1000 * i])
+        perm = rng.permutation(len(feats))
+        kps.append(np.ascontiguousarray(feats[perm]))
+        gts.append(g[perm])
+    pairs, n_in, n_out = [], 0, 0
+    for i in range(images):
+        for j in range(i + 1, min(images, i + temporal + 1)):
+            fa_of = {int(p): f for f, p in enumerate(gts[i]) if p >= 0}
+            fb_of = {int(p): f for f, p in enumerate(gts[j]) if p >= 0}
+            shared = sorted(set(fa_of) & set(fb_of))
+            m = [(fa_of[p], fb_of[p]) for p in shared if rng.random() < keep]
+            n_in += len(m)
+            n_bad = len(m) * outlier_per_mille // 1000
+            for _ in range(n_bad):
+                fa, fb = int(rng.integers(len(gts[i]))), int(rng.integers(len(gts[j])))
+                if gts[i][fa] != gts[j][fb]:
+                    m.append((fa, fb))
+                    n_out += 1
+            m = np.array(m, np.int64).reshape(-1, 2)
+            m = m[rng.permutation(len(m))]
+            if (i + j) % 2:
+                pairs.append((j, i, np.ascontiguousarray(m[:, ::-1])))
+            else:
+                pairs.append((i, j, m))
+    return TrackScene(kps, gts, pairs, n_in, n_out)

@@ -587,3 +587,86 @@ def scene_texture(w, h, seed, n_blobs=None):
     """workloads.textured_image: the detector's textured test image."""
     import workloads
     return workloads.textured_image(w, h, seed, n_blobs)
+
+
+# ---- multi-view tracks (tracks.rs, oracle/tracks_oracle.c) ----
+
+class TrackCfg(ctypes.Structure):
+    """TrackConfig (tracks.rs:109-125): policy 0 = Drop, 1 = Split."""
+    _fields_ = [("policy", ctypes.c_int32), ("pad", ctypes.c_int32), ("min_length", _u64),
+                ("max_tracks", _u64)]
+
+
+TRACK_STATS = ("edges", "invalid_matches", "conflicts", "vote_cuts", "too_short", "inconsistent",
+               "truncated", "tracks")
+
+
+class TrackStats(ctypes.Structure):
+    _fields_ = [(k, _u64) for k in TRACK_STATS]
+
+
+_SIGS.update({
+    "orc_build_tracks": (ctypes.c_long, [_sz, _vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _sz,
+                                         _vp, _sz, _vp]),
+    "orc_displacement_outliers": (None, [_vp, _sz, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                         _vp]),
+    "orc_select_tracks": (ctypes.c_long, [_vp, _vp, _sz, _sz, _vp]),
+})
+
+
+def track_cfg(policy="split", min_length=3, max_tracks=100_000):
+    return TrackCfg(1 if policy == "split" else 0, 0, min_length, max_tracks)
+
+
+def pack_track_input(pairs, keypoints):
+    """Flat arrays of the C layout: pairs = [(image_a, image_b, [(fa, fb), ...]), ...],
+    keypoints = [array (n_i, 2), ...]."""
+    ia = np.array([p[0] for p in pairs], np.uint64).reshape(-1)
+    ib = np.array([p[1] for p in pairs], np.uint64).reshape(-1)
+    lens = [len(p[2]) for p in pairs]
+    m_off = np.zeros(len(pairs) + 1, np.uint64)
+    m_off[1:] = np.cumsum(lens, dtype=np.uint64) if pairs else []
+    m = (np.concatenate([np.asarray(p[2], np.uint64).reshape(-1, 2) for p in pairs])
+         if sum(lens) else np.zeros((0, 2), np.uint64))
+    kl = [len(k) for k in keypoints]
+    kp_off = np.zeros(len(keypoints) + 1, np.uint64)
+    kp_off[1:] = np.cumsum(kl, dtype=np.uint64)
+    kp = (np.concatenate([np.asarray(k, np.float64).reshape(-1, 2) for k in keypoints])
+          if sum(kl) else np.zeros((0, 2), np.float64))
+    return (np.ascontiguousarray(ia), np.ascontiguousarray(ib), m_off,
+            np.ascontiguousarray(m, dtype=np.uint64), kp_off, np.ascontiguousarray(kp))
+
+
+def build_tracks(pairs, keypoints, cfg=None):
+    """(tracks as lists of (image, feature), stats dict)."""
+    cfg = cfg or track_cfg()
+    ia, ib, m_off, m, kp_off, kp = pack_track_input(pairs, keypoints)
+    n = int(kp_off[-1])
+    toff = np.zeros(n + 2, np.uint64)
+    obs = np.zeros((max(1, n), 2), np.uint64)
+    st = TrackStats()
+    k = lib().orc_build_tracks(len(pairs), P(ia), P(ib), P(m_off), P(m), len(keypoints), P(kp_off),
+                               P(kp), ctypes.byref(cfg), P(toff), toff.shape[0], P(obs), n,
+                               ctypes.byref(st))
+    assert k >= 0, "orc_build_tracks failed"
+    tracks = [[(int(a), int(b)) for a, b in obs[toff[t]:toff[t + 1]]] for t in range(k)]
+    return tracks, {f: int(getattr(st, f)) for f in TRACK_STATS}
+
+
+def displacement_outliers(matches, kp_a, kp_b, floor=0.25, cap=3.0):
+    m = np.ascontiguousarray(np.asarray(matches, np.uint64).reshape(-1, 2))
+    a = np.ascontiguousarray(np.asarray(kp_a, np.float64).reshape(-1, 2))
+    b = np.ascontiguousarray(np.asarray(kp_b, np.float64).reshape(-1, 2))
+    out = np.zeros(max(1, m.shape[0]), np.uint8)
+    lib().orc_displacement_outliers(P(m), m.shape[0], P(a), P(b), floor, cap, P(out))
+    return out[:m.shape[0]].astype(bool)
+
+
+def select_tracks(tracks, max_tracks):
+    """Indices of the selected tracks (lists of (image, feature)) in output order."""
+    toff = np.zeros(len(tracks) + 1, np.uint64)
+    toff[1:] = np.cumsum([len(t) for t in tracks])
+    obs = np.ascontiguousarray(np.array([o for t in tracks for o in t], np.uint64).reshape(-1, 2))
+    out = np.zeros(max(1, len(tracks)), np.uint64)
+    k = lib().orc_select_tracks(P(toff), P(obs), len(tracks), max_tracks, P(out))
+    return [int(x) for x in out[:k]]

@@ -86,6 +86,8 @@ def _c_layout():
         "skh_detector_cfg": ["octaves", "scales", "sigma0", "contrast", "edge_ratio",
                              "upscale_fill", "max_features", "upscale_below"],
         "skh_keypoint": ["x", "y", "sigma", "response", "angle"],
+        "skh_track_cfg": ["policy", "pad", "min_length", "max_tracks"],
+        "skh_track_stats": list(skylens_hip.TRACK_STATS),
     }
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "skylens_hip.h"',
              "int main(void) {"]
@@ -113,7 +115,9 @@ def test_struct_layout_matches_c_header():
                "skh_model_sel": skylens_hip.ModelSelectionC,
                "skh_batch_desc": skylens_hip.BatchDesc,
                "skh_pair_schedule": skylens_hip.PairSchedule,
-               "skh_detector_cfg": skylens_hip.DetectorConfig}
+               "skh_detector_cfg": skylens_hip.DetectorConfig,
+               "skh_track_cfg": skylens_hip.TrackConfig,
+               "skh_track_stats": skylens_hip.TrackStats}
     assert got["skh_keypoint"] == 4 * len(skylens_hip.KEYPOINT_FIELDS)
     for i, f in enumerate(skylens_hip.KEYPOINT_FIELDS):
         assert got[f"skh_keypoint.{f}"] == 4 * i

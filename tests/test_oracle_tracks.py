@@ -1,0 +1,259 @@
+"""The track builder's oracle (oracle/tracks_oracle.c) against the reference's
+own tests of build_tracks / select_tracks (tracks.rs:1087-1525) and checks
+that the mathematics fixes:
+
+- the reference's known answers: conflicting_component_is_dropped,
+  short_and_invalid_matches_are_excluded, cap_prefers_long_then_wide_tracks
+  (exact tracks and stats);
+- order independence (tracks.rs:1449-1488): reversed, flipped, shuffled and
+  duplicated pairs give the same tracks;
+- Drop against a breadth-first restatement of its components (independent of
+  the union-find), on random graphs;
+- purity / completeness on a synthetic scene with outliers, the reference's
+  metric (tracks.rs:999-1085) restated with breadth-first search;
+- the displacement filter: a rigid motion flags nothing, planted gross
+  outliers are flagged, an affine field (rotation + scale) keeps its true
+  matches (tracks.rs:1554-1590).
+"""
+from collections import defaultdict, deque
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import workloads
+
+
+def _pairs(s):
+    return [(a, b, m) for a, b, m in s.pairs]
+
+
+def test_conflicting_component_is_dropped():
+    """tracks.rs:1347-1413."""
+    kp = [[(float(i), float(f)) for f in range(3)] for i in range(4)]
+    pairs = [(0, 1, [(0, 0), (2, 2)]), (1, 2, [(0, 0), (2, 2)]), (2, 0, [(0, 1)]),
+             (2, 3, [(2, 2)])]
+    t, st = ob.build_tracks(pairs, kp, ob.track_cfg("drop"))
+    assert st["conflicts"] == 1
+    assert t == [[(0, 2), (1, 2), (2, 2), (3, 2)]]
+    for pol in ("drop", "split"):
+        for tr in ob.build_tracks(pairs, kp, ob.track_cfg(pol))[0]:
+            assert all(a[0] < b[0] for a, b in zip(tr, tr[1:]))
+    t, st = ob.build_tracks(pairs, kp, ob.track_cfg("split", min_length=2))
+    assert st["conflicts"] == 1
+    assert t == [[(0, 0), (1, 0)], [(0, 1), (2, 0)], [(0, 2), (1, 2), (2, 2), (3, 2)]]
+
+
+def test_short_and_invalid_matches_are_excluded():
+    """tracks.rs:1415-1447."""
+    kp = [[(0.0, 0.0)] * 2 for _ in range(3)]
+    pairs = [(0, 0, [(0, 1)]), (0, 1, [(5, 0), (0, 0)])]
+    t, st = ob.build_tracks(pairs, kp, ob.track_cfg(min_length=3))
+    assert st["invalid_matches"] == 2 and st["too_short"] == 1 and t == []
+    t, _ = ob.build_tracks(pairs, kp, ob.track_cfg(min_length=2))
+    assert len(t) == 1 and len(t[0]) == 2
+
+
+def test_cap_prefers_long_then_wide_tracks():
+    """tracks.rs:1490-1525 (select_tracks)."""
+    def tr(imgs, f):
+        return [(i, f) for i in imgs]
+    tracks = [tr([0, 1], 0), tr([0, 1, 2], 1), tr([0, 5], 2), tr([3, 4, 5], 3), tr([0, 1, 9], 4)]
+    sel = ob.select_tracks(tracks, 3)
+    assert [tracks[k][0][1] for k in sel] == [1, 4, 3]
+    sel = ob.select_tracks(tracks, 4)
+    assert [tracks[k][0][1] for k in sel] == [1, 2, 4, 3]
+
+
+def _hash(x):
+    """The reference test's mixer (tracks.rs:823-828)."""
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+@pytest.mark.parametrize("policy", ["drop", "split"])
+def test_order_independent(policy):
+    """tracks.rs:1449-1488: reversed pair order, flipped directions, shuffled
+    and duplicated matches leave tracks and counts unchanged."""
+    s = workloads.track_scene(seed=1, images=10, points=1500, outlier_per_mille=10)
+    cfg = ob.track_cfg(policy)
+    t0, st0 = ob.build_tracks(_pairs(s), s.keypoints, cfg)
+    pairs = []
+    for k, (a, b, m) in enumerate(reversed(s.pairs)):
+        m = sorted(map(tuple, m.tolist()), key=lambda e: _hash((k << 40) ^ (e[0] << 20) ^ e[1]))
+        if k % 2 == 0:
+            pairs.append((b, a, [(y, x) for x, y in m]))
+        else:
+            pairs.append((a, b, m))
+    pairs.append(s.pairs[0])
+    t1, st1 = ob.build_tracks(pairs, s.keypoints, cfg)
+    assert t0 == t1
+    assert st0["conflicts"] == st1["conflicts"] and st0["tracks"] == st1["tracks"]
+
+
+def _drop_bfs(pairs, keypoints, min_len, max_tracks):
+    """Drop policy restated without union-find: connected components of the
+    deduplicated valid edges by breadth-first search; a component with two
+    features of one image is dropped; the rest (>= min_len nodes) in
+    first-observation order (no cap reached here)."""
+    nimg = len(keypoints)
+    edges = set()
+    for a, b, m in pairs:
+        for fa, fb in m:
+            if a == b or a >= nimg or b >= nimg or fa >= len(keypoints[a]) or fb >= len(keypoints[b]):
+                continue
+            u, v = (a, fa), (b, fb)
+            edges.add((min(u, v), max(u, v)))
+    adj = defaultdict(list)
+    for u, v in edges:
+        adj[u].append(v)
+        adj[v].append(u)
+    seen, tracks, conflicts = set(), [], 0
+    for s0 in sorted(adj):
+        if s0 in seen:
+            continue
+        comp, q = [], deque([s0])
+        seen.add(s0)
+        while q:
+            x = q.popleft()
+            comp.append(x)
+            for y in adj[x]:
+                if y not in seen:
+                    seen.add(y)
+                    q.append(y)
+        comp.sort()
+        if len({i for i, _ in comp}) < len(comp):
+            conflicts += 1
+        elif len(comp) >= max(min_len, 2):
+            tracks.append(comp)
+    tracks.sort(key=lambda t: t[0])
+    assert len(tracks) <= max_tracks
+    return tracks, conflicts
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_drop_equals_breadth_first_components(seed):
+    rng = np.random.default_rng(seed)
+    nimg = 6
+    kp = [np.zeros((int(rng.integers(3, 9)), 2)) for _ in range(nimg)]
+    pairs = []
+    for _ in range(12):
+        a, b = int(rng.integers(nimg)), int(rng.integers(nimg))
+        m = [(int(rng.integers(10)), int(rng.integers(10))) for _ in range(int(rng.integers(1, 6)))]
+        pairs.append((a, b, m))
+    for ml in (2, 3):
+        t, st = ob.build_tracks(pairs, kp, ob.track_cfg("drop", min_length=ml))
+        want, conflicts = _drop_bfs(pairs, kp, ml, 100000)
+        assert t == want
+        assert st["conflicts"] == conflicts
+
+
+def purity_completeness(s, tracks):
+    """The reference's metric (tracks.rs:999-1085): share of tracks whose
+    observations all see one scene point; per point (reaching >= 3 views
+    through its true matches), the largest share of its reachable
+    observations that one track holds."""
+    gt = s.gt
+    pure = sum(all(gt[i][f] == gt[tr[0][0]][tr[0][1]] for i, f in tr) for tr in tracks)
+    edges = defaultdict(list)
+    for a, b, m in s.pairs:
+        for fa, fb in m:
+            ga, gb = gt[a][fa], gt[b][fb]
+            if ga == gb:
+                edges[int(ga)].append(((a, int(fa)), (b, int(fb))))
+    reach = {}
+    for p, es in edges.items():
+        adj = defaultdict(list)
+        for x, y in es:
+            adj[x].append(y)
+            adj[y].append(x)
+        seen, best = set(), 0
+        for s0 in adj:
+            if s0 in seen:
+                continue
+            q, c = deque([s0]), 0
+            seen.add(s0)
+            while q:
+                x = q.popleft()
+                c += 1
+                for y in adj[x]:
+                    if y not in seen:
+                        seen.add(y)
+                        q.append(y)
+            best = max(best, c)
+        if best >= 3:
+            reach[p] = best
+    hold = defaultdict(lambda: defaultdict(int))
+    for ti, tr in enumerate(tracks):
+        for i, f in tr:
+            hold[int(gt[i][f])][ti] += 1
+    comp = [min(max(hold[p].values(), default=0), n) / n for p, n in reach.items()]
+    return pure / len(tracks), float(np.mean(comp))
+
+
+def test_clean_matches_give_exact_tracks():
+    """tracks.rs:1087-1110: without wrong matches Drop recovers every point."""
+    s = workloads.track_scene(seed=2, images=10, points=1500, outlier_per_mille=0)
+    assert s.outliers == 0
+    for pol in ("drop", "split"):
+        t, st = ob.build_tracks(_pairs(s), s.keypoints, ob.track_cfg(pol))
+        pur, comp = purity_completeness(s, t)
+        assert pur == 1.0
+        if pol == "drop":
+            assert st["conflicts"] == 0 and comp == 1.0
+        else:
+            assert comp >= 0.995
+
+
+def test_purity_and_completeness_with_outliers():
+    """tracks.rs:1112-1155: 1 % wrong matches."""
+    s = workloads.track_scene(seed=3, images=12, points=2500, outlier_per_mille=10)
+    assert s.outliers * 200 > s.inliers
+    (pd, cd, sd), (ps, cs, ss) = [
+        (*purity_completeness(s, t), st) for t, st in
+        (ob.build_tracks(_pairs(s), s.keypoints, ob.track_cfg(p)) for p in ("drop", "split"))]
+    assert pd >= 0.95 and ps >= 0.99 and cs >= 0.98 and cd < cs
+    assert sd["conflicts"] > 0
+    cap = ss["tracks"] // 2
+    t, st = ob.build_tracks(_pairs(s), s.keypoints, ob.track_cfg("split", max_tracks=cap))
+    assert len(t) == cap and st["truncated"] == ss["tracks"] - cap > 0
+
+
+def _field(n, seed, f):
+    rng = np.random.default_rng(seed)
+    a = np.stack([rng.uniform(0, 960, n), rng.uniform(0, 540, n)], 1)
+    return a, f(a) + rng.normal(0, 0.2, a.shape)
+
+
+def test_displacement_rigid_motion_flags_nothing():
+    a, b = _field(400, 4, lambda p: p + np.array([35.0, -12.0]))
+    m = [(k, k) for k in range(len(a))]
+    assert not ob.displacement_outliers(m, a, b).any()
+
+
+def test_displacement_flags_planted_outliers():
+    a, b = _field(400, 5, lambda p: p + np.array([20.0, 5.0]))
+    bad = [3, 50, 111, 250, 399]
+    b[bad] += np.array([[150.0, -90.0]])
+    flags = ob.displacement_outliers([(k, k) for k in range(len(a))], a, b)
+    assert flags[bad].all()
+    assert flags.sum() == len(bad)
+
+
+def test_displacement_keeps_an_affine_field():
+    """tracks.rs:1554-1571: a rotated (and scaled) view keeps its true matches."""
+    th = np.deg2rad(8.0)
+    R = 1.03 * np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    c = np.array([480.0, 270.0])
+    a, b = _field(600, 6, lambda p: (p - c) @ R.T + c + np.array([10.0, 0.0]))
+    flags = ob.displacement_outliers([(k, k) for k in range(len(a))], a, b)
+    assert flags.sum() <= 2
+
+
+def test_displacement_needs_eight_matches():
+    a, b = _field(7, 7, lambda p: p)
+    b[0] += 500.0
+    assert not ob.displacement_outliers([(k, k) for k in range(7)], a, b).any()
