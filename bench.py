@@ -268,10 +268,62 @@ def secondary(name, precs, ctx):
         out[prec]["identical_to_fp32_gpu"] = bool(same_results(res[prec][0], ref, b.P))
         out[prec]["speedup_vs_fp32_gpu"] = round(out["fp32"]["isolated_step_ms"] /
                                                  out[prec]["isolated_step_ms"], 3)
+    ver = verified_input(b, res[precs[0]][1]) if name == "full" else None
     for _, p in res.values():
         p.close()
     out["verified_pairs"] = int(ref.found.sum())
-    return out, b, {k: v[0] for k, v in res.items()}
+    return out, b, {k: v[0] for k, v in res.items()}, ver
+
+
+def verified_input(b, plan):
+    """build_tracks' input from a plan's last run: every pair's verified matches
+    (skh_plan_verified) and each view's keypoints."""
+    import numpy as np
+
+    vl, vij = plan.verified()
+    base = plan.match_base
+    pairs = [(int(i), int(j), vij[int(base[p]):int(base[p]) + int(vl[p])].astype(np.int64))
+             for p, (i, j) in enumerate(b.sc.pairs) if vl[p]]
+    kps = [b.sc.view_kp(v).astype(np.float64) for v in range(len(b.sc.offsets) - 1)]
+    return pairs, kps
+
+
+def tracks_line(ctx, ver, reps=3):
+    """build_tracks (tracks.rs:190-508) on the full scene's verified matches (the
+    RANSAC output of its BF16 plan), both policies: the call's wall time with
+    the inputs and outputs in host memory (the device displacement filter
+    included), tracks and stats."""
+    import skylens_hip
+
+    pairs, kps = ver
+    packed = skylens_hip.TrackInput(pairs, kps)
+    out = {"workload": f"full scene verified matches: {len(pairs)} pairs, "
+                       f"{int(packed.m_off[-1])} matches, {len(kps)} views, {packed.n} keypoints"}
+    for name, pol in (("split", skylens_hip.TRACKS_SPLIT), ("drop", skylens_hip.TRACKS_DROP)):
+        cfg = skylens_hip.TrackConfig.default(policy=pol)
+        ctx.build_tracks_packed(packed, cfg)
+        t = time.perf_counter()
+        for _ in range(reps):
+            nt, st = ctx.build_tracks_packed(packed, cfg)
+        out[name] = {"ms": round((time.perf_counter() - t) / reps * 1e3, 3), **st}
+    return out, (pairs, kps)
+
+
+def tracks_cpu(ver, gpu):
+    """The oracle's build_tracks (one host thread) on the same input: time and
+    whether its tracks and stats equal the device path's."""
+    import skylens_hip
+    import oracle_bind as ob
+
+    pairs, kps = ver
+    t = time.perf_counter()
+    o_tracks, o_st = ob.build_tracks(pairs, kps, ob.track_cfg("split"))
+    ms = (time.perf_counter() - t) * 1e3
+    g_tracks, g_st = skylens_hip.default_context().build_tracks(pairs, kps)
+    return {"value": round(ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": "the full scene's verified matches through the track oracle (Split)",
+            "identical_to_gpu": bool(o_tracks == g_tracks and o_st == g_st),
+            "gpu_split_ms": gpu["split"]["ms"]}
 
 
 def single_pair_latency(ctx):
@@ -607,11 +659,14 @@ def main():
     fixed = None
     single = None
     det = None
+    tracks_in = None
     if not args.no_secondary:
         sec = {}
-        sec["full_scene"], fb, _ = secondary("full", ["bf16", "fp8"], ctxs[0])
+        sec["full_scene"], fb, _, ver = secondary("full", ["bf16", "fp8"], ctxs[0])
         del fb
-        sec["fixed_scene_fp8"], xb, xres = secondary("fixed", ["fp8"], ctxs[0])
+        sec["tracks"], tracks_in = tracks_line(ctxs[0], ver)
+        log(f"tracks: {sec['tracks']}")
+        sec["fixed_scene_fp8"], xb, xres, _ = secondary("fixed", ["fp8"], ctxs[0])
         fixed = (xb, xres["fp8"])
         sec["single_pair_latency"], single = single_pair_latency(ctxs[0])
         log(f"single pair: {sec['single_pair_latency']}")
@@ -628,6 +683,9 @@ def main():
         if det is not None:
             cpu["detector"] = detector_cpu(det)
             log(f"detector cpu: {cpu['detector']}")
+        if tracks_in is not None:
+            cpu["tracks"] = tracks_cpu(tracks_in, sec["tracks"])
+            log(f"tracks cpu: {cpu['tracks']}")
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,

@@ -170,6 +170,32 @@ class TrackStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in TRACK_STATS]
 
 
+class TrackInput:
+    """build_tracks' inputs in the C-ABI's flat layout, with output buffers:
+    pairs = [(image_a, image_b, [(fa, fb), ...]), ...] (PairMatches),
+    keypoints = [(n_i, 2) pixels per image]."""
+
+    def __init__(self, pairs, keypoints):
+        self.n_pairs, self.n_images = len(pairs), len(keypoints)
+        self.ia = np.ascontiguousarray([p[0] for p in pairs], dtype=np.uint64)
+        self.ib = np.ascontiguousarray([p[1] for p in pairs], dtype=np.uint64)
+        lens = [len(p[2]) for p in pairs]
+        self.m_off = np.zeros(len(pairs) + 1, np.uint64)
+        self.m_off[1:] = np.cumsum(lens)
+        m = (np.concatenate([np.asarray(p[2], np.uint64).reshape(-1, 2) for p in pairs])
+             if sum(lens) else np.zeros((0, 2), np.uint64))
+        self.m = np.ascontiguousarray(m, dtype=np.uint64)
+        kl = [len(k) for k in keypoints]
+        self.kp_off = np.zeros(len(keypoints) + 1, np.uint64)
+        self.kp_off[1:] = np.cumsum(kl)
+        kp = (np.concatenate([np.asarray(k, np.float64).reshape(-1, 2) for k in keypoints])
+              if sum(kl) else np.zeros((0, 2), np.float64))
+        self.kp = np.ascontiguousarray(kp, dtype=np.float64)
+        self.n = int(self.kp_off[-1])
+        self.track_off = np.zeros(self.n + 2, np.uint64)
+        self.obs = np.zeros((max(1, self.n), 2), np.uint64)
+
+
 class PairSchedule(ctypes.Structure):
     """matching.rs:47-99 (skh_pair_schedule): cross_kind 0 = Spec { range },
     1 = Formation { right_min, left_min, max, step }."""
@@ -457,33 +483,25 @@ class Context:
         pixels per image. Returns (tracks, stats): each track a list of
         (image, feature) observations (the pixel is keypoints[image][feature]),
         stats a dict of TrackStats."""
+        packed = TrackInput(pairs, keypoints)
+        nt, st = self.build_tracks_packed(packed, cfg)
+        toff, obs = packed.track_off, packed.obs
+        tracks = [[(int(a), int(b)) for a, b in obs[toff[t]:toff[t + 1]]] for t in range(nt)]
+        return tracks, st
+
+    def build_tracks_packed(self, packed: "TrackInput", cfg: Optional[TrackConfig] = None):
+        """skh_build_tracks on pre-packed inputs: (track count, stats dict); the
+        tracks are left in packed.track_off / packed.obs."""
         cfg = cfg or TrackConfig.default()
-        ia = np.ascontiguousarray([p[0] for p in pairs], dtype=np.uint64)
-        ib = np.ascontiguousarray([p[1] for p in pairs], dtype=np.uint64)
-        lens = [len(p[2]) for p in pairs]
-        m_off = np.zeros(len(pairs) + 1, np.uint64)
-        m_off[1:] = np.cumsum(lens)
-        m = (np.concatenate([np.asarray(p[2], np.uint64).reshape(-1, 2) for p in pairs])
-             if sum(lens) else np.zeros((0, 2), np.uint64))
-        m = np.ascontiguousarray(m, dtype=np.uint64)
-        kl = [len(k) for k in keypoints]
-        kp_off = np.zeros(len(keypoints) + 1, np.uint64)
-        kp_off[1:] = np.cumsum(kl)
-        kp = (np.concatenate([np.asarray(k, np.float64).reshape(-1, 2) for k in keypoints])
-              if sum(kl) else np.zeros((0, 2), np.float64))
-        kp = np.ascontiguousarray(kp, dtype=np.float64)
-        n = int(kp_off[-1])
-        toff = np.zeros(n + 2, np.uint64)
-        obs = np.zeros((max(1, n), 2), np.uint64)
+        q = packed
         nt = ctypes.c_size_t(0)
         st = TrackStats()
         self._check(self.lib.skh_build_tracks(
-            self.handle, len(pairs), _ptr(ia), _ptr(ib), _ptr(m_off), _ptr(m), len(keypoints),
-            _ptr(kp_off), _ptr(kp), ctypes.byref(cfg), _ptr(toff), toff.shape[0], _ptr(obs), n,
-            ctypes.byref(nt), ctypes.byref(st)), "skh_build_tracks")
-        tracks = [[(int(a), int(b)) for a, b in obs[toff[t]:toff[t + 1]]]
-                  for t in range(nt.value)]
-        return tracks, {k: int(getattr(st, k)) for k in TRACK_STATS}
+            self.handle, q.n_pairs, _ptr(q.ia), _ptr(q.ib), _ptr(q.m_off), _ptr(q.m),
+            q.n_images, _ptr(q.kp_off), _ptr(q.kp), ctypes.byref(cfg), _ptr(q.track_off),
+            q.track_off.shape[0], _ptr(q.obs), q.n, ctypes.byref(nt), ctypes.byref(st)),
+            "skh_build_tracks")
+        return nt.value, {k: int(getattr(st, k)) for k in TRACK_STATS}
 
     def detect_and_describe_dev(self, img_ptr: int, w: int, h: int, cfg: DetectorConfig,
                                 kp_ptr: int, desc_ptr: int, cap: int) -> int:
