@@ -534,6 +534,9 @@ def main():
     ap.add_argument("--detector-only", action="store_true",
                     help="run only the detector line (1920 x 1080 images) and print it; for "
                          "profiler runs")
+    ap.add_argument("--tracks-only", action="store_true",
+                    help="run the full scene's BF16 plan once and the build_tracks line on its "
+                         "verified matches, print it; for profiler runs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run a plain device-to-device copy of BYTES three times (the reference "
                          "kernel for HBM counter calibration); prints no JSON")
@@ -561,6 +564,17 @@ def main():
     if args.detector_only:
         out, _ = detector_line(n_ctx=args.streams)
         print(json.dumps({"detector": out}), flush=True)
+        return
+
+    if args.tracks_only:
+        ctx = skylens_hip.Context(0)
+        b = Batch("full", WORKLOADS["full"][1])
+        plan = b.plan(ctx, "bf16")
+        plan.run()
+        ver = verified_input(b, plan)
+        plan.close()
+        out, _ = tracks_line(ctx, ver)
+        print(json.dumps({"tracks": out}), flush=True)
         return
 
     S = args.streams

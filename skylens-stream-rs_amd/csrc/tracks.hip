@@ -20,7 +20,6 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <iterator>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -279,15 +278,17 @@ bool try_join(UnionFind& uf, std::vector<std::vector<size_t>>& images,
               const std::vector<size_t>& node_image, size_t u, size_t v) {
     const size_t ru = uf.find(u), rv = uf.find(v);
     if (ru == rv) return true;
-    const std::vector<size_t> su{node_image[ru]}, sv{node_image[rv]};
-    const std::vector<size_t>& iu = images[ru].empty() ? su : images[ru];
-    const std::vector<size_t>& iv = images[rv].empty() ? sv : images[rv];
-    if (common_count(iu.data(), iu.size(), iv.data(), iv.size()) > 0) return false;
-    std::vector<size_t> merged;
-    merged.reserve(iu.size() + iv.size());
-    std::merge(iu.begin(), iu.end(), iv.begin(), iv.end(), std::back_inserter(merged));
-    images[ru].clear();
-    images[rv].clear();
+    // a lone node's image list is its own image (no allocation for it)
+    const size_t su = node_image[ru], sv = node_image[rv];
+    const size_t* iu = images[ru].empty() ? &su : images[ru].data();
+    const size_t* iv = images[rv].empty() ? &sv : images[rv].data();
+    const size_t nu = images[ru].empty() ? 1 : images[ru].size();
+    const size_t nv = images[rv].empty() ? 1 : images[rv].size();
+    if (common_count(iu, nu, iv, nv) > 0) return false;
+    std::vector<size_t> merged(nu + nv);
+    std::merge(iu, iu + nu, iv, iv + nv, merged.begin());
+    images[ru] = std::vector<size_t>();
+    images[rv] = std::vector<size_t>();
     images[uf.link(ru, rv)] = std::move(merged);
     return true;
 }
