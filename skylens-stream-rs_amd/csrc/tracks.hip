@@ -21,6 +21,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -234,6 +235,25 @@ __global__ __launch_bounds__(64) void displacement_kernel(const DispPair* __rest
 
 // ------------------------------------------------------------ host: graph
 
+// f(i) for every i in [0, n), on up to 16 host threads in contiguous blocks
+// (f must only write state of its own i).
+template <class F>
+void parallel_for(size_t n, F&& f) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = std::min<size_t>(std::min<size_t>(hw, 16), (n + 255) / 256);
+    if (T <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (size_t t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (size_t i = n * t / T, e = n * (t + 1) / T; i < e; ++i) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
 struct UnionFind {  // tracks.rs:160-188
     std::vector<size_t> parent;
     explicit UnionFind(size_t n) : parent(n) {
@@ -345,7 +365,7 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
     };
     for (int round = 0; round < kVoteRounds; ++round) {
         std::vector<size_t> sup(edges.size());
-        for (size_t e = 0; e < edges.size(); ++e) sup[e] = support(edges[e].first, edges[e].second);
+        parallel_for(edges.size(), [&](size_t e) { sup[e] = support(edges[e].first, edges[e].second); });
         std::vector<char> cut(edges.size(), 0);
         size_t cuts = 0;
         for (size_t x = 0; x < n; ++x) {
@@ -398,9 +418,10 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
         fe.resize(w);
     }
     // (support, u, v): support descending, then (u, v) ascending
-    std::vector<std::array<size_t, 3>> order;
-    order.reserve(edges.size());
-    for (const Edge& e : edges) order.push_back({support(e.first, e.second), e.first, e.second});
+    std::vector<std::array<size_t, 3>> order(edges.size());
+    parallel_for(edges.size(), [&](size_t e) {
+        order[e] = {support(edges[e].first, edges[e].second), edges[e].first, edges[e].second};
+    });
     std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
         if (a[0] != b[0]) return a[0] > b[0];
         return std::tie(a[1], a[2]) < std::tie(b[1], b[2]);
@@ -471,20 +492,25 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
 int Ctx::displacement_flags(const std::vector<std::vector<std::array<uint64_t, 2>>>& valid,
                             const std::vector<std::pair<const double*, const double*>>& kps,
                             std::vector<std::vector<uint8_t>>& flags) {
-    std::vector<DispPair> pairs;
-    std::vector<uint32_t> point_pair;
-    std::vector<double4> pts;
-    std::vector<CellKey> keys;
-    std::vector<std::pair<size_t, uint32_t>> where;  // pair index, first point
+    // per pair: points, extent, grid cell and sorted grid keys (independent per pair)
+    struct Prep {
+        bool use = false;
+        double cell = 0.0, tol = 0.0;
+        std::vector<double4> q;
+        std::vector<CellKey> kk;
+    };
+    std::vector<Prep> prep(valid.size());
     flags.assign(valid.size(), {});
-    for (size_t p = 0; p < valid.size(); ++p) {
+    parallel_for(valid.size(), [&](size_t p) {
         const auto& m = valid[p];
         const size_t n = m.size();
         flags[p].assign(n, 0);
-        if (n < 8) continue;  // tracks.rs:527-529
+        if (n < 8) return;  // tracks.rs:527-529
         const double* ka = kps[p].first;
         const double* kb = kps[p].second;
-        std::vector<double4> q(n);
+        Prep& P = prep[p];
+        std::vector<double4>& q = P.q;
+        q.resize(n);
         double max_x = 0.0, max_y = 0.0, min_x = DBL_MAX, min_y = DBL_MAX;
         for (size_t k = 0; k < n; ++k) {
             const uint64_t fa = m[k][0], fb = m[k][1];
@@ -496,7 +522,7 @@ int Ctx::displacement_flags(const std::vector<std::vector<std::array<uint64_t, 2
             min_y = std::fmin(min_y, q[k].y);
         }
         const double size = std::fmax(max_x / kRefW, max_y / kRefH);
-        if (size <= 0.0 || !std::isfinite(size)) continue;
+        if (size <= 0.0 || !std::isfinite(size)) return;
         const double tol = kDispTolerance * size;
         // 5 % / 95 % order statistics of each coordinate (tracks.rs:552-558)
         auto q_range = [&](bool y) {
@@ -513,24 +539,40 @@ int Ctx::displacement_flags(const std::vector<std::vector<std::array<uint64_t, 2
         const double area = full > 16.0 * quant ? quant : full;
         const double cell =
             std::fmax(std::fmax(std::sqrt(area * 8.0 / (double)n), size * 4.0), 1e-9);
-        const uint32_t p0 = (uint32_t)pts.size();
-        std::vector<CellKey> kk(n);
+        P.kk.resize(n);
         for (size_t k = 0; k < n; ++k)
-            kk[k] = {sat_i64(std::floor(q[k].x / cell)), sat_i64(std::floor(q[k].y / cell)),
-                     (uint32_t)k, 0u};
-        std::sort(kk.begin(), kk.end(), [](const CellKey& a, const CellKey& b) {
+            P.kk[k] = {sat_i64(std::floor(q[k].x / cell)), sat_i64(std::floor(q[k].y / cell)),
+                       (uint32_t)k, 0u};
+        std::sort(P.kk.begin(), P.kk.end(), [](const CellKey& a, const CellKey& b) {
             return std::tie(a.kx, a.ky, a.i) < std::tie(b.kx, b.ky, b.i);
         });
-        pairs.push_back({p0, (uint32_t)n, cell, tol});
-        for (size_t k = 0; k < n; ++k) {
-            pts.push_back(q[k]);
-            keys.push_back(kk[k]);
-            point_pair.push_back((uint32_t)(pairs.size() - 1));
+        P.cell = cell;
+        P.tol = tol;
+        P.use = true;
+    });
+    std::vector<DispPair> pairs;
+    std::vector<std::pair<size_t, uint32_t>> where;  // pair index, first point
+    size_t total = 0;
+    for (size_t p = 0; p < prep.size(); ++p)
+        if (prep[p].use) {
+            if (total + prep[p].q.size() >= 0xffffffffull)
+                return fail(SKH_EINVAL, "skh_build_tracks: too many matches");
+            pairs.push_back({(uint32_t)total, (uint32_t)prep[p].q.size(), prep[p].cell,
+                             prep[p].tol});
+            where.emplace_back(p, (uint32_t)total);
+            total += prep[p].q.size();
         }
-        where.emplace_back(p, p0);
-    }
+    std::vector<uint32_t> point_pair(total);
+    std::vector<double4> pts(total);
+    std::vector<CellKey> keys(total);
+    parallel_for(pairs.size(), [&](size_t d) {
+        const Prep& P = prep[where[d].first];
+        const size_t p0 = pairs[d].p0, n = P.q.size();
+        std::copy(P.q.begin(), P.q.end(), pts.begin() + p0);
+        std::copy(P.kk.begin(), P.kk.end(), keys.begin() + p0);
+        std::fill(point_pair.begin() + p0, point_pair.begin() + p0 + n, (uint32_t)d);
+    });
     if (pts.empty()) return SKH_OK;
-    if (pts.size() >= 0xffffffffull) return fail(SKH_EINVAL, "skh_build_tracks: too many matches");
     const size_t np = pts.size();
     int rc;
     if ((rc = ensure(t_pairs, pairs.size() * sizeof(DispPair), "track pairs")) ||
@@ -604,13 +646,15 @@ extern "C" int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* im
     // tracks.rs:207-230: valid matches in the lower image's direction, sorted, deduplicated
     std::vector<std::vector<std::array<uint64_t, 2>>> valid(n_pairs);
     std::vector<std::pair<const double*, const double*>> kps(n_pairs, {nullptr, nullptr});
-    for (size_t p = 0; p < n_pairs; ++p) {
+    std::vector<size_t> invalid(n_pairs, 0);
+    parallel_for(n_pairs, [&](size_t p) {
         const uint64_t ia = image_a[p], ib = image_b[p];
         auto& v = valid[p];
+        v.reserve(m_off[p + 1] - m_off[p]);
         for (uint64_t k = m_off[p]; k < m_off[p + 1]; ++k) {
             const uint64_t fa = m_ab[2 * k], fb = m_ab[2 * k + 1];
             if (ia == ib || ia >= n_images || ib >= n_images || fa >= n_kp(ia) || fb >= n_kp(ib)) {
-                st.invalid_matches++;
+                invalid[p]++;
                 continue;
             }
             v.push_back(ia > ib ? std::array<uint64_t, 2>{fb, fa} : std::array<uint64_t, 2>{fa, fb});
@@ -621,7 +665,8 @@ extern "C" int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* im
             const uint64_t lo = std::min(ia, ib), hi = std::max(ia, ib);
             kps[p] = {kp + 2 * kp_off[lo], kp + 2 * kp_off[hi]};
         }
-    }
+    });
+    for (size_t p = 0; p < n_pairs; ++p) st.invalid_matches += invalid[p];
     std::vector<std::vector<uint8_t>> flags;
     if (cfg->policy == SKH_TRACKS_SPLIT) {
         const int rc = ctx->displacement_flags(valid, kps, flags);
