@@ -305,12 +305,47 @@ bool try_join(UnionFind& uf, std::vector<std::vector<size_t>>& images,
     const size_t nu = images[ru].empty() ? 1 : images[ru].size();
     const size_t nv = images[rv].empty() ? 1 : images[rv].size();
     if (common_count(iu, nu, iv, nv) > 0) return false;
-    std::vector<size_t> merged(nu + nv);
+    // merged into a reused buffer, then into whichever of the two lists has the
+    // larger allocation (kept by the new root); the other root's list is freed
+    static thread_local std::vector<size_t> merged;
+    merged.resize(nu + nv);
     std::merge(iu, iu + nu, iv, iv + nv, merged.begin());
-    images[ru] = std::vector<size_t>();
-    images[rv] = std::vector<size_t>();
-    images[uf.link(ru, rv)] = std::move(merged);
+    const size_t r = uf.link(ru, rv), o = r == ru ? rv : ru;
+    if (images[o].capacity() > images[r].capacity()) std::swap(images[r], images[o]);
+    images[o] = std::vector<size_t>();
+    if (images[r].capacity() < nu + nv) images[r].reserve(std::max(nu + nv, 2 * images[r].capacity()));
+    images[r].assign(merged.begin(), merged.end());
     return true;
+}
+
+// Stable sort of v by key(x) descending (a counting sort: the keys are counts
+// bounded by the input size).
+template <class T, class K>
+void stable_desc(std::vector<T>& v, K key) {
+    size_t top = 0;
+    for (const T& x : v) top = std::max(top, (size_t)key(x));
+    std::vector<size_t> at(top + 2, 0);  // at[top - k]: first slot of key k
+    for (const T& x : v) at[top - key(x) + 1]++;
+    for (size_t k = 0; k <= top; ++k) at[k + 1] += at[k];
+    std::vector<T> out(v.size());
+    for (const T& x : v) out[at[top - key(x)]++] = x;
+    v.swap(out);
+}
+
+// Ascending order of (first, second) pairs whose first is below n: a counting
+// sort on first, then each first's run by second.
+void sort_pairs(std::vector<std::pair<size_t, size_t>>& v, size_t n) {
+    std::vector<size_t> at(n + 1, 0);
+    for (const auto& e : v) at[e.first + 1]++;
+    for (size_t x = 0; x < n; ++x) at[x + 1] += at[x];
+    std::vector<std::pair<size_t, size_t>> out(v.size());
+    {
+        std::vector<size_t> fill(at.begin(), at.end() - 1);
+        for (const auto& e : v) out[fill[e.first]++] = e;
+    }
+    for (size_t x = 0; x < n; ++x)
+        if (at[x + 1] - at[x] > 1) std::sort(out.begin() + at[x], out.begin() + at[x + 1]);
+    v.swap(out);
 }
 
 struct CompPair {
@@ -326,7 +361,7 @@ std::vector<CompPair> component_pairs(const std::vector<std::array<size_t, 3>>& 
         const size_t ru = uf.find(e[1]), rv = uf.find(e[2]);
         if (ru != rv) groups.emplace_back(std::min(ru, rv), std::max(ru, rv));
     }
-    std::sort(groups.begin(), groups.end());
+    sort_pairs(groups, uf.parent.size());
     std::vector<CompPair> out;
     for (const auto& g : groups) {
         if (!out.empty() && out.back().a == g.first && out.back().b == g.second) out.back().count++;
@@ -417,15 +452,13 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
         flat.resize(w);
         fe.resize(w);
     }
-    // (support, u, v): support descending, then (u, v) ascending
+    // (support, u, v): support descending, then (u, v) ascending (the edges are
+    // in ascending (u, v) order and the sort on the support is stable)
     std::vector<std::array<size_t, 3>> order(edges.size());
     parallel_for(edges.size(), [&](size_t e) {
         order[e] = {support(edges[e].first, edges[e].second), edges[e].first, edges[e].second};
     });
-    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
-        if (a[0] != b[0]) return a[0] > b[0];
-        return std::tie(a[1], a[2]) < std::tie(b[1], b[2]);
-    });
+    stable_desc(order, [](const std::array<size_t, 3>& x) { return x[0]; });
     std::vector<std::vector<size_t>> images(n);
     size_t split = order.size();
     for (size_t e = 0; e < order.size(); ++e)
@@ -458,11 +491,10 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
         for (CompPair& c : counted)
             c.sup = common_count(cflat.data() + cstart[c.a], cstart[c.a + 1] - cstart[c.a],
                                  cflat.data() + cstart[c.b], cstart[c.b + 1] - cstart[c.b]);
-        std::sort(counted.begin(), counted.end(), [](const CompPair& x, const CompPair& y) {
-            if (x.count != y.count) return x.count > y.count;
-            if (x.sup != y.sup) return x.sup > y.sup;
-            return std::tie(x.a, x.b) < std::tie(y.a, y.b);
-        });
+        // count descending, then support descending, then (a, b) ascending
+        // (component_pairs lists them by (a, b); both sorts are stable)
+        stable_desc(counted, [](const CompPair& x) { return x.sup; });
+        stable_desc(counted, [](const CompPair& x) { return x.count; });
         const bool last = round == 8;
         size_t merged = 0;
         for (const CompPair& c : counted) {
@@ -474,10 +506,7 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
         if (last) break;
         if (merged == 0) {
             std::vector<CompPair> rest = component_pairs(zero, uf);
-            std::sort(rest.begin(), rest.end(), [](const CompPair& x, const CompPair& y) {
-                if (x.count != y.count) return x.count > y.count;
-                return std::tie(x.a, x.b) < std::tie(y.a, y.b);
-            });
+            stable_desc(rest, [](const CompPair& x) { return x.count; });
             for (const CompPair& c : rest)
                 if (!try_join(uf, images, node_image, c.a, c.b)) st.conflicts += c.count;
             break;
@@ -685,7 +714,7 @@ extern "C" int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* im
             edges.emplace_back(std::min(u, v), std::max(u, v));
         }
     }
-    std::sort(edges.begin(), edges.end());
+    sort_pairs(edges, n);
     edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
     st.edges = edges.size();
     UnionFind uf(n);
@@ -697,15 +726,25 @@ extern "C" int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* im
     } else {
         split_policy(edges, n, node_image, uf, st);
     }
-    // components of the remaining edges' end points (tracks.rs:467-508)
+    // components of the remaining edges' end points (tracks.rs:467-508): the
+    // distinct (root, node) pairs in ascending order, by a counting sort on
+    // the root over the nodes taken in ascending order
     std::vector<std::pair<size_t, size_t>> members;  // (root, node)
-    members.reserve(2 * edges.size());
-    for (const Edge& e : edges) {
-        members.emplace_back(uf.find(e.first), e.first);
-        members.emplace_back(uf.find(e.second), e.second);
+    {
+        std::vector<size_t> root(n, SIZE_MAX), at(n + 1, 0);
+        for (const Edge& e : edges) root[e.first] = root[e.second] = 0;
+        size_t used = 0;
+        for (size_t x = 0; x < n; ++x)
+            if (root[x] != SIZE_MAX) {
+                root[x] = uf.find(x);
+                at[root[x] + 1]++;
+                ++used;
+            }
+        for (size_t r = 0; r < n; ++r) at[r + 1] += at[r];
+        members.resize(used);
+        for (size_t x = 0; x < n; ++x)
+            if (root[x] != SIZE_MAX) members[at[root[x]]++] = {root[x], x};
     }
-    std::sort(members.begin(), members.end());
-    members.erase(std::unique(members.begin(), members.end()), members.end());
     const size_t min_len = std::max<uint64_t>(cfg->min_length, 2);
     struct Tr {
         size_t first, len;
