@@ -249,6 +249,11 @@ long orc_build_tracks(size_t n_pairs, const uint64_t* image_a, const uint64_t* i
  * (fa, fb) of one pair: out[k] = 1 for a flagged match. */
 void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
                                double floor_, double cap_, uint8_t* out);
+/* Tests only: the flags and each match's fitted offset r and threshold (NAN where no fit ran). */
+void orc_displacement_offsets(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
+                              double floor_, double cap_, uint8_t* out, double* offs);
+/* Test knob: the order of the displacement filter's 24 kept candidates (0-3). */
+void orc_set_displacement_order(int mode);
 /* select_tracks (tracks.rs:786-799): the selected tracks' indices in output
  * order; returns their count (-1 on allocation failure). */
 long orc_select_tracks(const uint64_t* track_off, const uint64_t* obs, size_t n_tracks,

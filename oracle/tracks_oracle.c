@@ -13,7 +13,9 @@
  *     library leaves unspecified; here the candidates are ordered by
  *     (squared distance, index) and the first 24 kept. The fit's sums then
  *     run in that order, so a flag can differ from the reference only when a
- *     point's offset lies within rounding of its threshold.
+ *     point's offset lies within rounding of its threshold. A test knob
+ *     (orc_set_displacement_order) reorders the 24 so that the tests can
+ *     check that the flags do not depend on the order.
  *   - nalgebra 0.33.3 (Cargo.lock) supplies the 3x3 determinant and inverse
  *     of fit_affine; they are restated from its published cofactor formulas
  *     (determinant: first-row expansion with the minors m22 m33 - m32 m23,
@@ -119,9 +121,47 @@ static int fit_affine(const double (*nb)[4], const uint8_t* keep, size_t n, doub
     return 1;
 }
 
-/* displacement_outliers_with_floor (tracks.rs:519-670). m: n (fa, fb). */
-void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
-                               double floor_, double cap_, uint8_t* out) {
+/* The order in which the 24 kept candidates enter the fits (test knob): 0
+ * (squared distance, index); 1 the reverse; 2 ascending index; 3 a seeded
+ * shuffle. The reference leaves this order unspecified (tracks.rs:612-615),
+ * so the tests check that the flags do not depend on it. */
+static int g_disp_order = 0;
+void orc_set_displacement_order(int mode) { g_disp_order = mode; }
+
+static int cmp_cand_index(const void* x, const void* y) {
+    const cand_t* a = x;
+    const cand_t* b = y;
+    return a->j < b->j ? -1 : a->j > b->j;
+}
+
+static void permute_candidates(cand_t* c, size_t nc, uint64_t seed) {
+    if (g_disp_order == 1) {
+        for (size_t a = 0, b = nc - 1; a < b; ++a, --b) {
+            const cand_t t = c[a];
+            c[a] = c[b];
+            c[b] = t;
+        }
+    } else if (g_disp_order == 2) {
+        qsort(c, nc, sizeof(cand_t), cmp_cand_index);
+    } else if (g_disp_order == 3) {
+        uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+        for (size_t k = nc - 1; k > 0; --k) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            const size_t r = (size_t)(x % (k + 1));
+            const cand_t t = c[k];
+            c[k] = c[r];
+            c[r] = t;
+        }
+    }
+}
+
+/* displacement_outliers_with_floor (tracks.rs:519-670). m: n (fa, fb).
+ * offs (tests only, may be NULL): each match's fitted offset r and its
+ * threshold, NAN where no fit ran. */
+static void displacement_impl(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
+                              double floor_, double cap_, uint8_t* out, double* offs) {
     memset(out, 0, n);
     if (n < 8) return;
     double* px = malloc(n * sizeof(double));
@@ -188,6 +228,7 @@ void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, c
         if (nc > DISP_K) {
             qsort(cand, nc, sizeof(cand_t), cmp_cand);
             nc = DISP_K;
+            if (g_disp_order) permute_candidates(cand, nc, i);
         }
         double nb[DISP_K][4], res[DISP_K], sorted[DISP_K], f[6] = {0};
         uint8_t keep[DISP_K];
@@ -230,6 +271,10 @@ void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, c
         if (lim < lo_l) lim = lo_l;
         if (lim > hi_l) lim = hi_l;
         out[i] = r > lim;
+        if (offs) {
+            offs[2 * i] = r;
+            offs[2 * i + 1] = lim;
+        }
     }
 done:
     free(px);
@@ -239,6 +284,17 @@ done:
     free(tmp);
     free(idx);
     free(cand);
+}
+
+void orc_displacement_outliers(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
+                               double floor_, double cap_, uint8_t* out) {
+    displacement_impl(m, n, kpa, kpb, floor_, cap_, out, NULL);
+}
+
+void orc_displacement_offsets(const uint64_t* m, size_t n, const double* kpa, const double* kpb,
+                              double floor_, double cap_, uint8_t* out, double* offs) {
+    for (size_t i = 0; i < 2 * n; ++i) offs[i] = NAN;
+    displacement_impl(m, n, kpa, kpb, floor_, cap_, out, offs);
 }
 
 /* ----------------------------------------------------------- union-find */

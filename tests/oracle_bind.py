@@ -611,6 +611,9 @@ _SIGS.update({
     "orc_displacement_outliers": (None, [_vp, _sz, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                          _vp]),
     "orc_select_tracks": (ctypes.c_long, [_vp, _vp, _sz, _sz, _vp]),
+    "orc_set_displacement_order": (None, [ctypes.c_int]),
+    "orc_displacement_offsets": (None, [_vp, _sz, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                        _vp, _vp]),
 })
 
 
@@ -660,6 +663,35 @@ def displacement_outliers(matches, kp_a, kp_b, floor=0.25, cap=3.0):
     out = np.zeros(max(1, m.shape[0]), np.uint8)
     lib().orc_displacement_outliers(P(m), m.shape[0], P(a), P(b), floor, cap, P(out))
     return out[:m.shape[0]].astype(bool)
+
+
+def displacement_offsets(matches, kp_a, kp_b, floor=0.25, cap=3.0):
+    """(flags, r, threshold) per match; r and threshold NAN where no fit ran."""
+    m = np.ascontiguousarray(np.asarray(matches, np.uint64).reshape(-1, 2))
+    a = np.ascontiguousarray(np.asarray(kp_a, np.float64).reshape(-1, 2))
+    b = np.ascontiguousarray(np.asarray(kp_b, np.float64).reshape(-1, 2))
+    out = np.zeros(max(1, m.shape[0]), np.uint8)
+    offs = np.zeros((max(1, m.shape[0]), 2))
+    lib().orc_displacement_offsets(P(m), m.shape[0], P(a), P(b), floor, cap, P(out), P(offs))
+    n = m.shape[0]
+    return out[:n].astype(bool), offs[:n, 0], offs[:n, 1]
+
+
+class displacement_order:
+    """Context manager: the oracle's displacement filter takes its 24 kept
+    candidates in order `mode` (0 by (squared distance, index), 1 reversed,
+    2 by index, 3 a seeded shuffle) inside the block."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        lib().orc_set_displacement_order(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_displacement_order(0)
+        return False
 
 
 def select_tracks(tracks, max_tracks):

@@ -257,3 +257,49 @@ def test_displacement_needs_eight_matches():
     a, b = _field(7, 7, lambda p: p)
     b[0] += 500.0
     assert not ob.displacement_outliers([(k, k) for k in range(7)], a, b).any()
+
+
+@pytest.mark.parametrize("seed", [3, 5, 8])
+def test_flags_do_not_depend_on_candidate_order(seed):
+    """The reference takes the 24 nearest candidates with select_nth_unstable_by
+    (tracks.rs:612-615), which leaves their order unspecified, and the fits sum
+    them in that order. Under four different orders of the same 24 (by distance,
+    reversed, by index, shuffled) every match of these scenes gets the same flag
+    and the builder the same tracks and stats, so the order fixed here decides
+    nothing on them."""
+    s = workloads.track_scene(seed=seed, images=10, points=2000, keep=0.8,
+                              outlier_per_mille=30, temporal=3)
+    base = ob.build_tracks(_pairs(s), s.keypoints, ob.track_cfg("split"))
+    assert base[1]["inconsistent"] > 0
+    flags = {}
+    for mode in (0, 1, 2, 3):
+        with ob.displacement_order(mode):
+            assert ob.build_tracks(_pairs(s), s.keypoints, ob.track_cfg("split")) == base
+            flags[mode] = [ob.displacement_outliers(m, s.keypoints[a], s.keypoints[b])
+                           for a, b, m in s.pairs if len(m) > 24]
+    assert sum(len(f) for f in flags[0]) > 1000
+    for mode in (1, 2, 3):
+        assert all((x == y).all() for x, y in zip(flags[0], flags[mode]))
+
+
+
+def test_candidate_order_moves_only_rounding():
+    """The order reaches the fits (the knob is live: some offsets change) and moves
+    each match's fitted offset r by rounding only (relative 1e-9 of the cell-sized
+    displacements), far from the thresholds the flags compare it with."""
+    s = workloads.track_scene(seed=5, images=8, points=2000, keep=0.8, outlier_per_mille=30)
+    changed, worst, margin = 0, 0.0, np.inf
+    for a, b, m in s.pairs:
+        if len(m) <= 24:
+            continue
+        f0, r0, t0 = ob.displacement_offsets(m, s.keypoints[a], s.keypoints[b])
+        ok = np.isfinite(r0)
+        margin = min(margin, float(np.min(np.abs(r0[ok] - t0[ok]) / t0[ok])))
+        for mode in (1, 2, 3):
+            with ob.displacement_order(mode):
+                f, r, t = ob.displacement_offsets(m, s.keypoints[a], s.keypoints[b])
+            assert (f == f0).all() and (np.isfinite(r) == ok).all()
+            changed += int(np.sum(r[ok] != r0[ok]))
+            worst = max(worst, float(np.max(np.abs(r[ok] - r0[ok]) / np.maximum(t0[ok], 1.0))))
+    assert changed > 0
+    assert worst < 1e-9 and margin > 1e3 * worst
