@@ -112,13 +112,15 @@ def test_capacity_and_argument_errors(ctx):
     assert t2 == t and st2 == st
 
 
-def test_plan_verified_matches_feed_tracks(ctx):
+@pytest.mark.parametrize("positions", [10, 80], ids=["10_positions", "full_scene"])
+def test_plan_verified_matches_feed_tracks(ctx, positions):
     """The plan's verified lists (matches kept by RANSAC, tracks.rs:54-69) and
     the batch's keypoints feed build_tracks directly, as the production loop
-    does (pipeline.rs:504-508); device result = oracle result."""
+    does (pipeline.rs:504-508); device result = oracle result. At 80 positions
+    this is the full scene (3663 pairs, about 349k verified matches)."""
     import torch
 
-    sc = workloads.full_scene(seed=1, positions=10)
+    sc = workloads.full_scene(seed=1, positions=positions)
     desc = torch.from_numpy(sc.desc).cuda()
     kp = torch.from_numpy(np.ascontiguousarray(sc.kp)).cuda()
     plan = skylens_hip.Plan(ctx, sc.offsets, desc.data_ptr(), kp.data_ptr(), sc.pairs, 0.8, True,
