@@ -304,7 +304,13 @@ bool try_join(UnionFind& uf, std::vector<std::vector<size_t>>& images,
     const size_t* iv = images[rv].empty() ? &sv : images[rv].data();
     const size_t nu = images[ru].empty() ? 1 : images[ru].size();
     const size_t nv = images[rv].empty() ? 1 : images[rv].size();
-    if (common_count(iu, nu, iv, nv) > 0) return false;
+    // any common image refuses the join (tracks.rs:715 counts them; only zero
+    // or not is used)
+    for (size_t i = 0, j = 0; i < nu && j < nv;) {
+        if (iu[i] < iv[j]) ++i;
+        else if (iv[j] < iu[i]) ++j;
+        else return false;
+    }
     // merged into a reused buffer, then into whichever of the two lists has the
     // larger allocation (kept by the new root); the other root's list is freed
     static thread_local std::vector<size_t> merged;
@@ -469,28 +475,43 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
     for (size_t e = 0; e < split; ++e)
         if (!try_join(uf, images, node_image, order[e][1], order[e][2])) st.conflicts++;
     const std::vector<std::array<size_t, 3>> zero(order.begin() + split, order.end());
-    std::vector<size_t> cstart(n + 1);
+    // dense[r]: the round's index of component r (SIZE_MAX when absent), so that
+    // a round's work is proportional to its component pairs, not to n
+    std::vector<size_t> dense(n, SIZE_MAX), ids;
     for (int round = 0; round <= 8; ++round) {
         std::vector<CompPair> counted = component_pairs(zero, uf);
         if (counted.empty()) break;
-        // component graph CSR, each list sorted; support = common neighbour components
-        std::fill(cstart.begin(), cstart.end(), 0);
+        // component graph CSR, each list sorted; support = common neighbour
+        // components (the lists hold dense ids: any one order of both lists
+        // gives the same count)
+        ids.clear();
+        for (const CompPair& c : counted)
+            for (const size_t r : {c.a, c.b})
+                if (dense[r] == SIZE_MAX) {
+                    dense[r] = ids.size();
+                    ids.push_back(r);
+                }
+        const size_t m = ids.size();
+        std::vector<size_t> cstart(m + 1, 0);
         for (const CompPair& c : counted) {
-            cstart[c.a + 1]++;
-            cstart[c.b + 1]++;
+            cstart[dense[c.a] + 1]++;
+            cstart[dense[c.b] + 1]++;
         }
-        for (size_t i = 0; i < n; ++i) cstart[i + 1] += cstart[i];
+        for (size_t i = 0; i < m; ++i) cstart[i + 1] += cstart[i];
         std::vector<size_t> cfill(cstart.begin(), cstart.end() - 1), cflat(2 * counted.size());
         for (const CompPair& c : counted) {
-            cflat[cfill[c.a]++] = c.b;
-            cflat[cfill[c.b]++] = c.a;
+            cflat[cfill[dense[c.a]]++] = dense[c.b];
+            cflat[cfill[dense[c.b]]++] = dense[c.a];
         }
-        for (size_t i = 0; i < n; ++i)
+        for (size_t i = 0; i < m; ++i)
             if (cstart[i + 1] - cstart[i] > 1)
                 std::sort(cflat.begin() + cstart[i], cflat.begin() + cstart[i + 1]);
-        for (CompPair& c : counted)
-            c.sup = common_count(cflat.data() + cstart[c.a], cstart[c.a + 1] - cstart[c.a],
-                                 cflat.data() + cstart[c.b], cstart[c.b + 1] - cstart[c.b]);
+        for (CompPair& c : counted) {
+            const size_t a = dense[c.a], b = dense[c.b];
+            c.sup = common_count(cflat.data() + cstart[a], cstart[a + 1] - cstart[a],
+                                 cflat.data() + cstart[b], cstart[b + 1] - cstart[b]);
+        }
+        for (const size_t r : ids) dense[r] = SIZE_MAX;
         // count descending, then support descending, then (a, b) ascending
         // (component_pairs lists them by (a, b); both sorts are stable)
         stable_desc(counted, [](const CompPair& x) { return x.sup; });
