@@ -291,19 +291,25 @@ size_t common_count(const size_t* a, size_t na, const size_t* b, size_t nb) {
     return c;
 }
 
+// The sorted image lists of the union-find's roots, as regions of one arena:
+// root r's list is arena[off[r], off[r] + len[r]) inside a region of cap[r]
+// entries; len[r] = 0 for a lone node, whose list is its own image.
+struct ImageLists {
+    std::vector<size_t> off, len, cap;
+    std::vector<uint32_t> arena;  // image ids (below 2^32, checked at entry)
+    explicit ImageLists(size_t n) : off(n, 0), len(n, 0), cap(n, 0) {}
+};
+
 // Joins the components of u and v if their image sets are disjoint
-// (tracks.rs:692-724); images[r]: sorted images of root r, empty for a lone
-// node.
-bool try_join(UnionFind& uf, std::vector<std::vector<size_t>>& images,
-              const std::vector<size_t>& node_image, size_t u, size_t v) {
+// (tracks.rs:692-724).
+bool try_join(UnionFind& uf, ImageLists& L, const std::vector<size_t>& node_image, size_t u,
+              size_t v) {
     const size_t ru = uf.find(u), rv = uf.find(v);
     if (ru == rv) return true;
-    // a lone node's image list is its own image (no allocation for it)
-    const size_t su = node_image[ru], sv = node_image[rv];
-    const size_t* iu = images[ru].empty() ? &su : images[ru].data();
-    const size_t* iv = images[rv].empty() ? &sv : images[rv].data();
-    const size_t nu = images[ru].empty() ? 1 : images[ru].size();
-    const size_t nv = images[rv].empty() ? 1 : images[rv].size();
+    const uint32_t su = (uint32_t)node_image[ru], sv = (uint32_t)node_image[rv];
+    const uint32_t* iu = L.len[ru] ? L.arena.data() + L.off[ru] : &su;
+    const uint32_t* iv = L.len[rv] ? L.arena.data() + L.off[rv] : &sv;
+    const size_t nu = L.len[ru] ? L.len[ru] : 1, nv = L.len[rv] ? L.len[rv] : 1;
     // any common image refuses the join (tracks.rs:715 counts them; only zero
     // or not is used)
     for (size_t i = 0, j = 0; i < nu && j < nv;) {
@@ -311,16 +317,25 @@ bool try_join(UnionFind& uf, std::vector<std::vector<size_t>>& images,
         else if (iv[j] < iu[i]) ++j;
         else return false;
     }
-    // merged into a reused buffer, then into whichever of the two lists has the
-    // larger allocation (kept by the new root); the other root's list is freed
-    static thread_local std::vector<size_t> merged;
+    // merged into a reused buffer, then into the larger of the two regions when
+    // it holds the result, else into a new region (twice the size) at the end
+    // of the arena; the new root keeps it
+    static thread_local std::vector<uint32_t> merged;
     merged.resize(nu + nv);
     std::merge(iu, iu + nu, iv, iv + nv, merged.begin());
     const size_t r = uf.link(ru, rv), o = r == ru ? rv : ru;
-    if (images[o].capacity() > images[r].capacity()) std::swap(images[r], images[o]);
-    images[o] = std::vector<size_t>();
-    if (images[r].capacity() < nu + nv) images[r].reserve(std::max(nu + nv, 2 * images[r].capacity()));
-    images[r].assign(merged.begin(), merged.end());
+    const size_t big = L.cap[o] > L.cap[r] ? o : r;
+    size_t off = L.off[big], cap = L.cap[big];
+    if (cap < nu + nv) {
+        off = L.arena.size();
+        cap = std::max<size_t>(2 * (nu + nv), 4);
+        L.arena.resize(L.arena.size() + cap);
+    }
+    std::copy(merged.begin(), merged.end(), L.arena.begin() + off);
+    L.off[r] = off;
+    L.cap[r] = cap;
+    L.len[r] = nu + nv;
+    L.len[o] = L.cap[o] = 0;
     return true;
 }
 
@@ -465,7 +480,7 @@ void split_policy(std::vector<Edge>& edges, size_t n, const std::vector<size_t>&
         order[e] = {support(edges[e].first, edges[e].second), edges[e].first, edges[e].second};
     });
     stable_desc(order, [](const std::array<size_t, 3>& x) { return x[0]; });
-    std::vector<std::vector<size_t>> images(n);
+    ImageLists images(n);
     size_t split = order.size();
     for (size_t e = 0; e < order.size(); ++e)
         if (order[e][0] == 0) {
@@ -680,6 +695,7 @@ extern "C" int skh_build_tracks(skh_ctx* ctx, size_t n_pairs, const uint64_t* im
         (n_pairs && (!image_a || !image_b || !m_off)) ||
         (cfg->policy != SKH_TRACKS_DROP && cfg->policy != SKH_TRACKS_SPLIT))
         return ctx->fail(SKH_EINVAL, "skh_build_tracks: invalid argument");
+    if (n_images > UINT32_MAX) return ctx->fail(SKH_EINVAL, "skh_build_tracks: n_images");
     const size_t n = (size_t)kp_off[n_images];
     if ((n && !kp) || (n_pairs && m_off[n_pairs] && !m_ab))
         return ctx->fail(SKH_EINVAL, "skh_build_tracks: invalid argument");
